@@ -1,0 +1,715 @@
+// C-ABI of libshellgpu (declared in include/shell_gpu.h): context, device
+// memory, prime tables, and the entry points of the RNS hot path.  Argument
+// checks mirror the reference's InvalidArgument paths; on error nothing is
+// launched and outputs are untouched.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "rlwe/montgomery.h"
+#include "rlwe/ntt_parameters.h"
+#include "shell_gpu.h"
+#include "shg_device.h"
+#include "shg_kernels.h"
+
+using shg::LimbPrimes;
+using shg::PrimeDev;
+using ModInt = rlwe::MontgomeryInt<uint64_t>;
+using Params = rlwe::MontgomeryIntParams<uint64_t>;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return code;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+  return fail(SHG_E_DEVICE, "%s: %s", what, hipGetErrorString(e));
+}
+
+struct PrimeRec {
+  uint64_t q = 0;
+  int log_n = 0;
+  std::unique_ptr<const Params> mp;
+  rlwe::NttParameters<ModInt> ntt;
+  void* d_tables = nullptr;  // fw, iw (2n u64 each), fwd, iwd (2n double each)
+  PrimeDev dev{};
+};
+
+}  // namespace
+
+struct shg_context {
+  int device = 0;
+  hipStream_t own_stream = nullptr;
+  hipStream_t stream = nullptr;
+  std::vector<std::unique_ptr<PrimeRec>> primes;
+  std::map<std::pair<uint64_t, int>, int> prime_index;
+  PrimeDev* d_primes = nullptr;
+  int d_capacity = 0;
+  void* scratch = nullptr;
+  size_t scratch_size = 0;
+  std::map<std::pair<int, int>, int> tuned;  // (log_n, bucket) -> variant
+  std::mutex mu;
+};
+
+namespace {
+
+int ensure_scratch(shg_context* ctx, size_t bytes) {
+  if (bytes <= ctx->scratch_size) return SHG_OK;
+  if (ctx->scratch) {
+    hipStreamSynchronize(ctx->stream);
+    hipFree(ctx->scratch);
+    ctx->scratch = nullptr;
+    ctx->scratch_size = 0;
+  }
+  hipError_t e = hipMalloc(&ctx->scratch, bytes);
+  if (e != hipSuccess) return fail(SHG_E_NO_MEMORY, "scratch allocation of %zu bytes failed", bytes);
+  ctx->scratch_size = bytes;
+  return SHG_OK;
+}
+
+// Validates a prime list and returns log_n and the LimbPrimes table.
+int check_primes(shg_context* ctx, const int* primes, int num_limbs, int* log_n, LimbPrimes* lp) {
+  if (!ctx) return fail(SHG_E_INVALID_ARGUMENT, "context must not be null.");
+  if (num_limbs <= 0 || num_limbs > SHG_MAX_LIMBS)
+    return fail(SHG_E_INVALID_ARGUMENT, "`moduli` must contain between 1 and %d RNS moduli.",
+                SHG_MAX_LIMBS);
+  if (!primes) return fail(SHG_E_INVALID_ARGUMENT, "`moduli` must not be empty.");
+  int ln = -1;
+  for (int i = 0; i < num_limbs; ++i) {
+    const int p = primes[i];
+    if (p < 0 || p >= (int)ctx->primes.size())
+      return fail(SHG_E_INVALID_ARGUMENT, "invalid prime handle %d.", p);
+    const int pl = ctx->primes[p]->log_n;
+    if (ln < 0) ln = pl;
+    if (pl != ln)
+      return fail(SHG_E_INVALID_ARGUMENT, "all RNS moduli must share one ring degree.");
+    lp->id[i] = p;
+  }
+  *log_n = ln;
+  return SHG_OK;
+}
+
+int resolve_variant(shg_context* ctx, int variant, const int* primes, int num_limbs, int log_n,
+                    long long rows) {
+  bool f64_ok = true;
+  for (int i = 0; i < num_limbs; ++i)
+    if (ctx->primes[primes[i]]->q >= (1ull << 50)) f64_ok = false;
+  if (variant == SHG_NTT_F64 && !f64_ok) return -1;
+  if (variant == SHG_NTT_INT_SHOUP) return SHG_NTT_INT_SHOUP;
+  if (variant == SHG_NTT_F64) return SHG_NTT_F64;
+  if (!f64_ok) return SHG_NTT_INT_SHOUP;
+  int bucket = 0;
+  while ((1LL << (bucket + 1)) <= rows && bucket < 30) ++bucket;
+  auto it = ctx->tuned.find({log_n, bucket});
+  if (it != ctx->tuned.end()) return it->second;
+  return SHG_NTT_F64;
+}
+
+bool aligned_ptr(const void* p) { return p != nullptr && ((uintptr_t)p % 8) == 0; }
+
+int build_gadget(shg_context* ctx, const int* primes, int num_limbs, const int* log_bs,
+                 shg::GadgetDesc* gd) {
+  if (!log_bs) return fail(SHG_E_INVALID_ARGUMENT, "`log_bs` must contain %d elements.", num_limbs);
+  int k = 0;
+  for (int i = 0; i < num_limbs; ++i) {
+    const int b = log_bs[i];
+    const unsigned lq = ctx->primes[primes[i]]->mp->log_modulus;
+    if (b <= 0 || b >= 64) return fail(SHG_E_INVALID_ARGUMENT, "gadget base log must be in [1, 63].");
+    const int dims = (int)((lq + (b - 1)) / b);  // GadgetSize (gadget.h:43-47)
+    for (int d = 0; d < dims; ++d) {
+      if (k >= 256) return fail(SHG_E_UNSUPPORTED, "gadget dimension above 256.");
+      gd->src_limb[k] = (unsigned char)i;
+      gd->shift[k] = (unsigned char)(d * b);
+      gd->log_b[k] = (unsigned char)b;
+      ++k;
+    }
+  }
+  gd->k = k;
+  return SHG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* shg_last_error(void) { return g_last_error.c_str(); }
+
+int shg_create(int device, shg_context** out) {
+  if (!out) return fail(SHG_E_INVALID_ARGUMENT, "out must not be null.");
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n == 0) return fail(SHG_E_DEVICE, "no HIP device available.");
+  if (device < 0 || device >= n) return fail(SHG_E_INVALID_ARGUMENT, "device %d out of range.", device);
+  e = hipSetDevice(device);
+  if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+  auto* ctx = new shg_context();
+  ctx->device = device;
+  e = hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete ctx;
+    return hip_fail(e, "hipStreamCreate");
+  }
+  ctx->stream = ctx->own_stream;
+  ctx->d_capacity = 256;
+  e = hipMalloc(&ctx->d_primes, sizeof(PrimeDev) * ctx->d_capacity);
+  if (e != hipSuccess) {
+    hipStreamDestroy(ctx->own_stream);
+    delete ctx;
+    return hip_fail(e, "hipMalloc(prime table)");
+  }
+  *out = ctx;
+  return SHG_OK;
+}
+
+int shg_destroy(shg_context* ctx) {
+  if (!ctx) return SHG_OK;
+  hipSetDevice(ctx->device);
+  hipStreamSynchronize(ctx->stream);
+  for (auto& p : ctx->primes)
+    if (p->d_tables) hipFree(p->d_tables);
+  if (ctx->d_primes) hipFree(ctx->d_primes);
+  if (ctx->scratch) hipFree(ctx->scratch);
+  if (ctx->own_stream) hipStreamDestroy(ctx->own_stream);
+  delete ctx;
+  return SHG_OK;
+}
+
+int shg_set_stream(shg_context* ctx, void* stream) {
+  if (!ctx) return fail(SHG_E_INVALID_ARGUMENT, "context must not be null.");
+  ctx->stream = stream ? (hipStream_t)stream : ctx->own_stream;
+  return SHG_OK;
+}
+
+void* shg_get_stream(shg_context* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+int shg_sync(shg_context* ctx) {
+  if (!ctx) return fail(SHG_E_INVALID_ARGUMENT, "context must not be null.");
+  hipError_t e = hipStreamSynchronize(ctx->stream);
+  return e == hipSuccess ? SHG_OK : hip_fail(e, "hipStreamSynchronize");
+}
+
+int shg_alloc(shg_context* ctx, size_t bytes, void** dptr) {
+  if (!ctx || !dptr) return fail(SHG_E_INVALID_ARGUMENT, "null argument.");
+  hipError_t e = hipMalloc(dptr, bytes ? bytes : 8);
+  return e == hipSuccess ? SHG_OK : fail(SHG_E_NO_MEMORY, "hipMalloc(%zu) failed", bytes);
+}
+
+int shg_free(shg_context* ctx, void* dptr) {
+  if (!ctx) return fail(SHG_E_INVALID_ARGUMENT, "context must not be null.");
+  if (!dptr) return SHG_OK;
+  hipStreamSynchronize(ctx->stream);
+  hipError_t e = hipFree(dptr);
+  return e == hipSuccess ? SHG_OK : hip_fail(e, "hipFree");
+}
+
+int shg_upload(shg_context* ctx, void* dptr, const void* host, size_t bytes) {
+  if (!ctx || (!dptr && bytes) || (!host && bytes)) return fail(SHG_E_INVALID_ARGUMENT, "null argument.");
+  hipError_t e = hipMemcpyAsync(dptr, host, bytes, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  return e == hipSuccess ? SHG_OK : hip_fail(e, "upload");
+}
+
+int shg_download(shg_context* ctx, void* host, const void* dptr, size_t bytes) {
+  if (!ctx || (!dptr && bytes) || (!host && bytes)) return fail(SHG_E_INVALID_ARGUMENT, "null argument.");
+  hipError_t e = hipMemcpyAsync(host, dptr, bytes, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  return e == hipSuccess ? SHG_OK : hip_fail(e, "download");
+}
+
+int shg_prime_create(shg_context* ctx, uint64_t q, int log_n, int* prime) {
+  if (!ctx || !prime) return fail(SHG_E_INVALID_ARGUMENT, "null argument.");
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  auto key = std::make_pair(q, log_n);
+  auto it = ctx->prime_index.find(key);
+  if (it != ctx->prime_index.end()) {
+    *prime = it->second;
+    return SHG_OK;
+  }
+  auto mp_or = Params::Create(q);
+  if (!mp_or.ok()) return fail(SHG_E_INVALID_ARGUMENT, "%s", mp_or.status().message().c_str());
+  auto rec = std::make_unique<PrimeRec>();
+  rec->q = q;
+  rec->log_n = log_n;
+  rec->mp = std::move(mp_or).value();
+  auto ntt_or = rlwe::InitializeNttParameters<ModInt>(log_n, rec->mp.get());
+  if (!ntt_or.ok()) return fail(SHG_E_INVALID_ARGUMENT, "%s", ntt_or.status().message().c_str());
+  rec->ntt = std::move(ntt_or).value();
+  const size_t n = (size_t)1 << log_n;
+  std::vector<uint64_t> fw(2 * n), iw(2 * n);
+  std::vector<double> fwd(2 * n), iwd(2 * n);
+  const double qd = (double)q;
+  for (size_t i = 0; i < n; ++i) {
+    fw[2 * i] = std::get<0>(rec->ntt.psis_bitrev_constant[i]);
+    fw[2 * i + 1] = std::get<1>(rec->ntt.psis_bitrev_constant[i]);
+    iw[2 * i] = std::get<0>(rec->ntt.psis_inv_bitrev_constant[i]);
+    iw[2 * i + 1] = std::get<1>(rec->ntt.psis_inv_bitrev_constant[i]);
+    fwd[2 * i] = (double)fw[2 * i];
+    fwd[2 * i + 1] = (double)fw[2 * i] / qd;
+    iwd[2 * i] = (double)iw[2 * i];
+    iwd[2 * i + 1] = (double)iw[2 * i] / qd;
+  }
+  const size_t bytes = 4 * 2 * n * 8;
+  hipError_t e = hipMalloc(&rec->d_tables, bytes);
+  if (e != hipSuccess) return fail(SHG_E_NO_MEMORY, "twiddle table allocation failed");
+  char* base = (char*)rec->d_tables;
+  hipMemcpy(base, fw.data(), 2 * n * 8, hipMemcpyHostToDevice);
+  hipMemcpy(base + 2 * n * 8, iw.data(), 2 * n * 8, hipMemcpyHostToDevice);
+  hipMemcpy(base + 4 * n * 8, fwd.data(), 2 * n * 8, hipMemcpyHostToDevice);
+  e = hipMemcpy(base + 6 * n * 8, iwd.data(), 2 * n * 8, hipMemcpyHostToDevice);
+  if (e != hipSuccess) return hip_fail(e, "twiddle upload");
+  PrimeDev& d = rec->dev;
+  const Params& P = *rec->mp;
+  d.q = q;
+  d.inv_modulus = P.inv_modulus;
+  d.r_mod_q = P.r_mod_modulus;
+  d.r_barrett = P.r_mod_modulus_barrett;
+  d.inv_r = P.inv_r;
+  d.inv_r_barrett = P.inv_r_barrett;
+  d.bn = (uint64_t)P.barrett_numerator;
+  auto ninv_c = rec->ntt.n_inv_ptr->GetConstant(&P);
+  d.n_inv_c = std::get<0>(ninv_c);
+  d.n_inv_cb = std::get<1>(ninv_c);
+  d.qd = qd;
+  d.qinv = 1.0 / qd;
+  d.n_inv_d = (double)d.n_inv_c;
+  d.n_inv_dq = (double)d.n_inv_c / qd;
+  d.log_n = log_n;
+  d.log_q = (int)P.log_modulus;
+  d.fw = (const uint64_t*)base;
+  d.iw = (const uint64_t*)(base + 2 * n * 8);
+  d.fwd = (const double*)(base + 4 * n * 8);
+  d.iwd = (const double*)(base + 6 * n * 8);
+  const int id = (int)ctx->primes.size();
+  if (id >= ctx->d_capacity) {
+    PrimeDev* bigger = nullptr;
+    e = hipMalloc(&bigger, sizeof(PrimeDev) * ctx->d_capacity * 2);
+    if (e != hipSuccess) return fail(SHG_E_NO_MEMORY, "prime table growth failed");
+    hipStreamSynchronize(ctx->stream);
+    hipMemcpy(bigger, ctx->d_primes, sizeof(PrimeDev) * ctx->d_capacity, hipMemcpyDeviceToDevice);
+    hipFree(ctx->d_primes);
+    ctx->d_primes = bigger;
+    ctx->d_capacity *= 2;
+  }
+  e = hipMemcpy(ctx->d_primes + id, &d, sizeof(PrimeDev), hipMemcpyHostToDevice);
+  if (e != hipSuccess) return hip_fail(e, "prime upload");
+  ctx->primes.push_back(std::move(rec));
+  ctx->prime_index[key] = id;
+  *prime = id;
+  return SHG_OK;
+}
+
+int shg_prime_info(shg_context* ctx, int prime, uint64_t* params, uint64_t* tables) {
+  if (!ctx || prime < 0 || prime >= (int)ctx->primes.size())
+    return fail(SHG_E_INVALID_ARGUMENT, "invalid prime handle.");
+  const PrimeRec& r = *ctx->primes[prime];
+  const Params& P = *r.mp;
+  if (params) {
+    params[0] = P.modulus;
+    params[1] = P.log_modulus;
+    params[2] = P.r_mod_modulus;
+    params[3] = P.r_mod_modulus_barrett;
+    params[4] = P.inv_modulus;
+    params[5] = P.inv_r;
+    params[6] = P.inv_r_barrett;
+    params[7] = (uint64_t)P.barrett_numerator;
+    params[8] = r.ntt.psi->GetMontgomeryRepresentation();
+    params[9] = r.ntt.n_inv_ptr->GetMontgomeryRepresentation();
+  }
+  if (tables) {
+    const size_t n = (size_t)1 << r.log_n;
+    for (size_t i = 0; i < n; ++i) {
+      tables[i] = std::get<0>(r.ntt.psis_bitrev_constant[i]);
+      tables[n + i] = std::get<1>(r.ntt.psis_bitrev_constant[i]);
+      tables[2 * n + i] = std::get<0>(r.ntt.psis_inv_bitrev_constant[i]);
+      tables[3 * n + i] = std::get<1>(r.ntt.psis_inv_bitrev_constant[i]);
+    }
+  }
+  return SHG_OK;
+}
+
+static int ntt_common(shg_context* ctx, bool inverse, uint64_t* polys, int count, int num_limbs,
+                      const int* primes, int variant) {
+  LimbPrimes lp{};
+  int log_n = 0;
+  int rc = check_primes(ctx, primes, num_limbs, &log_n, &lp);
+  if (rc) return rc;
+  if (count < 0) return fail(SHG_E_INVALID_ARGUMENT, "count must be non-negative.");
+  if (count == 0) return SHG_OK;
+  if (!aligned_ptr(polys)) return fail(SHG_E_INVALID_ARGUMENT, "polynomial data must not be null.");
+  const long long rows = (long long)count * num_limbs;
+  if ((rows << (log_n > shg::kMaxLogM ? log_n - shg::kMaxLogM : 0)) > 0x7fffffffLL)
+    return fail(SHG_E_INVALID_ARGUMENT, "batch too large for one launch.");
+  const int v = resolve_variant(ctx, variant, primes, num_limbs, log_n, rows);
+  if (v < 0) return fail(SHG_E_INVALID_ARGUMENT, "FP64 transform requires moduli below 2^50.");
+  hipError_t e = shg::launch_ntt(inverse, v, polys, ctx->d_primes, lp, (int)rows, num_limbs, log_n,
+                                 shg::kMaxLogM, ctx->stream);
+  return e == hipSuccess ? SHG_OK : hip_fail(e, "ntt launch");
+}
+
+int shg_ntt_fwd(shg_context* ctx, uint64_t* polys, int count, int num_limbs, const int* primes,
+                int variant) {
+  return ntt_common(ctx, false, polys, count, num_limbs, primes, variant);
+}
+
+int shg_ntt_inv(shg_context* ctx, uint64_t* polys, int count, int num_limbs, const int* primes,
+                int variant) {
+  return ntt_common(ctx, true, polys, count, num_limbs, primes, variant);
+}
+
+int shg_pointwise(shg_context* ctx, int op, uint64_t* x, const uint64_t* a, size_t a_stride,
+                  const uint64_t* b, size_t b_stride, int count, int num_limbs, const int* primes) {
+  LimbPrimes lp{};
+  int log_n = 0;
+  int rc = check_primes(ctx, primes, num_limbs, &log_n, &lp);
+  if (rc) return rc;
+  if (op < SHG_OP_ADD || op > SHG_OP_FMA) return fail(SHG_E_INVALID_ARGUMENT, "unknown op %d.", op);
+  if (count <= 0) return count == 0 ? SHG_OK : fail(SHG_E_INVALID_ARGUMENT, "count must be non-negative.");
+  if (!aligned_ptr(x)) return fail(SHG_E_INVALID_ARGUMENT, "`this` must not be null.");
+  if (op != SHG_OP_NEG && !aligned_ptr(a)) return fail(SHG_E_INVALID_ARGUMENT, "`that` must not be null.");
+  if (op == SHG_OP_FMA && !aligned_ptr(b)) return fail(SHG_E_INVALID_ARGUMENT, "`b` must not be null.");
+  hipError_t e = shg::launch_pointwise(op, x, a, a_stride, b, b_stride, ctx->d_primes, lp, count,
+                                       num_limbs, log_n, ctx->stream);
+  return e == hipSuccess ? SHG_OK : hip_fail(e, "pointwise launch");
+}
+
+int shg_mul_scalar(shg_context* ctx, uint64_t* x, const uint64_t* host_scalars, int count,
+                   int num_limbs, const int* primes) {
+  LimbPrimes lp{};
+  int log_n = 0;
+  int rc = check_primes(ctx, primes, num_limbs, &log_n, &lp);
+  if (rc) return rc;
+  if (!host_scalars)
+    return fail(SHG_E_INVALID_ARGUMENT, "`scalar_mod_qs` must contain %d modular integers.", num_limbs);
+  if (count <= 0) return count == 0 ? SHG_OK : fail(SHG_E_INVALID_ARGUMENT, "count must be non-negative.");
+  if (!aligned_ptr(x)) return fail(SHG_E_INVALID_ARGUMENT, "`this` must not be null.");
+  LimbPrimes lo{}, hi{};
+  for (int i = 0; i < num_limbs; ++i) {
+    lo.id[i] = (int)(uint32_t)host_scalars[i];
+    hi.id[i] = (int)(uint32_t)(host_scalars[i] >> 32);
+  }
+  hipError_t e = shg::launch_mul_scalar(x, lo, hi, ctx->d_primes, lp, count, num_limbs, log_n,
+                                        ctx->stream);
+  return e == hipSuccess ? SHG_OK : hip_fail(e, "mul_scalar launch");
+}
+
+int shg_mul_int(shg_context* ctx, uint64_t* x, uint64_t scalar, int count, int num_limbs,
+                const int* primes) {
+  LimbPrimes lp{};
+  int log_n = 0;
+  int rc = check_primes(ctx, primes, num_limbs, &log_n, &lp);
+  if (rc) return rc;
+  std::vector<uint64_t> s(num_limbs);
+  for (int i = 0; i < num_limbs; ++i)
+    s[i] = ModInt::ImportInt(scalar, ctx->primes[primes[i]]->mp.get()).value().GetMontgomeryRepresentation();
+  return shg_mul_scalar(ctx, x, s.data(), count, num_limbs, primes);
+}
+
+int shg_substitute(shg_context* ctx, const uint64_t* in, uint64_t* out, int power, int count,
+                   int num_limbs, const int* primes) {
+  LimbPrimes lp{};
+  int log_n = 0;
+  int rc = check_primes(ctx, primes, num_limbs, &log_n, &lp);
+  if (rc) return rc;
+  const int n = 1 << log_n;
+  if (power < 0 || (power % 2) == 0 || power >= 2 * n)
+    return fail(SHG_E_INVALID_ARGUMENT,
+                "Substitution power must be a non-negative odd integer less than 2*n.");
+  if (count <= 0) return count == 0 ? SHG_OK : fail(SHG_E_INVALID_ARGUMENT, "count must be non-negative.");
+  if (!aligned_ptr(in) || !aligned_ptr(out) || in == out)
+    return fail(SHG_E_INVALID_ARGUMENT, "substitute needs distinct input and output buffers.");
+  hipError_t e = shg::launch_substitute(in, out, power, count, num_limbs, log_n, ctx->stream);
+  return e == hipSuccess ? SHG_OK : hip_fail(e, "substitute launch");
+}
+
+int shg_mod_reduce_lsb(shg_context* ctx, uint64_t* polys, uint64_t t, const uint64_t* host_ql_inv,
+                       int is_ntt, int count, int num_limbs, const int* primes) {
+  LimbPrimes lp{};
+  int log_n = 0;
+  int rc = check_primes(ctx, primes, num_limbs, &log_n, &lp);
+  if (rc) return rc;
+  if (num_limbs <= 1)
+    return fail(SHG_E_INVALID_ARGUMENT, "ModReduceLsb cannot apply with only one prime modulus.");
+  if (!host_ql_inv)
+    return fail(SHG_E_INVALID_ARGUMENT, "`ql_inv` must be defined with respect to %d moduli.",
+                num_limbs - 1);
+  if (t == 0) return fail(SHG_E_INVALID_ARGUMENT, "plaintext modulus must be positive.");
+  if (count <= 0) return count == 0 ? SHG_OK : fail(SHG_E_INVALID_ARGUMENT, "count must be non-negative.");
+  if (!aligned_ptr(polys)) return fail(SHG_E_INVALID_ARGUMENT, "polynomial data must not be null.");
+  const int nr = num_limbs - 1;
+  const int last = primes[nr];
+  const Params* Pl = ctx->primes[last]->mp.get();
+  const uint64_t r = Pl->modulus % t;
+  const uint64_t k = (Pl->modulus - r) / t;
+  shg::ModReduceArgs args{};
+  args.t = t;
+  args.k_mod_ql = ModInt::ImportInt(k, Pl).value().GetMontgomeryRepresentation();
+  for (int i = 0; i < nr; ++i) {
+    const Params* Pi = ctx->primes[primes[i]]->mp.get();
+    args.r_mod_qi[i] = ModInt::ImportInt(r, Pi).value().GetMontgomeryRepresentation();
+    args.t_mod_qi[i] = ModInt::ImportInt(t, Pi).value().GetMontgomeryRepresentation();
+    args.ql_inv[i] = host_ql_inv[i];
+  }
+  const size_t n = (size_t)1 << log_n;
+  const size_t need = (size_t)count * n * 8 * (1 + nr);
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  rc = ensure_scratch(ctx, need);
+  if (rc) return rc;
+  uint64_t* delta_l = (uint64_t*)ctx->scratch;
+  uint64_t* delta = delta_l + (size_t)count * n;
+  hipStream_t s = ctx->stream;
+  hipError_t e = shg::launch_modreduce_extract(polys, delta_l, ctx->d_primes, last, args.k_mod_ql,
+                                               count, num_limbs, log_n, s);
+  LimbPrimes lpl{};
+  lpl.id[0] = last;
+  LimbPrimes lpr = lp;
+  if (e == hipSuccess && is_ntt)
+    e = shg::launch_ntt(true, resolve_variant(ctx, SHG_NTT_AUTO, &last, 1, log_n, count), delta_l,
+                        ctx->d_primes, lpl, count, 1, log_n, shg::kMaxLogM, s);
+  if (e == hipSuccess)
+    e = shg::launch_modreduce_lift(delta_l, delta, ctx->d_primes, last, lpr, count, num_limbs,
+                                   log_n, s);
+  if (e == hipSuccess && is_ntt)
+    e = shg::launch_ntt(false, resolve_variant(ctx, SHG_NTT_AUTO, primes, nr, log_n,
+                                               (long long)count * nr),
+                        delta, ctx->d_primes, lpr, count * nr, nr, log_n, shg::kMaxLogM, s);
+  if (e == hipSuccess)
+    e = shg::launch_modreduce_combine(polys, delta, args, ctx->d_primes, lpr, count, num_limbs,
+                                      log_n, s);
+  return e == hipSuccess ? SHG_OK : hip_fail(e, "mod_reduce launch");
+}
+
+int shg_tensor_product(shg_context* ctx, const uint64_t* a, int len_a, const uint64_t* b, int len_b,
+                       uint64_t* out, int count, int num_limbs, const int* primes) {
+  LimbPrimes lp{};
+  int log_n = 0;
+  int rc = check_primes(ctx, primes, num_limbs, &log_n, &lp);
+  if (rc) return rc;
+  if (len_a <= 0 || len_b <= 0 || len_a > 8 || len_b > 8)
+    return fail(SHG_E_INVALID_ARGUMENT, "ciphertext lengths must be in [1, 8].");
+  if (count <= 0) return count == 0 ? SHG_OK : fail(SHG_E_INVALID_ARGUMENT, "count must be non-negative.");
+  if (!aligned_ptr(a) || !aligned_ptr(b) || !aligned_ptr(out))
+    return fail(SHG_E_INVALID_ARGUMENT, "ciphertext data must not be null.");
+  hipError_t e = shg::launch_tensor_product(a, len_a, b, len_b, out, ctx->d_primes, lp, count,
+                                            num_limbs, log_n, ctx->stream);
+  return e == hipSuccess ? SHG_OK : hip_fail(e, "tensor launch");
+}
+
+int shg_gadget_decompose(shg_context* ctx, const uint64_t* a, const int* host_log_bs,
+                         uint64_t* digits, int count, int num_limbs, const int* primes) {
+  LimbPrimes lp{};
+  int log_n = 0;
+  int rc = check_primes(ctx, primes, num_limbs, &log_n, &lp);
+  if (rc) return rc;
+  shg::GadgetDesc gd{};
+  rc = build_gadget(ctx, primes, num_limbs, host_log_bs, &gd);
+  if (rc) return rc;
+  if (count <= 0) return count == 0 ? SHG_OK : fail(SHG_E_INVALID_ARGUMENT, "count must be non-negative.");
+  if (!aligned_ptr(a) || !aligned_ptr(digits))
+    return fail(SHG_E_INVALID_ARGUMENT, "data must not be null.");
+  hipError_t e = shg::launch_gadget_decompose(a, digits, gd, ctx->d_primes, lp, count, num_limbs,
+                                              log_n, ctx->stream);
+  return e == hipSuccess ? SHG_OK : hip_fail(e, "decompose launch");
+}
+
+// Shared body of the two key-switch entry points: for each component index
+// in `srcs`, copy it to scratch, bring it to coefficient form, and run the
+// fused key-switch kernel (first call initialises from c0/c1, later ones
+// accumulate).
+static int keyswitch_common(shg_context* ctx, const uint64_t* comps, int comps_per_ct,
+                            const int* src_comp, const int* src_is_ntt, const int* key_base,
+                            int num_src, bool init_c1, const uint64_t* key_as,
+                            const uint64_t* key_bs, const int* log_bs, uint64_t* out_c0,
+                            uint64_t* out_c1, int count, int num_limbs, const int* primes,
+                            const LimbPrimes& lp, int log_n, const shg::GadgetDesc& gd) {
+  const size_t n = (size_t)1 << log_n;
+  const size_t poly_words = (size_t)num_limbs * n;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  int rc = ensure_scratch(ctx, (size_t)count * poly_words * 8);
+  if (rc) return rc;
+  uint64_t* src = (uint64_t*)ctx->scratch;
+  hipStream_t s = ctx->stream;
+  const int v = resolve_variant(ctx, SHG_NTT_AUTO, primes, num_limbs, log_n,
+                                (long long)count * num_limbs);
+  hipError_t e = hipSuccess;
+  for (int i = 0; i < num_src && e == hipSuccess; ++i) {
+    e = hipMemcpy2DAsync(src, poly_words * 8, comps + (size_t)src_comp[i] * poly_words,
+                         (size_t)comps_per_ct * poly_words * 8, poly_words * 8, count,
+                         hipMemcpyDeviceToDevice, s);
+    if (e == hipSuccess && src_is_ntt[i])
+      e = shg::launch_ntt(true, v, src, ctx->d_primes, lp, count * num_limbs, num_limbs, log_n,
+                          shg::kMaxLogM, s);
+    if (e != hipSuccess) break;
+    if (log_n <= shg::kMaxLogM && log_n >= 4) {
+      shg::KeySwitchArgs ka{};
+      ka.src = src;
+      ka.src_stride = poly_words;
+      ka.c0 = comps;
+      ka.c0_stride = (size_t)comps_per_ct * poly_words;
+      ka.c1 = init_c1 ? comps + poly_words : nullptr;
+      ka.c1_stride = (size_t)comps_per_ct * poly_words;
+      ka.key_a = key_as;
+      ka.key_b = key_bs;
+      ka.key_base = key_base[i];
+      ka.out0 = out_c0;
+      ka.out1 = out_c1;
+      ka.out_stride = poly_words;
+      ka.accumulate = i > 0;
+      e = shg::launch_keyswitch(ka, gd, v, ctx->d_primes, lp, count, num_limbs, log_n, s);
+    } else {
+      // Unfused form for ring degrees outside the fused kernel's range:
+      // materialise the digits, transform them, and multiply-accumulate.
+      const size_t need = (size_t)count * poly_words * 8 * (1 + (size_t)gd.k);
+      if (need > ctx->scratch_size) {
+        // Grow while keeping the coefficient-form source.
+        void* bigger = nullptr;
+        if (hipMalloc(&bigger, need) != hipSuccess)
+          return fail(SHG_E_NO_MEMORY, "scratch allocation of %zu bytes failed", need);
+        hipMemcpyAsync(bigger, src, (size_t)count * poly_words * 8, hipMemcpyDeviceToDevice, s);
+        hipStreamSynchronize(s);
+        hipFree(ctx->scratch);
+        ctx->scratch = bigger;
+        ctx->scratch_size = need;
+        src = (uint64_t*)bigger;
+      }
+      uint64_t* digits = src + (size_t)count * poly_words;
+      e = shg::launch_gadget_decompose(src, digits, gd, ctx->d_primes, lp, count, num_limbs,
+                                       log_n, s);
+      if (e == hipSuccess)
+        e = shg::launch_ntt(false, v, digits, ctx->d_primes, lp, count * gd.k * num_limbs,
+                            num_limbs, log_n, shg::kMaxLogM, s);
+      if (e == hipSuccess && i == 0) {
+        e = hipMemcpy2DAsync(out_c0, poly_words * 8, comps, (size_t)comps_per_ct * poly_words * 8,
+                             poly_words * 8, count, hipMemcpyDeviceToDevice, s);
+        if (e == hipSuccess) {
+          if (init_c1)
+            e = hipMemcpy2DAsync(out_c1, poly_words * 8, comps + poly_words,
+                                 (size_t)comps_per_ct * poly_words * 8, poly_words * 8, count,
+                                 hipMemcpyDeviceToDevice, s);
+          else
+            e = hipMemsetAsync(out_c1, 0, (size_t)count * poly_words * 8, s);
+        }
+      }
+      for (int j = 0; j < gd.k && e == hipSuccess; ++j) {
+        const uint64_t* dj = digits + (size_t)j * poly_words;
+        e = shg::launch_pointwise(SHG_OP_FMA, out_c0, dj, (size_t)gd.k * poly_words,
+                                  key_bs + (size_t)(key_base[i] + j) * poly_words, 0,
+                                  ctx->d_primes, lp, count, num_limbs, log_n, s);
+        if (e == hipSuccess)
+          e = shg::launch_pointwise(SHG_OP_FMA, out_c1, dj, (size_t)gd.k * poly_words,
+                                    key_as + (size_t)(key_base[i] + j) * poly_words, 0,
+                                    ctx->d_primes, lp, count, num_limbs, log_n, s);
+      }
+    }
+  }
+  (void)log_bs;
+  return e == hipSuccess ? SHG_OK : hip_fail(e, "key switch launch");
+}
+
+int shg_relin_apply(shg_context* ctx, const uint64_t* comps, int degree, int ntt_mask,
+                    const uint64_t* key_as, const uint64_t* key_bs, int num_key_polys,
+                    const int* host_log_bs, uint64_t* out_c0, uint64_t* out_c1, int count,
+                    int num_limbs, const int* primes) {
+  LimbPrimes lp{};
+  int log_n = 0;
+  int rc = check_primes(ctx, primes, num_limbs, &log_n, &lp);
+  if (rc) return rc;
+  shg::GadgetDesc gd{};
+  rc = build_gadget(ctx, primes, num_limbs, host_log_bs, &gd);
+  if (rc) return rc;
+  if (degree < 1 || degree > 16)
+    return fail(SHG_E_INVALID_ARGUMENT, "ciphertext degree must be in [1, 16].");
+  if (!(ntt_mask & 1)) return fail(SHG_E_INVALID_ARGUMENT, "RNS polynomial `this` must be in NTT form.");
+  if (!(ntt_mask & 2)) return fail(SHG_E_INVALID_ARGUMENT, "RNS polynomial `this` must be in NTT form.");
+  const int needed = degree >= 2 ? (degree - 2) * degree + gd.k : 0;
+  if (num_key_polys < needed)
+    return fail(SHG_E_INVALID_ARGUMENT,
+                "`ciphertext` degree is larger than degree of this relinearization key.");
+  if (count <= 0) return count == 0 ? SHG_OK : fail(SHG_E_INVALID_ARGUMENT, "count must be non-negative.");
+  if (!aligned_ptr(comps) || !aligned_ptr(out_c0) || !aligned_ptr(out_c1) ||
+      (degree >= 2 && (!aligned_ptr(key_as) || !aligned_ptr(key_bs))))
+    return fail(SHG_E_INVALID_ARGUMENT, "data must not be null.");
+  const size_t poly_words = (size_t)num_limbs << log_n;
+  if (degree == 1) {
+    hipStream_t s = ctx->stream;
+    hipError_t e = hipMemcpy2DAsync(out_c0, poly_words * 8, comps, 2 * poly_words * 8,
+                                    poly_words * 8, count, hipMemcpyDeviceToDevice, s);
+    if (e == hipSuccess)
+      e = hipMemcpy2DAsync(out_c1, poly_words * 8, comps + poly_words, 2 * poly_words * 8,
+                           poly_words * 8, count, hipMemcpyDeviceToDevice, s);
+    return e == hipSuccess ? SHG_OK : hip_fail(e, "copy");
+  }
+  std::vector<int> src_comp, src_ntt, base;
+  for (int i = 2; i <= degree; ++i) {
+    src_comp.push_back(i);
+    src_ntt.push_back((ntt_mask >> i) & 1);
+    base.push_back((i - 2) * degree);
+  }
+  return keyswitch_common(ctx, comps, degree + 1, src_comp.data(), src_ntt.data(), base.data(),
+                          (int)src_comp.size(), true, key_as, key_bs, host_log_bs, out_c0, out_c1,
+                          count, num_limbs, primes, lp, log_n, gd);
+}
+
+int shg_galois_apply(shg_context* ctx, const uint64_t* comps, int c1_is_ntt, const uint64_t* key_as,
+                     const uint64_t* key_bs, const int* host_log_bs, uint64_t* out_c0,
+                     uint64_t* out_c1, int count, int num_limbs, const int* primes) {
+  LimbPrimes lp{};
+  int log_n = 0;
+  int rc = check_primes(ctx, primes, num_limbs, &log_n, &lp);
+  if (rc) return rc;
+  shg::GadgetDesc gd{};
+  rc = build_gadget(ctx, primes, num_limbs, host_log_bs, &gd);
+  if (rc) return rc;
+  if (count <= 0) return count == 0 ? SHG_OK : fail(SHG_E_INVALID_ARGUMENT, "count must be non-negative.");
+  if (!aligned_ptr(comps) || !aligned_ptr(out_c0) || !aligned_ptr(out_c1) || !aligned_ptr(key_as) ||
+      !aligned_ptr(key_bs))
+    return fail(SHG_E_INVALID_ARGUMENT, "data must not be null.");
+  const int src_comp = 1, src_ntt = c1_is_ntt ? 1 : 0, base = 0;
+  return keyswitch_common(ctx, comps, 2, &src_comp, &src_ntt, &base, 1, false, key_as, key_bs,
+                          host_log_bs, out_c0, out_c1, count, num_limbs, primes, lp, log_n, gd);
+}
+
+int shg_lazy_fma(shg_context* ctx, const uint64_t* as, size_t as_stride, const uint64_t* bs,
+                 size_t bs_stride, int num_terms, uint64_t* out, int num_limbs, const int* primes) {
+  LimbPrimes lp{};
+  int log_n = 0;
+  int rc = check_primes(ctx, primes, num_limbs, &log_n, &lp);
+  if (rc) return rc;
+  if (num_terms < 0) return fail(SHG_E_INVALID_ARGUMENT, "num_terms must be non-negative.");
+  if (!aligned_ptr(out) || (num_terms > 0 && (!aligned_ptr(as) || !aligned_ptr(bs))))
+    return fail(SHG_E_INVALID_ARGUMENT, "data must not be null.");
+  for (int i = 0; i < num_limbs; ++i)
+    if (127 - 2 * (int)ctx->primes[primes[i]]->mp->log_modulus < 1)
+      return fail(SHG_E_INVALID_ARGUMENT,
+                  "The RNS moduli do not allow for lazy polynomials, as the logarithm of the "
+                  "maximum level is stricly smaller than 1.");
+  hipError_t e = shg::launch_lazy_fma(as, as_stride, bs, bs_stride, num_terms, out, ctx->d_primes,
+                                      lp, num_limbs, log_n, ctx->stream);
+  return e == hipSuccess ? SHG_OK : hip_fail(e, "lazy fma launch");
+}
+
+int shg_batch_limbs(shg_context* ctx, const shg_limb_job* host_jobs, int num_jobs, int op) {
+  return fail(SHG_E_UNSUPPORTED, "batched limb jobs are not available in this build.");
+}
+
+int shg_autotune_ntt(shg_context* ctx, int log_n, int limbs, int* chosen) {
+  return fail(SHG_E_UNSUPPORTED, "autotuning is not available in this build.");
+}
+
+}  // extern "C"

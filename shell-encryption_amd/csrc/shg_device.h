@@ -1,0 +1,145 @@
+// Device-side data structures and modular arithmetic for the gfx950 kernels.
+//
+// Residues are MontgomeryInt<Uint64> representations in [0, q), q < 2^62
+// (q < 2^50 for the FP64 transform variant).  Every routine here computes an
+// exact modular result, so any of them may stand in for the reference's
+// arithmetic as long as the final value is reduced to the same canonical
+// representative (shell_encryption/montgomery.h:318-465).
+#ifndef SHG_DEVICE_H_
+#define SHG_DEVICE_H_
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "shell_gpu.h"
+
+namespace shg {
+
+// Per-(q, log_n) constants, resident in device memory.  Twiddle tables are
+// the reference's psis_bitrev_constant / psis_inv_bitrev_constant
+// (ntt_parameters.h:160-179) interleaved as (c, cb) pairs, plus FP64 copies
+// interleaved as (w, w/q).
+struct PrimeDev {
+  uint64_t q;
+  uint64_t inv_modulus;    // q * inv_modulus = -1 mod 2^64 (REDC)
+  uint64_t r_mod_q;        // Montgomery form of 1
+  uint64_t r_barrett;      // floor(r_mod_q * 2^64 / q): ImportInt
+  uint64_t inv_r;          // 2^-64 mod q: ExportInt
+  uint64_t inv_r_barrett;  // floor(inv_r * 2^64 / q)
+  uint64_t bn;             // floor(2^64 / q): Barrett reduction
+  uint64_t n_inv_c;        // value of n^-1 (GetConstant().first)
+  uint64_t n_inv_cb;       // its Shoup constant
+  double qd;               // q as double
+  double qinv;             // fl(1/q)
+  double n_inv_d;          // value of n^-1 as double
+  double n_inv_dq;         // fl(n^-1 / q)
+  int log_n;
+  int log_q;
+  const uint64_t* fw;      // [2n] forward (c, cb), bit-reversed psi powers
+  const uint64_t* iw;      // [2n] inverse (c, cb)
+  const double* fwd;       // [2n] forward (w, w/q) as doubles
+  const double* iwd;       // [2n] inverse (w, w/q) as doubles
+};
+
+struct LimbPrimes {
+  int id[SHG_MAX_LIMBS];
+};
+
+// ---------------------------------------------------------------------------
+// 64-bit integer arithmetic from 32-bit VALU multiplies.
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ uint64_t mulhi64(uint64_t a, uint64_t b) {
+  return __umul64hi(a, b);
+}
+
+// Shoup product x * c mod q with cb = floor(c * 2^64 / q): the result is
+// congruent to x*c and lies in [0, 2q) for any 64-bit x.
+__device__ __forceinline__ uint64_t shoup_lazy(uint64_t x, uint64_t c, uint64_t cb, uint64_t q) {
+  return x * c - mulhi64(x, cb) * q;
+}
+
+__device__ __forceinline__ uint64_t csub(uint64_t x, uint64_t m) { return x >= m ? x - m : x; }
+
+// x mod q for any 64-bit x (Barrett with bn = floor(2^64/q)).
+__device__ __forceinline__ uint64_t reduce_full(uint64_t x, uint64_t q, uint64_t bn) {
+  return csub(x - mulhi64(x, bn) * q, q);
+}
+
+// REDC(a*b) = a*b*2^-64 mod q, canonical (montgomery.h:318-336).
+__device__ __forceinline__ uint64_t mont_mul(uint64_t a, uint64_t b, const PrimeDev& P) {
+  const uint64_t lo = a * b;
+  const uint64_t hi = mulhi64(a, b);
+  const uint64_t u = lo * P.inv_modulus;
+  // (a*b + u*q) / 2^64; the low word sums to 0 with a carry iff lo != 0.
+  const uint64_t t = hi + mulhi64(u, P.q) + (lo != 0 ? 1 : 0);
+  return csub(t, P.q);
+}
+
+// this + a*b*2^-64 mod q, canonical (montgomery.h:446-465).
+__device__ __forceinline__ uint64_t mont_fma(uint64_t acc, uint64_t a, uint64_t b,
+                                             const PrimeDev& P) {
+  return csub(acc + mont_mul(a, b, P), P.q);
+}
+
+// ImportInt: n * 2^64 mod q for any 64-bit n (montgomery.cc:26-40).
+__device__ __forceinline__ uint64_t mont_import(uint64_t n, const PrimeDev& P) {
+  return csub(shoup_lazy(n, P.r_mod_q, P.r_barrett, P.q), P.q);
+}
+
+// ExportInt: x * 2^-64 mod q (montgomery.h:156-164).
+__device__ __forceinline__ uint64_t mont_export(uint64_t x, const PrimeDev& P) {
+  return csub(shoup_lazy(x, P.inv_r, P.inv_r_barrett, P.q), P.q);
+}
+
+// AddInPlace / SubInPlace for inputs in [0, q] (q itself arises from
+// NegateInPlace(0)); the result is canonical as BarrettReduce's is
+// (montgomery.h:383-415).
+__device__ __forceinline__ uint64_t mod_add(uint64_t a, uint64_t b, uint64_t q) {
+  return csub(csub(a + b, q), q);
+}
+__device__ __forceinline__ uint64_t mod_sub(uint64_t a, uint64_t b, uint64_t q) {
+  return csub(csub(a + (q - b), q), q);
+}
+
+// ---------------------------------------------------------------------------
+// FP64 exact modular arithmetic (q < 2^50).  Integers are held exactly in
+// doubles; x*w = h + l exactly (h = fl(x*w), l = fma(x, w, -h)), the quotient
+// c = rint(x * fl(w/q)) is within 1 + |x|/2^52 of x*w/q, so h - c*q is an
+// exact small integer and t = (h - c*q) + l = x*w - c*q is exact with
+// |t| <= q * (0.5 + |x| * 2^-52).
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ double u64_to_f64(uint64_t x) {
+  const double hi = (double)(uint32_t)(x >> 32);
+  const double lo = (double)(uint32_t)x;
+  return fma(hi, 4294967296.0, lo);
+}
+
+// x in [0, 2^53) integral -> uint64.
+__device__ __forceinline__ uint64_t f64_to_u64(double x) {
+  const double hd = trunc(x * (1.0 / 4294967296.0));
+  const uint32_t hi = (uint32_t)hd;
+  const uint32_t lo = (uint32_t)fma(hd, -4294967296.0, x);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ double f64_mulmod(double x, double w, double wq, double q) {
+  const double h = x * w;
+  const double l = fma(x, w, -h);
+  const double c = rint(x * wq);
+  const double d = fma(-c, q, h);
+  return d + l;
+}
+
+// x - rint(x/q)*q: |result| <= q/2 (+ a negligible margin) for |x| < 2^53.
+__device__ __forceinline__ double f64_reduce(double x, double q, double qinv) {
+  return fma(-rint(x * qinv), q, x);
+}
+
+// Balanced (|x| < q) -> canonical [0, q).
+__device__ __forceinline__ double f64_canon(double x, double q) { return x < 0.0 ? x + q : x; }
+
+}  // namespace shg
+
+#endif  // SHG_DEVICE_H_

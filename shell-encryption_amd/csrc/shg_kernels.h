@@ -1,0 +1,111 @@
+// Host-callable launchers for the gfx950 kernels (internal to libshellgpu).
+#ifndef SHG_KERNELS_H_
+#define SHG_KERNELS_H_
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "shg_device.h"
+
+namespace shg {
+
+// Largest LDS slice (log2 words) a transform workgroup holds: 2^14 words
+// (136 KiB with padding) fits the 160 KiB of LDS per CU.
+constexpr int kMaxLogM = 14;
+
+hipError_t launch_ntt(bool inverse, int variant, uint64_t* data, const PrimeDev* primes,
+                      const LimbPrimes& lp, int rows, int num_limbs, int logn, int max_logm,
+                      hipStream_t s);
+hipError_t launch_scale_ninv(uint64_t* data, const PrimeDev* primes, const LimbPrimes& lp,
+                             int rows, int num_limbs, int logn, hipStream_t s);
+
+hipError_t launch_pointwise(int op, uint64_t* x, const uint64_t* a, size_t a_stride,
+                            const uint64_t* b, size_t b_stride, const PrimeDev* primes,
+                            const LimbPrimes& lp, int count, int num_limbs, int logn,
+                            hipStream_t s);
+// x[limb] *= scalars[limb] (Montgomery reps; REDC product).
+hipError_t launch_mul_scalar(uint64_t* x, const LimbPrimes& scalars_lo,
+                             const LimbPrimes& scalars_hi, const PrimeDev* primes,
+                             const LimbPrimes& lp, int count, int num_limbs, int logn,
+                             hipStream_t s);
+hipError_t launch_substitute(const uint64_t* in, uint64_t* out, int power, int count,
+                             int num_limbs, int logn, hipStream_t s);
+
+// ModReduceLsb building blocks.
+struct ModReduceArgs {
+  uint64_t t;
+  uint64_t k_mod_ql;                 // ImportInt(k) mod q_l (Montgomery rep)
+  uint64_t r_mod_qi[SHG_MAX_LIMBS];  // ImportInt(r) mod q_i
+  uint64_t t_mod_qi[SHG_MAX_LIMBS];  // ImportInt(t) mod q_i
+  uint64_t ql_inv[SHG_MAX_LIMBS];    // q_l^-1 mod q_i (Montgomery reps)
+};
+// delta_l (scratch, count limbs) <- last limb of each polynomial times k.
+hipError_t launch_modreduce_extract(const uint64_t* polys, uint64_t* delta_l,
+                                    const PrimeDev* primes, int last_prime, uint64_t k_mod_ql,
+                                    int count, int num_limbs, int logn, hipStream_t s);
+// delta (count x (num_limbs-1) limbs) <- balanced conversion of delta_l.
+hipError_t launch_modreduce_lift(const uint64_t* delta_l, uint64_t* delta,
+                                 const PrimeDev* primes, int last_prime, const LimbPrimes& lp,
+                                 int count, int num_limbs, int logn, hipStream_t s);
+// polys[i] <- (r * polys[i] + t * delta[i]) * ql_inv[i] for i < num_limbs-1.
+hipError_t launch_modreduce_combine(uint64_t* polys, const uint64_t* delta,
+                                    const ModReduceArgs& args, const PrimeDev* primes,
+                                    const LimbPrimes& lp, int count, int num_limbs, int logn,
+                                    hipStream_t s);
+
+hipError_t launch_tensor_product(const uint64_t* a, int len_a, const uint64_t* b, int len_b,
+                                 uint64_t* out, const PrimeDev* primes, const LimbPrimes& lp,
+                                 int count, int num_limbs, int logn, hipStream_t s);
+
+// Gadget description: digit j comes from limb src_limb[j], bit offset
+// shift[j], mask (1<<log_b[j])-1.
+struct GadgetDesc {
+  int k;
+  unsigned char src_limb[256];
+  unsigned char shift[256];
+  unsigned char log_b[256];
+};
+hipError_t launch_gadget_decompose(const uint64_t* a, uint64_t* digits, const GadgetDesc& gd,
+                                   const PrimeDev* primes, const LimbPrimes& lp, int count,
+                                   int num_limbs, int logn, hipStream_t s);
+
+// Fused key switch: for each (ciphertext, output limb), for every gadget
+// digit, extract + import + forward NTT in LDS and multiply-accumulate with
+// the key; acc0 starts from c0 (or zero), acc1 from c1 (or zero).
+struct KeySwitchArgs {
+  const uint64_t* src;       // coefficient-form input component(s), count x L x n
+  size_t src_stride;         // words between ciphertexts in src
+  const uint64_t* c0;        // NTT-form c0 (added), or null
+  size_t c0_stride;
+  const uint64_t* c1;        // NTT-form c1 (added), or null
+  size_t c1_stride;
+  const uint64_t* key_a;     // key polys, index key_base + j
+  const uint64_t* key_b;
+  int key_base;
+  uint64_t* out0;
+  uint64_t* out1;
+  size_t out_stride;
+  int accumulate;            // 1: out += ..., else out = init + ...
+};
+hipError_t launch_keyswitch(const KeySwitchArgs& ka, const GadgetDesc& gd, int variant,
+                            const PrimeDev* primes, const LimbPrimes& lp, int count,
+                            int num_limbs, int logn, hipStream_t s);
+
+hipError_t launch_lazy_fma(const uint64_t* as, size_t as_stride, const uint64_t* bs,
+                           size_t bs_stride, int num_terms, uint64_t* out,
+                           const PrimeDev* primes, const LimbPrimes& lp, int num_limbs,
+                           int logn, hipStream_t s);
+
+// Persistent launch over uneven limb jobs.
+struct LimbJobDev {
+  uint64_t* data;
+  const uint64_t* other;
+  int prime;
+  int log_n;
+};
+hipError_t launch_batch_limbs(const LimbJobDev* jobs, int num_jobs, int op, int variant,
+                              const PrimeDev* primes, int grid, hipStream_t s);
+
+}  // namespace shg
+
+#endif  // SHG_KERNELS_H_

@@ -1,0 +1,219 @@
+// Negacyclic NTT / INTT kernels for gfx950 (see shg_ntt_core.h for the
+// register/LDS pass structure and the two arithmetic policies).
+//
+// A slice crosses HBM once per kernel: the first forward pass loads straight
+// from HBM (coalesced across lanes), the last forward pass goes through LDS
+// for a coalesced store, and the inverse mirrors this.  Limbs with
+// n > 2^kMaxLogM (n = 2^15 does not fit in 160 KiB of LDS) first run their
+// leading forward stages (or trailing inverse stages) as streaming radix-2
+// passes over HBM.
+#include <hip/hip_runtime.h>
+
+#include "shg_device.h"
+#include "shg_kernels.h"
+#include "shg_ntt_core.h"
+
+namespace shg {
+
+// ---------------------------------------------------------------------------
+// Kernels.  blockIdx.x = row * (2^hbits) + h where row indexes limbs
+// (count * num_limbs of them) and h the slice within the limb.
+// ---------------------------------------------------------------------------
+
+template <int LOGM, int LOGE, class A>
+__global__ void __launch_bounds__((1 << LOGM) >> LOGE)
+    ntt_fwd_kernel(uint64_t* __restrict__ data, const PrimeDev* __restrict__ primes,
+                   LimbPrimes lp, int num_limbs, int logn) {
+  using V = typename A::V;
+  constexpr int E = 1 << LOGE;
+  constexpr int T = (1 << LOGM) >> LOGE;
+  extern __shared__ __align__(16) unsigned char smem[];
+  V* lds = reinterpret_cast<V*>(smem);
+  const int hbits = logn - LOGM;
+  const int h = blockIdx.x & ((1 << hbits) - 1);
+  const int row = blockIdx.x >> hbits;
+  const PrimeDev& P = primes[lp.id[row % num_limbs]];
+  const A ar(P);
+  uint64_t* g = data + ((size_t)row << logn) + ((size_t)h << LOGM);
+  const int t = threadIdx.x;
+  V x[E];
+  // First pass reads straight from HBM: element e of thread t is word e*T+t.
+#pragma unroll
+  for (int e = 0; e < E; ++e) x[e] = ar.load(g[e * T + t]);
+  if (hbits > 0) {
+    // Streaming stages already applied growth; bring values back in range.
+#pragma unroll
+    for (int e = 0; e < E; ++e) ar.fwd_reduce(x[e]);
+  }
+  FwdPasses<LOGM, LOGE, 0, A>::run(x, lds, t, h, hbits, ar);
+  // Coalesced store from LDS.
+#pragma unroll
+  for (int e = 0; e < E; ++e) g[e * T + t] = ar.fwd_out(lds[lpad(e * T + t)]);
+}
+
+template <int LOGM, int LOGE, class A>
+__global__ void __launch_bounds__((1 << LOGM) >> LOGE)
+    ntt_inv_kernel(uint64_t* __restrict__ data, const PrimeDev* __restrict__ primes,
+                   LimbPrimes lp, int num_limbs, int logn, int scale) {
+  using V = typename A::V;
+  constexpr int E = 1 << LOGE;
+  constexpr int T = (1 << LOGM) >> LOGE;
+  extern __shared__ __align__(16) unsigned char smem[];
+  V* lds = reinterpret_cast<V*>(smem);
+  const int hbits = logn - LOGM;
+  const int h = blockIdx.x & ((1 << hbits) - 1);
+  const int row = blockIdx.x >> hbits;
+  const PrimeDev& P = primes[lp.id[row % num_limbs]];
+  const A ar(P);
+  uint64_t* g = data + ((size_t)row << logn) + ((size_t)h << LOGM);
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int e = 0; e < E; ++e) lds[lpad(e * T + t)] = ar.load(g[e * T + t]);
+  __syncthreads();
+  V x[E];
+  InvPasses<LOGM, LOGE, 0, A>::run(x, lds, t, h, logn, ar);
+  // The last pass toggles the top local bits: word (e << I0) | gamma.
+  constexpr int I0 = InvLast<LOGM, LOGE, 0, A>::value;
+  constexpr int R = LOGM - I0;
+  constexpr int G = E >> R;
+#pragma unroll
+  for (int gg = 0; gg < G; ++gg) {
+#pragma unroll
+    for (int e = 0; e < (1 << R); ++e) {
+      const V v = x[gg * (1 << R) + e];
+      g[(e << I0) | (gg * T + t)] = scale ? ar.inv_out(v) : ar.fwd_out(v);
+    }
+  }
+}
+
+// Streaming radix-2 stage over HBM for limbs larger than the LDS slice:
+// forward stage s (pairs at distance n/2^(s+1)), or inverse stage i.
+// Integer values are kept in [0, 2q); F64 values are kept balanced.
+template <class A, bool INVERSE>
+__global__ void ntt_stream_stage(uint64_t* __restrict__ data, const PrimeDev* __restrict__ primes,
+                                 LimbPrimes lp, int num_limbs, int logn, int stage,
+                                 long long total_pairs) {
+  const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= total_pairs) return;
+  const int half_pairs_log = logn - 1;
+  const long long row = gid >> half_pairs_log;
+  const int p = (int)(gid & ((1 << half_pairs_log) - 1));
+  const PrimeDev& P = primes[lp.id[row % num_limbs]];
+  const A ar(P);
+  uint64_t* g = data + ((size_t)row << logn);
+  const int bit = INVERSE ? stage : (logn - 1 - stage);
+  const int lo = p & ((1 << bit) - 1);
+  const int hi = p >> bit;
+  const int i0 = (hi << (bit + 1)) | lo;
+  const int i1 = i0 | (1 << bit);
+  typename A::V u = ar.load(g[i0]);
+  typename A::V v = ar.load(g[i1]);
+  if (!INVERSE) {
+    const auto w = ar.fwd_tw((1 << stage) + hi);
+    ar.ct(u, v, w);
+    g[i0] = ar.fwd_out(u);
+    g[i1] = ar.fwd_out(v);
+  } else {
+    const int n = 1 << logn;
+    const auto w = ar.inv_tw(n - (n >> stage) + hi);
+    ar.gs(u, v, w);
+    g[i0] = ar.fwd_out(u);
+    g[i1] = ar.fwd_out(v);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------
+
+template <int LOGM, int LOGE, class A>
+static hipError_t launch_fused(bool inverse, uint64_t* data, const PrimeDev* primes,
+                               const LimbPrimes& lp, int rows, int num_limbs, int logn,
+                               int scale, hipStream_t stream) {
+  constexpr int T = (1 << LOGM) >> LOGE;
+  const size_t shmem = (size_t)lds_words<LOGM>() * 8;
+  const unsigned grid = (unsigned)rows << (logn - LOGM);
+  if (!inverse) {
+    auto k = ntt_fwd_kernel<LOGM, LOGE, A>;
+    if (shmem > 65536)
+      (void)hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
+    hipLaunchKernelGGL(k, dim3(grid), dim3(T), shmem, stream, data, primes, lp, num_limbs, logn);
+  } else {
+    auto k = ntt_inv_kernel<LOGM, LOGE, A>;
+    if (shmem > 65536)
+      (void)hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
+    hipLaunchKernelGGL(k, dim3(grid), dim3(T), shmem, stream, data, primes, lp, num_limbs, logn,
+                       scale);
+  }
+  return hipGetLastError();
+}
+
+template <class A>
+static hipError_t launch_stream(bool inverse, uint64_t* data, const PrimeDev* primes,
+                                const LimbPrimes& lp, int rows, int num_limbs, int logn,
+                                int stage, hipStream_t stream) {
+  const long long pairs = (long long)rows << (logn - 1);
+  const int block = 256;
+  const unsigned grid = (unsigned)((pairs + block - 1) / block);
+  if (!inverse)
+    hipLaunchKernelGGL((ntt_stream_stage<A, false>), dim3(grid), dim3(block), 0, stream, data,
+                       primes, lp, num_limbs, logn, stage, pairs);
+  else
+    hipLaunchKernelGGL((ntt_stream_stage<A, true>), dim3(grid), dim3(block), 0, stream, data,
+                       primes, lp, num_limbs, logn, stage, pairs);
+  return hipGetLastError();
+}
+
+template <class A>
+static hipError_t dispatch_fused(int logm, bool inverse, uint64_t* data, const PrimeDev* primes,
+                                 const LimbPrimes& lp, int rows, int num_limbs, int logn, int scale,
+                                 hipStream_t s) {
+  switch (logm) {
+    case 1: return launch_fused<1, 1, A>(inverse, data, primes, lp, rows, num_limbs, logn, scale, s);
+    case 2: return launch_fused<2, 2, A>(inverse, data, primes, lp, rows, num_limbs, logn, scale, s);
+    case 3: return launch_fused<3, 3, A>(inverse, data, primes, lp, rows, num_limbs, logn, scale, s);
+    case 4: return launch_fused<4, 4, A>(inverse, data, primes, lp, rows, num_limbs, logn, scale, s);
+    case 5: return launch_fused<5, 4, A>(inverse, data, primes, lp, rows, num_limbs, logn, scale, s);
+    case 6: return launch_fused<6, 4, A>(inverse, data, primes, lp, rows, num_limbs, logn, scale, s);
+    case 7: return launch_fused<7, 4, A>(inverse, data, primes, lp, rows, num_limbs, logn, scale, s);
+    case 8: return launch_fused<8, 4, A>(inverse, data, primes, lp, rows, num_limbs, logn, scale, s);
+    case 9: return launch_fused<9, 4, A>(inverse, data, primes, lp, rows, num_limbs, logn, scale, s);
+    case 10: return launch_fused<10, 4, A>(inverse, data, primes, lp, rows, num_limbs, logn, scale, s);
+    case 11: return launch_fused<11, 4, A>(inverse, data, primes, lp, rows, num_limbs, logn, scale, s);
+    case 12: return launch_fused<12, 4, A>(inverse, data, primes, lp, rows, num_limbs, logn, scale, s);
+    case 13: return launch_fused<13, 4, A>(inverse, data, primes, lp, rows, num_limbs, logn, scale, s);
+    case 14: return launch_fused<14, 4, A>(inverse, data, primes, lp, rows, num_limbs, logn, scale, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+template <class A>
+static hipError_t run_ntt(bool inverse, uint64_t* data, const PrimeDev* primes,
+                          const LimbPrimes& lp, int rows, int num_limbs, int logn, int max_logm,
+                          hipStream_t s) {
+  const int logm = logn < max_logm ? logn : max_logm;
+  hipError_t e = hipSuccess;
+  if (!inverse) {
+    for (int st = 0; st < logn - logm && e == hipSuccess; ++st)
+      e = launch_stream<A>(false, data, primes, lp, rows, num_limbs, logn, st, s);
+    if (e == hipSuccess) e = dispatch_fused<A>(logm, false, data, primes, lp, rows, num_limbs, logn, 1, s);
+  } else {
+    e = dispatch_fused<A>(logm, true, data, primes, lp, rows, num_limbs, logn,
+                          logn == logm ? 1 : 0, s);
+    for (int st = logm; st < logn && e == hipSuccess; ++st) {
+      e = launch_stream<A>(true, data, primes, lp, rows, num_limbs, logn, st, s);
+    }
+    if (e == hipSuccess && logn > logm) e = launch_scale_ninv(data, primes, lp, rows, num_limbs, logn, s);
+  }
+  return e;
+}
+
+hipError_t launch_ntt(bool inverse, int variant, uint64_t* data, const PrimeDev* primes,
+                      const LimbPrimes& lp, int rows, int num_limbs, int logn, int max_logm,
+                      hipStream_t s) {
+  if (variant == SHG_NTT_F64)
+    return run_ntt<F64Arith>(inverse, data, primes, lp, rows, num_limbs, logn, max_logm, s);
+  return run_ntt<IntArith>(inverse, data, primes, lp, rows, num_limbs, logn, max_logm, s);
+}
+
+}  // namespace shg

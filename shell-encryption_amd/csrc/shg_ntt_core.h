@@ -1,0 +1,302 @@
+// Register/LDS building blocks of the negacyclic NTT on gfx950, shared by the
+// transform kernels and the fused key-switch kernel.
+//
+// Semantics: ForwardNumberTheoreticTransform (Cooley-Tukey, natural order in,
+// bit-reversed order out, twiddle psis_bitrev[2^s + block] at stage s) and
+// InverseNumberTheoreticTransform (Gentleman-Sande, twiddle
+// psis_inv_bitrev[n - n/2^i + block] at stage i, then times n^-1)
+// (shell_encryption/dft_transformations.h:59-201).
+//
+// A workgroup owns a 2^LOGM-word slice in LDS; each thread keeps E = 2^LOGE
+// words in VGPRs and runs up to LOGE radix-2 stages on them between LDS
+// exchanges.  Two arithmetic policies, bit-exact with each other:
+//  * IntArith: 64-bit Shoup products composed from 32-bit mul_lo/mul_hi;
+//    values grow lazily by < 2q per stage (q < 2^50 leaves 14 spare bits),
+//    one Barrett reduction at the end.
+//  * F64Arith: exact products from FP64 FMA (q < 2^50), values kept balanced
+//    and reduced every few stages.
+#ifndef SHG_NTT_CORE_H_
+#define SHG_NTT_CORE_H_
+
+#include <hip/hip_runtime.h>
+
+#include "shg_device.h"
+
+namespace shg {
+
+
+// One padding word per 16 keeps strided u64 LDS accesses spread over banks.
+__device__ __forceinline__ int lpad(int i) { return i + (i >> 4); }
+
+template <int LOGM>
+constexpr int lds_words() {
+  return (1 << LOGM) + ((1 << LOGM) >> 4);
+}
+
+// ---------------------------------------------------------------------------
+// Arithmetic policies
+// ---------------------------------------------------------------------------
+
+struct IntArith {
+  using V = uint64_t;
+  uint64_t q, q2, bn;
+  const uint64_t* __restrict__ fw;
+  const uint64_t* __restrict__ iw;
+  uint64_t ninv_c, ninv_cb;
+  // Lazy growth is < 2q per CT stage, so up to 15 stages stay below 2^64 for
+  // q < 2^58; wider moduli (up to the reference's 2^62 limit) are reduced
+  // after every stage.
+  int fwd_reduce_every, inv_reduce_every;
+  __device__ __forceinline__ explicit IntArith(const PrimeDev& P)
+      : q(P.q), q2(2 * P.q), bn(P.bn), fw(P.fw), iw(P.iw), ninv_c(P.n_inv_c),
+        ninv_cb(P.n_inv_cb), fwd_reduce_every(P.q < (1ull << 58) ? 64 : 1),
+        inv_reduce_every(64) {}
+  struct Tw {
+    uint64_t c, cb;
+  };
+  __device__ __forceinline__ Tw fwd_tw(int i) const { return Tw{fw[2 * i], fw[2 * i + 1]}; }
+  __device__ __forceinline__ Tw inv_tw(int i) const { return Tw{iw[2 * i], iw[2 * i + 1]}; }
+  __device__ __forceinline__ V load(uint64_t x) const { return x; }
+  // Cooley-Tukey: values grow by less than 2q per stage.
+  __device__ __forceinline__ void ct(V& u, V& v, const Tw& w) const {
+    const uint64_t t = shoup_lazy(v, w.c, w.cb, q);
+    v = u + q2 - t;
+    u = u + t;
+  }
+  // Gentleman-Sande on [0, 2q) inputs, outputs in [0, 2q).
+  __device__ __forceinline__ void gs(V& u, V& v, const Tw& w) const {
+    const uint64_t s = u + v;
+    const uint64_t d = u + q2 - v;
+    u = csub(s, q2);
+    v = shoup_lazy(d, w.c, w.cb, q);
+  }
+  __device__ __forceinline__ void fwd_reduce(V& x) const { x = reduce_full(x, q, bn); }
+  __device__ __forceinline__ void inv_reduce(V&) const {}
+  __device__ __forceinline__ uint64_t fwd_out(V x) const { return reduce_full(x, q, bn); }
+  __device__ __forceinline__ uint64_t inv_out(V x) const {
+    return csub(shoup_lazy(x, ninv_c, ninv_cb, q), q);
+  }
+};
+
+struct F64Arith {
+  using V = double;
+  double q, qinv, ninv, ninvq;
+  const double* __restrict__ fw;
+  const double* __restrict__ iw;
+  __device__ __forceinline__ explicit F64Arith(const PrimeDev& P)
+      : q(P.qd), qinv(P.qinv), ninv(P.n_inv_d), ninvq(P.n_inv_dq), fw(P.fwd), iw(P.iwd) {}
+  struct Tw {
+    double w, wq;
+  };
+  __device__ __forceinline__ Tw fwd_tw(int i) const { return Tw{fw[2 * i], fw[2 * i + 1]}; }
+  __device__ __forceinline__ Tw inv_tw(int i) const { return Tw{iw[2 * i], iw[2 * i + 1]}; }
+  __device__ __forceinline__ V load(uint64_t x) const { return u64_to_f64(x); }
+  __device__ __forceinline__ void ct(V& u, V& v, const Tw& w) const {
+    const double t = f64_mulmod(v, w.w, w.wq, q);
+    v = u - t;
+    u = u + t;
+  }
+  __device__ __forceinline__ void gs(V& u, V& v, const Tw& w) const {
+    const double s = u + v;
+    const double d = u - v;
+    u = s;
+    v = f64_mulmod(d, w.w, w.wq, q);
+  }
+  __device__ __forceinline__ void fwd_reduce(V& x) const { x = f64_reduce(x, q, qinv); }
+  __device__ __forceinline__ void inv_reduce(V& x) const { x = f64_reduce(x, q, qinv); }
+  __device__ __forceinline__ uint64_t fwd_out(V x) const {
+    return f64_to_u64(f64_canon(f64_reduce(x, q, qinv), q));
+  }
+  __device__ __forceinline__ uint64_t inv_out(V x) const {
+    return f64_to_u64(f64_canon(f64_reduce(f64_mulmod(x, ninv, ninvq, q), q, qinv), q));
+  }
+  // Bounds (q < 2^50, |x| <= q/2 after a reduction): four CT stages keep
+  // |x| < 5.4 q < 2^52.5; two GS stages keep |x| < 2^52.
+  static constexpr int fwd_reduce_every = 4;
+  static constexpr int inv_reduce_every = 2;
+};
+
+// ---------------------------------------------------------------------------
+// Register passes.  A pass covers R consecutive stages; a thread holds
+// G = E / 2^R groups of 2^R words; group gamma = gg*T + t.
+// ---------------------------------------------------------------------------
+
+// Forward pass: stages S0..S0+R-1 (global stage s toggles index bit
+// LOGN-1-s).  The slice is sub-block `h` of a limb with LOGN bits; local
+// indices have LOGM bits, so the twiddle block carries h in its top bits.
+template <int LOGM, int LOGE, int S0, int R, class A>
+__device__ __forceinline__ void fwd_pass(typename A::V (&x)[1 << LOGE], int t, int h,
+                                         int hbits, const A& ar) {
+  constexpr int E = 1 << LOGE;
+  constexpr int T = (1 << LOGM) >> LOGE;
+  constexpr int G = E >> R;
+  constexpr int PLO = LOGM - S0 - R;
+#pragma unroll
+  for (int j = 0; j < R; ++j) {
+    const int sl = S0 + j;            // local stage
+    const int s = sl + hbits;         // global stage
+    const int eb = R - 1 - j;
+#pragma unroll
+    for (int gg = 0; gg < G; ++gg) {
+      const int gam = gg * T + t;
+      const int hi = ((h << S0) | (gam >> PLO));
+#pragma unroll
+      for (int bh = 0; bh < (1 << j); ++bh) {
+        const auto w = ar.fwd_tw((1 << s) + ((hi << j) | bh));
+#pragma unroll
+        for (int el = 0; el < (1 << eb); ++el) {
+          const int e = (bh << (eb + 1)) | el;
+          ar.ct(x[gg * (1 << R) + e], x[gg * (1 << R) + e + (1 << eb)], w);
+        }
+      }
+    }
+    if (((s + 1) % ar.fwd_reduce_every) == 0) {
+#pragma unroll
+      for (int i = 0; i < E; ++i) ar.fwd_reduce(x[i]);
+    }
+  }
+}
+
+template <int LOGM, int LOGE, int S0, int R, class V>
+__device__ __forceinline__ void fwd_to_lds(const V (&x)[1 << LOGE], V* lds, int t) {
+  constexpr int T = (1 << LOGM) >> LOGE;
+  constexpr int G = (1 << LOGE) >> R;
+  constexpr int PLO = LOGM - S0 - R;
+#pragma unroll
+  for (int gg = 0; gg < G; ++gg) {
+    const int gam = gg * T + t;
+    const int lo = gam & ((1 << PLO) - 1);
+    const int hi = gam >> PLO;
+#pragma unroll
+    for (int e = 0; e < (1 << R); ++e)
+      lds[lpad((hi << (PLO + R)) | (e << PLO) | lo)] = x[gg * (1 << R) + e];
+  }
+}
+
+template <int LOGM, int LOGE, int S0, int R, class V>
+__device__ __forceinline__ void fwd_from_lds(V (&x)[1 << LOGE], const V* lds, int t) {
+  constexpr int T = (1 << LOGM) >> LOGE;
+  constexpr int G = (1 << LOGE) >> R;
+  constexpr int PLO = LOGM - S0 - R;
+#pragma unroll
+  for (int gg = 0; gg < G; ++gg) {
+    const int gam = gg * T + t;
+    const int lo = gam & ((1 << PLO) - 1);
+    const int hi = gam >> PLO;
+#pragma unroll
+    for (int e = 0; e < (1 << R); ++e)
+      x[gg * (1 << R) + e] = lds[lpad((hi << (PLO + R)) | (e << PLO) | lo)];
+  }
+}
+
+// Runs forward passes S0.. to LOGM-1 (recursive over passes).
+template <int LOGM, int LOGE, int S0, class A>
+struct FwdPasses {
+  static constexpr int R = (LOGM - S0) < LOGE ? (LOGM - S0) : LOGE;
+  __device__ __forceinline__ static void run(typename A::V (&x)[1 << LOGE], typename A::V* lds,
+                                             int t, int h, int hbits, const A& ar) {
+    if constexpr (S0 > 0) {
+      fwd_from_lds<LOGM, LOGE, S0, R>(x, lds, t);
+      __syncthreads();
+    }
+    fwd_pass<LOGM, LOGE, S0, R>(x, t, h, hbits, ar);
+    fwd_to_lds<LOGM, LOGE, S0, R>(x, lds, t);
+    __syncthreads();
+    if constexpr (S0 + R < LOGM) FwdPasses<LOGM, LOGE, S0 + R, A>::run(x, lds, t, h, hbits, ar);
+  }
+};
+
+// Inverse pass: stages I0..I0+R-1 (stage i toggles local bit i).  Local
+// stage i of sub-block h is global stage i with block index carrying h.
+template <int LOGM, int LOGE, int I0, int R, class A>
+__device__ __forceinline__ void inv_pass(typename A::V (&x)[1 << LOGE], int t, int h, int logn,
+                                         const A& ar) {
+  constexpr int E = 1 << LOGE;
+  constexpr int T = (1 << LOGM) >> LOGE;
+  constexpr int G = E >> R;
+  const int n = 1 << logn;
+#pragma unroll
+  for (int j = 0; j < R; ++j) {
+    const int i = I0 + j;
+#pragma unroll
+    for (int gg = 0; gg < G; ++gg) {
+      const int gam = gg * T + t;
+      const int hi = (h << (LOGM - I0 - R)) | (gam >> I0);
+#pragma unroll
+      for (int bh = 0; bh < (1 << (R - 1 - j)); ++bh) {
+        const int block = (hi << (R - 1 - j)) | bh;
+        const auto w = ar.inv_tw(n - (n >> i) + block);
+#pragma unroll
+        for (int el = 0; el < (1 << j); ++el) {
+          const int e = (bh << (j + 1)) | el;
+          ar.gs(x[gg * (1 << R) + e], x[gg * (1 << R) + e + (1 << j)], w);
+        }
+      }
+    }
+    if (((i + 1) % ar.inv_reduce_every) == 0) {
+#pragma unroll
+      for (int k = 0; k < E; ++k) ar.inv_reduce(x[k]);
+    }
+  }
+}
+
+template <int LOGM, int LOGE, int I0, int R, class V>
+__device__ __forceinline__ void inv_to_lds(const V (&x)[1 << LOGE], V* lds, int t) {
+  constexpr int T = (1 << LOGM) >> LOGE;
+  constexpr int G = (1 << LOGE) >> R;
+#pragma unroll
+  for (int gg = 0; gg < G; ++gg) {
+    const int gam = gg * T + t;
+    const int lo = gam & ((1 << I0) - 1);
+    const int hi = gam >> I0;
+#pragma unroll
+    for (int e = 0; e < (1 << R); ++e)
+      lds[lpad((hi << (I0 + R)) | (e << I0) | lo)] = x[gg * (1 << R) + e];
+  }
+}
+
+template <int LOGM, int LOGE, int I0, int R, class V>
+__device__ __forceinline__ void inv_from_lds(V (&x)[1 << LOGE], const V* lds, int t) {
+  constexpr int T = (1 << LOGM) >> LOGE;
+  constexpr int G = (1 << LOGE) >> R;
+#pragma unroll
+  for (int gg = 0; gg < G; ++gg) {
+    const int gam = gg * T + t;
+    const int lo = gam & ((1 << I0) - 1);
+    const int hi = gam >> I0;
+#pragma unroll
+    for (int e = 0; e < (1 << R); ++e)
+      x[gg * (1 << R) + e] = lds[lpad((hi << (I0 + R)) | (e << I0) | lo)];
+  }
+}
+
+// Inverse passes from I0; the last pass is left in registers.
+template <int LOGM, int LOGE, int I0, class A>
+struct InvPasses {
+  static constexpr int R = (LOGM - I0) < LOGE ? (LOGM - I0) : LOGE;
+  __device__ __forceinline__ static void run(typename A::V (&x)[1 << LOGE], typename A::V* lds,
+                                             int t, int h, int logn, const A& ar) {
+    inv_from_lds<LOGM, LOGE, I0, R>(x, lds, t);
+    __syncthreads();
+    inv_pass<LOGM, LOGE, I0, R>(x, t, h, logn, ar);
+    if constexpr (I0 + R < LOGM) {
+      inv_to_lds<LOGM, LOGE, I0, R>(x, lds, t);
+      __syncthreads();
+      InvPasses<LOGM, LOGE, I0 + R, A>::run(x, lds, t, h, logn, ar);
+    }
+  }
+};
+
+template <int LOGM, int LOGE, int I0, class A>
+struct InvLast {
+  static constexpr int R = (LOGM - I0) < LOGE ? (LOGM - I0) : LOGE;
+  static constexpr int value = (I0 + R >= LOGM) ? I0 : InvLast<LOGM, LOGE, I0 + R, A>::value;
+};
+template <int LOGM, int LOGE, class A>
+struct InvLast<LOGM, LOGE, LOGM, A> {
+  static constexpr int value = LOGM;
+};
+
+}  // namespace shg
+
+#endif  // SHG_NTT_CORE_H_

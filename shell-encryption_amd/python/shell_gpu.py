@@ -1,0 +1,247 @@
+"""ctypes binding of libshellgpu.so (include/shell_gpu.h) for the tests, the
+smoke check and the benchmark.
+
+The binding fails loudly when the library is missing: there is no CPU
+fallback anywhere on this path.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "libshellgpu.so")
+
+U64P = ctypes.POINTER(ctypes.c_uint64)
+INTP = ctypes.POINTER(ctypes.c_int)
+VP = ctypes.c_void_p
+
+OP_ADD, OP_SUB, OP_MUL, OP_NEG, OP_FMA = 0, 1, 2, 3, 4
+NTT_AUTO, NTT_INT, NTT_F64 = 0, 1, 2
+
+_lib = None
+
+
+class ShgError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("shell_gpu error %d: %s" % (code, msg))
+        self.code = code
+        self.msg = msg
+
+
+class LimbJob(ctypes.Structure):
+    _fields_ = [("data", VP), ("other", VP), ("prime", ctypes.c_int), ("log_n", ctypes.c_int)]
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("libshellgpu.so is not built (%s); run __graft_entry__.build()" % LIB_PATH)
+    L = ctypes.CDLL(LIB_PATH)
+    i = ctypes.c_int
+    sz = ctypes.c_size_t
+    sig = {
+        "shg_create": [i, ctypes.POINTER(VP)],
+        "shg_destroy": [VP],
+        "shg_set_stream": [VP, VP],
+        "shg_sync": [VP],
+        "shg_alloc": [VP, sz, ctypes.POINTER(VP)],
+        "shg_free": [VP, VP],
+        "shg_upload": [VP, VP, VP, sz],
+        "shg_download": [VP, VP, VP, sz],
+        "shg_prime_create": [VP, ctypes.c_uint64, i, INTP],
+        "shg_prime_info": [VP, i, U64P, U64P],
+        "shg_ntt_fwd": [VP, VP, i, i, INTP, i],
+        "shg_ntt_inv": [VP, VP, i, i, INTP, i],
+        "shg_pointwise": [VP, i, VP, VP, sz, VP, sz, i, i, INTP],
+        "shg_mul_scalar": [VP, VP, U64P, i, i, INTP],
+        "shg_mul_int": [VP, VP, ctypes.c_uint64, i, i, INTP],
+        "shg_substitute": [VP, VP, VP, i, i, i, INTP],
+        "shg_mod_reduce_lsb": [VP, VP, ctypes.c_uint64, U64P, i, i, i, INTP],
+        "shg_tensor_product": [VP, VP, i, VP, i, VP, i, i, INTP],
+        "shg_relin_apply": [VP, VP, i, i, VP, VP, i, INTP, VP, VP, i, i, INTP],
+        "shg_galois_apply": [VP, VP, i, VP, VP, INTP, VP, VP, i, i, INTP],
+        "shg_gadget_decompose": [VP, VP, INTP, VP, i, i, INTP],
+        "shg_lazy_fma": [VP, VP, sz, VP, sz, i, VP, i, INTP],
+        "shg_batch_limbs": [VP, ctypes.POINTER(LimbJob), i, i],
+        "shg_autotune_ntt": [VP, i, i, INTP],
+    }
+    for name, args in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = ctypes.c_int
+    L.shg_last_error.restype = ctypes.c_char_p
+    L.shg_get_stream.argtypes = [VP]
+    L.shg_get_stream.restype = VP
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc != 0:
+        raise ShgError(rc, lib().shg_last_error().decode())
+
+
+def _ints(v):
+    return (ctypes.c_int * len(v))(*[int(x) for x in v])
+
+
+class DeviceArray:
+    """A device buffer of uint64 words with a numpy-like shape."""
+
+    def __init__(self, ctx, shape):
+        self.ctx = ctx
+        self.shape = tuple(int(s) for s in shape)
+        self.size = int(np.prod(self.shape)) if self.shape else 1
+        self.nbytes = self.size * 8
+        p = VP()
+        _check(lib().shg_alloc(ctx.h, self.nbytes, ctypes.byref(p)))
+        self.ptr = p.value
+
+    @property
+    def p(self):
+        return VP(self.ptr)
+
+    def offset(self, words):
+        return VP(self.ptr + 8 * int(words))
+
+    def upload(self, arr):
+        a = np.ascontiguousarray(arr, dtype=np.uint64)
+        assert a.size == self.size, (a.shape, self.shape)
+        _check(lib().shg_upload(self.ctx.h, self.p, a.ctypes.data_as(VP), self.nbytes))
+        return self
+
+    def download(self):
+        out = np.empty(self.shape, dtype=np.uint64)
+        _check(lib().shg_download(self.ctx.h, out.ctypes.data_as(VP), self.p, self.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib().shg_free(self.ctx.h, self.p)
+            self.ptr = 0
+
+    def __del__(self):
+        try:
+            if self.ptr and self.ctx.h:
+                lib().shg_free(self.ctx.h, VP(self.ptr))
+                self.ptr = 0
+        except Exception:
+            pass
+
+
+class Context:
+    def __init__(self, device=0):
+        h = VP()
+        _check(lib().shg_create(int(device), ctypes.byref(h)))
+        self.h = h.value
+        self._primes = {}
+
+    def close(self):
+        if self.h:
+            lib().shg_destroy(VP(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def sync(self):
+        _check(lib().shg_sync(self.h))
+
+    def array(self, arr):
+        a = np.ascontiguousarray(arr, dtype=np.uint64)
+        return DeviceArray(self, a.shape).upload(a)
+
+    def empty(self, shape):
+        return DeviceArray(self, shape)
+
+    def prime(self, q, log_n):
+        key = (int(q), int(log_n))
+        if key not in self._primes:
+            pid = ctypes.c_int()
+            _check(lib().shg_prime_create(self.h, int(q), int(log_n), ctypes.byref(pid)))
+            self._primes[key] = pid.value
+        return self._primes[key]
+
+    def primes(self, qs, log_n):
+        return [self.prime(q, log_n) for q in qs]
+
+    def prime_info(self, pid, log_n):
+        params = np.zeros(10, dtype=np.uint64)
+        tables = np.zeros(4 << log_n, dtype=np.uint64)
+        _check(lib().shg_prime_info(self.h, int(pid), params.ctypes.data_as(U64P),
+                                    tables.ctypes.data_as(U64P)))
+        return params, tables
+
+    # ---- ops ----------------------------------------------------------------
+    def ntt_fwd(self, buf, count, primes, variant=NTT_AUTO, offset=0):
+        _check(lib().shg_ntt_fwd(self.h, buf.offset(offset), int(count), len(primes),
+                                 _ints(primes), int(variant)))
+
+    def ntt_inv(self, buf, count, primes, variant=NTT_AUTO, offset=0):
+        _check(lib().shg_ntt_inv(self.h, buf.offset(offset), int(count), len(primes),
+                                 _ints(primes), int(variant)))
+
+    def pointwise(self, op, x, a, b, count, primes, a_stride=None, b_stride=None):
+        words = len(primes) * (x.size // (count * len(primes)))
+        a_stride = words if a_stride is None else a_stride
+        b_stride = words if b_stride is None else b_stride
+        _check(lib().shg_pointwise(self.h, int(op), x.p, a.p if a is not None else None,
+                                   int(a_stride), b.p if b is not None else None, int(b_stride),
+                                   int(count), len(primes), _ints(primes)))
+
+    def mul_scalar(self, x, scalars, count, primes):
+        s = np.ascontiguousarray(scalars, dtype=np.uint64)
+        _check(lib().shg_mul_scalar(self.h, x.p, s.ctypes.data_as(U64P), int(count),
+                                    len(primes), _ints(primes)))
+
+    def mul_int(self, x, scalar, count, primes):
+        _check(lib().shg_mul_int(self.h, x.p, int(scalar), int(count), len(primes),
+                                 _ints(primes)))
+
+    def substitute(self, src, dst, power, count, primes):
+        _check(lib().shg_substitute(self.h, src.p, dst.p, int(power), int(count), len(primes),
+                                    _ints(primes)))
+
+    def mod_reduce_lsb(self, x, t, ql_inv, is_ntt, count, primes):
+        qi = np.ascontiguousarray(ql_inv, dtype=np.uint64)
+        _check(lib().shg_mod_reduce_lsb(self.h, x.p, int(t), qi.ctypes.data_as(U64P),
+                                        int(bool(is_ntt)), int(count), len(primes),
+                                        _ints(primes)))
+
+    def tensor_product(self, a, len_a, b, len_b, out, count, primes):
+        _check(lib().shg_tensor_product(self.h, a.p, int(len_a), b.p, int(len_b), out.p,
+                                        int(count), len(primes), _ints(primes)))
+
+    def relin_apply(self, comps, degree, ntt_mask, key_as, key_bs, num_key_polys, log_bs,
+                    out0, out1, count, primes):
+        _check(lib().shg_relin_apply(self.h, comps.p, int(degree), int(ntt_mask), key_as.p,
+                                     key_bs.p, int(num_key_polys), _ints(log_bs), out0.p,
+                                     out1.p, int(count), len(primes), _ints(primes)))
+
+    def galois_apply(self, comps, c1_is_ntt, key_as, key_bs, log_bs, out0, out1, count, primes):
+        _check(lib().shg_galois_apply(self.h, comps.p, int(bool(c1_is_ntt)), key_as.p,
+                                      key_bs.p, _ints(log_bs), out0.p, out1.p, int(count),
+                                      len(primes), _ints(primes)))
+
+    def gadget_decompose(self, a, log_bs, digits, count, primes):
+        _check(lib().shg_gadget_decompose(self.h, a.p, _ints(log_bs), digits.p, int(count),
+                                          len(primes), _ints(primes)))
+
+    def lazy_fma(self, as_, as_stride, bs, bs_stride, num_terms, out, primes):
+        _check(lib().shg_lazy_fma(self.h, as_.p, int(as_stride), bs.p, int(bs_stride),
+                                  int(num_terms), out.p, len(primes), _ints(primes)))
+
+    def batch_limbs(self, jobs, op):
+        arr = (LimbJob * len(jobs))(*[LimbJob(d, o, p, ln) for (d, o, p, ln) in jobs])
+        _check(lib().shg_batch_limbs(self.h, arr, len(jobs), int(op)))
+
+    def autotune_ntt(self, log_n, limbs):
+        c = ctypes.c_int()
+        _check(lib().shg_autotune_ntt(self.h, int(log_n), int(limbs), ctypes.byref(c)))
+        return c.value

@@ -1,0 +1,285 @@
+"""Bit-exact parity of the HIP path (through the C-ABI) against the CPU
+oracle on identical seeded inputs.  Every output word is compared."""
+import numpy as np
+import pytest
+
+import oracle
+import shell_gpu as sg
+from common import Bgv, negacyclic_mul_mod_t, ntt_primes, substitute_coeffs
+
+pytestmark = pytest.mark.gpu
+
+VARIANTS = [sg.NTT_INT, sg.NTT_F64]
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = sg.Context(0)
+    yield c
+    c.close()
+
+
+def _rns(log_n, L, log_bound=50):
+    qs = ntt_primes(log_n, L, log_bound)
+    return qs, oracle.Rns(qs, log_n)
+
+
+def test_prime_tables_match_oracle(ctx):
+    for log_n, L in [(3, 1), (12, 2), (14, 2)]:
+        qs, rns = _rns(log_n, L)
+        for q, p in zip(qs, rns.primes):
+            pid = ctx.prime(q, log_n)
+            params, tables = ctx.prime_info(pid, log_n)
+            tb = p.tables()
+            s = p.mont.s
+            assert list(params[:8]) == [s.modulus, s.log_modulus, s.r_mod_modulus,
+                                        s.r_mod_modulus_barrett, s.inv_modulus, s.inv_r,
+                                        s.inv_r_barrett, s.barrett_numerator]
+            assert params[8] == tb["psi"] and params[9] == tb["n_inv"]
+            n = 1 << log_n
+            assert np.array_equal(tables[:n], tb["psis_bitrev_c"])
+            assert np.array_equal(tables[n:2 * n], tb["psis_bitrev_cb"])
+            assert np.array_equal(tables[2 * n:3 * n], tb["psis_inv_bitrev_c"])
+            assert np.array_equal(tables[3 * n:], tb["psis_inv_bitrev_cb"])
+
+
+@pytest.mark.parametrize("variant", VARIANTS)
+@pytest.mark.parametrize("log_n,L,count", [(1, 1, 3), (2, 2, 2), (4, 1, 5), (5, 2, 3),
+                                           (8, 3, 2), (11, 2, 3), (12, 1, 4), (13, 4, 2),
+                                           (14, 8, 2), (15, 2, 2)])
+def test_ntt_round_trip_bit_exact(ctx, variant, log_n, L, count):
+    qs, rns = _rns(log_n, L)
+    pids = ctx.primes(qs, log_n)
+    x = rns.uniform(1000 + log_n, count)
+    # extreme residues: 0, 1, q-1 and q (the image of NegateInPlace(0))
+    for i, q in enumerate(qs):
+        x[0, i, :4] = [0, 1, q - 1, q][: min(4, rns.n)]
+    d = ctx.array(x)
+    ctx.ntt_fwd(d, count, pids, variant)
+    got = d.download()
+    exp = np.stack([rns.ntt_fwd(x[c]) for c in range(count)])
+    assert np.array_equal(got, exp)
+    ctx.ntt_inv(d, count, pids, variant)
+    back = d.download()
+    exp_back = np.stack([rns.ntt_inv(exp[c]) for c in range(count)])
+    assert np.array_equal(back, exp_back)
+    xr = x.copy()
+    for i, q in enumerate(qs):
+        xr[0, i, :4][xr[0, i, :4] == q] = 0
+    assert np.array_equal(back, xr)
+
+
+def test_ntt_wide_modulus_int_only(ctx):
+    """60-bit primes take the integer path (FP64 needs q < 2^50)."""
+    qs, rns = _rns(12, 2, log_bound=60)
+    pids = ctx.primes(qs, 12)
+    x = rns.uniform(77, 3)
+    d = ctx.array(x)
+    ctx.ntt_fwd(d, 3, pids, sg.NTT_AUTO)
+    assert np.array_equal(d.download(), np.stack([rns.ntt_fwd(v) for v in x]))
+    with pytest.raises(sg.ShgError):
+        ctx.ntt_fwd(d, 3, pids, sg.NTT_F64)
+
+
+@pytest.mark.parametrize("op", ["add", "sub", "mul", "neg", "fma"])
+def test_pointwise(ctx, op):
+    qs, rns = _rns(11, 3)
+    pids = ctx.primes(qs, 11)
+    cnt = 3
+    x, a, b = rns.uniform(1, cnt), rns.uniform(2, cnt), rns.uniform(3, cnt)
+    x[0, :, 0] = 0
+    a[0, :, 1] = [q for q in qs]  # the Negate(0) = q quirk as an operand
+    dx, da, db = ctx.array(x), ctx.array(a), ctx.array(b)
+    opc = {"add": sg.OP_ADD, "sub": sg.OP_SUB, "mul": sg.OP_MUL, "neg": sg.OP_NEG,
+           "fma": sg.OP_FMA}[op]
+    ctx.pointwise(opc, dx, da, db, cnt, pids)
+    exp = np.stack([rns.pointwise(op, x[c], a[c], b[c]) for c in range(cnt)])
+    assert np.array_equal(dx.download(), exp)
+    if op == "neg":
+        assert np.all(dx.download()[0, :, 0] == np.array(qs, dtype=np.uint64))
+
+
+def test_pointwise_broadcast_and_scalars(ctx):
+    qs, rns = _rns(10, 2)
+    pids = ctx.primes(qs, 10)
+    x, a = rns.uniform(5, 4), rns.uniform(6, 1)
+    dx, da = ctx.array(x), ctx.array(a)
+    ctx.pointwise(sg.OP_MUL, dx, da, None, 4, pids, a_stride=0)
+    exp = np.stack([rns.pointwise("mul", x[c], a[0]) for c in range(4)])
+    assert np.array_equal(dx.download(), exp)
+    sc = np.array([rns.primes[i].mont.imp(12345 + i) for i in range(2)], dtype=np.uint64)
+    ctx.mul_scalar(dx, sc, 4, pids)
+    exp = np.stack([rns.mul_scalar(exp[c], sc) for c in range(4)])
+    assert np.array_equal(dx.download(), exp)
+    ctx.mul_int(dx, 65537, 4, pids)
+    exp = np.stack([rns.mul_int(exp[c], 65537) for c in range(4)])
+    assert np.array_equal(dx.download(), exp)
+
+
+@pytest.mark.parametrize("log_n,power", [(5, 1), (5, 63), (11, 5), (14, 5), (14, 2 * 16384 - 1)])
+def test_substitute(ctx, log_n, power):
+    qs, rns = _rns(log_n, 3)
+    pids = ctx.primes(qs, log_n)
+    x = rns.uniform(9, 2)
+    src, dst = ctx.array(x), ctx.empty(x.shape)
+    ctx.substitute(src, dst, power, 2, pids)
+    exp = np.stack([rns.substitute(x[c], power) for c in range(2)])
+    assert np.array_equal(dst.download(), exp)
+    with pytest.raises(sg.ShgError):
+        ctx.substitute(src, dst, 2, 2, pids)
+
+
+@pytest.mark.parametrize("is_ntt", [True, False])
+@pytest.mark.parametrize("log_n,L", [(9, 3), (14, 8)])
+def test_mod_reduce_lsb(ctx, is_ntt, log_n, L):
+    qs, rns = _rns(log_n, L)
+    pids = ctx.primes(qs, log_n)
+    t = 65537
+    x = rns.uniform(21, 2)
+    d = ctx.array(x)
+    ql_inv = rns.ql_inv()
+    ctx.mod_reduce_lsb(d, t, ql_inv, is_ntt, 2, pids)
+    got = d.download()[:, : L - 1]
+    exp = np.stack([rns.mod_reduce_lsb(x[c], t, ql_inv, is_ntt) for c in range(2)])
+    assert np.array_equal(got, exp)
+
+
+def test_tensor_product(ctx):
+    qs, rns = _rns(13, 4)
+    pids = ctx.primes(qs, 13)
+    cnt = 3
+    a = np.stack([rns.uniform(100 + c, 2) for c in range(cnt)])
+    b = np.stack([rns.uniform(200 + c, 2) for c in range(cnt)])
+    da, db = ctx.array(a), ctx.array(b)
+    out = ctx.empty((cnt, 3, 4, rns.n))
+    ctx.tensor_product(da, 2, db, 2, out, cnt, pids)
+    exp = np.stack([rns.tensor_product(a[c], b[c]) for c in range(cnt)])
+    assert np.array_equal(out.download(), exp)
+
+
+def test_gadget_decompose(ctx):
+    qs, rns = _rns(10, 3)
+    pids = ctx.primes(qs, 10)
+    log_bs = [10, 7, 20]
+    k, _ = rns.gadget_dims(log_bs)
+    a = rns.uniform(4, 2)
+    da = ctx.array(a)
+    dg = ctx.empty((2, k, 3, rns.n))
+    ctx.gadget_decompose(da, log_bs, dg, 2, pids)
+    exp = np.stack([rns.gadget_decompose(a[c], log_bs) for c in range(2)])
+    assert np.array_equal(dg.download(), exp)
+
+
+@pytest.mark.parametrize("log_n,L,log_b,count", [(6, 2, 10, 3), (12, 3, 10, 2), (14, 8, 10, 2),
+                                                 (15, 2, 25, 1)])
+def test_relin_apply(ctx, log_n, L, log_b, count):
+    qs, rns = _rns(log_n, L)
+    pids = ctx.primes(qs, log_n)
+    log_bs = [log_b] * L
+    k, _ = rns.gadget_dims(log_bs)
+    comps = np.stack([rns.uniform(300 + c, 3) for c in range(count)])
+    kas, kbs = rns.uniform(1, k), rns.uniform(2, k)
+    dc, dka, dkb = ctx.array(comps), ctx.array(kas), ctx.array(kbs)
+    o0, o1 = ctx.empty((count, L, rns.n)), ctx.empty((count, L, rns.n))
+    ctx.relin_apply(dc, 2, 0b111, dka, dkb, k, log_bs, o0, o1, count, pids)
+    for c in range(count):
+        e0, e1 = rns.relin_apply(comps[c], kas, kbs, log_bs)
+        assert np.array_equal(o0.download()[c], e0)
+        assert np.array_equal(o1.download()[c], e1)
+
+
+def test_relin_degree3_and_coeff_form_component(ctx):
+    qs, rns = _rns(8, 2)
+    pids = ctx.primes(qs, 8)
+    log_bs = [25, 25]
+    k, _ = rns.gadget_dims(log_bs)
+    comps = rns.uniform(5, 4)[None]
+    kas, kbs = rns.uniform(6, 2 * k), rns.uniform(7, 2 * k)
+    mask = 0b1011  # component 2 in coefficient form
+    dc, dka, dkb = ctx.array(comps), ctx.array(kas), ctx.array(kbs)
+    o0, o1 = ctx.empty((1, 2, rns.n)), ctx.empty((1, 2, rns.n))
+    ctx.relin_apply(dc, 3, mask, dka, dkb, 2 * k, log_bs, o0, o1, 1, pids)
+    e0, e1 = rns.relin_apply(comps[0], kas, kbs, log_bs, ntt_mask=mask)
+    assert np.array_equal(o0.download()[0], e0) and np.array_equal(o1.download()[0], e1)
+
+
+@pytest.mark.parametrize("log_n,L,count", [(7, 2, 2), (14, 8, 2)])
+def test_galois_apply(ctx, log_n, L, count):
+    qs, rns = _rns(log_n, L)
+    pids = ctx.primes(qs, log_n)
+    log_bs = [10] * L
+    k, _ = rns.gadget_dims(log_bs)
+    comps = np.stack([rns.uniform(400 + c, 2) for c in range(count)])
+    kas, kbs = rns.uniform(11, k), rns.uniform(12, k)
+    dc, dka, dkb = ctx.array(comps), ctx.array(kas), ctx.array(kbs)
+    o0, o1 = ctx.empty((count, L, rns.n)), ctx.empty((count, L, rns.n))
+    ctx.galois_apply(dc, True, dka, dkb, log_bs, o0, o1, count, pids)
+    for c in range(count):
+        e0, e1 = rns.galois_apply(comps[c, 0], comps[c, 1], kas, kbs, log_bs)
+        assert np.array_equal(o0.download()[c], e0)
+        assert np.array_equal(o1.download()[c], e1)
+
+
+@pytest.mark.parametrize("terms", [1, 5, 64])
+def test_lazy_fma(ctx, terms):
+    qs, rns = _rns(12, 2)
+    pids = ctx.primes(qs, 12)
+    as_, bs = rns.uniform(31, terms), rns.uniform(32, terms)
+    da, db = ctx.array(as_), ctx.array(bs)
+    out = ctx.empty((2, rns.n))
+    words = 2 * rns.n
+    ctx.lazy_fma(da, words, db, words, terms, out, pids)
+    assert np.array_equal(out.download(), rns.lazy_fma(as_, bs))
+
+
+def test_bgv_chain_decrypts_on_gpu(ctx):
+    """Tensor product -> relinearize -> Galois x->x^5 -> mod switch on the
+    GPU, each step bit-exact against the oracle and the result decrypting to
+    the plaintext computation."""
+    log_n, L, t = 10, 4, 65537
+    qs, rns = _rns(log_n, L)
+    pids = ctx.primes(qs, log_n)
+    bgv = Bgv(rns, t, seed=3)
+    r = np.random.default_rng(4)
+    m1, m2 = r.integers(0, t, rns.n), r.integers(0, t, rns.n)
+    c1, c2 = bgv.encrypt(m1), bgv.encrypt(m2)
+    log_bs = [10] * L
+    k, _ = rns.gadget_dims(log_bs)
+    rka, rkb = bgv.relin_key(log_bs)
+    gka, gkb = bgv.galois_key(5, log_bs)
+    da, db = ctx.array(c1[None]), ctx.array(c2[None])
+    prod = ctx.empty((1, 3, L, rns.n))
+    ctx.tensor_product(da, 2, db, 2, prod, 1, pids)
+    exp_prod = rns.tensor_product(c1, c2)
+    assert np.array_equal(prod.download()[0], exp_prod)
+    r0, r1 = ctx.empty((1, L, rns.n)), ctx.empty((1, L, rns.n))
+    ctx.relin_apply(prod, 2, 0b111, ctx.array(rka), ctx.array(rkb), k, log_bs, r0, r1, 1, pids)
+    e0, e1 = rns.relin_apply(exp_prod, rka, rkb, log_bs)
+    assert np.array_equal(r0.download()[0], e0) and np.array_equal(r1.download()[0], e1)
+    relin = ctx.array(np.stack([e0, e1])[None])
+    sub = ctx.empty((1, 2, L, rns.n))
+    ctx.substitute(relin, sub, 5, 2, pids)
+    g0, g1 = ctx.empty((1, L, rns.n)), ctx.empty((1, L, rns.n))
+    ctx.galois_apply(sub, True, ctx.array(gka), ctx.array(gkb), log_bs, g0, g1, 1, pids)
+    gs = np.stack([g0.download()[0], g1.download()[0]])
+    expect = np.array(substitute_coeffs(negacyclic_mul_mod_t(m1, m2, t), 5, t), dtype=np.int64)
+    assert np.array_equal(bgv.decrypt(gs), expect)
+    ms = ctx.array(gs)
+    ctx.mod_reduce_lsb(ms, t, rns.ql_inv(), True, 2, pids)
+    red = rns.sub(L - 1)
+    out = ms.download()[:, : L - 1]
+    bgv_r = Bgv.__new__(Bgv)
+    bgv_r.rns, bgv_r.t, bgv_r.s = red, t, bgv.s[: L - 1].copy()
+    assert np.array_equal(bgv_r.decrypt(out), expect)
+
+
+def test_errors_leave_outputs_untouched(ctx):
+    qs, rns = _rns(8, 2)
+    pids = ctx.primes(qs, 8)
+    x = rns.uniform(1, 1)
+    d = ctx.array(x)
+    with pytest.raises(sg.ShgError):
+        ctx.mod_reduce_lsb(d, 65537, [], True, 1, pids[:1])
+    with pytest.raises(sg.ShgError):
+        ctx.ntt_fwd(d, 1, [pids[0], 999])
+    assert np.array_equal(d.download(), x)
