@@ -1,0 +1,254 @@
+"""Benchmark of the RNS key-switching hot path on one MI355X.
+
+A step is one batched relinearization of 512 degree-2 BGV ciphertexts at
+n = 2^14 with L = 8 primes (the largest below 2^50 with q = 1 mod 2^15) and
+gadget base 2^10 (k = 40 digits), with the relinearization key resident in
+HBM.  Inputs are synthetic seeded residues, already in HBM when the timed
+region starts.  The line also reports NTTs/s at (n = 2^14, L = 8), the plain
+per-call key-switch rate, Galois key-switch and modulus-switch rates, the
+per-step latency p50/p99, the roofline of the dominant kernel, and the CPU
+oracle timed on this host as a reported baseline.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "shell-encryption_amd", "python"))
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+HBM_PEAK_GBS = 8000.0
+LOG_N = 14
+L = 8
+LOG_B = 10
+BATCH = 512
+NTT_BATCH = 1024
+
+
+def primes_for(log_n, count, log_bound=50):
+    from common import ntt_primes
+    return ntt_primes(log_n, count, log_bound)
+
+
+def uniform(rng, qs, shape_prefix, n):
+    """Residues uniform in [0, q_i) for limb i; shape prefix + (L, n)."""
+    out = np.empty(tuple(shape_prefix) + (len(qs), n), dtype=np.uint64)
+    for i, q in enumerate(qs):
+        out[..., i, :] = rng.integers(0, q, size=tuple(shape_prefix) + (n,), dtype=np.uint64)
+    return out
+
+
+def percentile(v, p):
+    return float(np.percentile(np.asarray(v), p))
+
+
+def cpu_baseline(qs, comps, kas, kbs, log_bs, budget_s=12.0):
+    """The oracle's relinearization (a scalar restatement of the reference's
+    RnsRelinKey::ApplyTo) on one host core, over a bounded sample."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    rns = oracle.Rns(qs, LOG_N)
+    done, t0 = 0, time.perf_counter()
+    while True:
+        rns.relin_apply(comps[done % len(comps)], kas, kbs, log_bs)
+        done += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s or done >= 64:
+            break
+    return {"value": done / el, "unit": "key-switches/s", "cores": 1, "kind": "port",
+            "sample": "%d relinearizations of degree-2 ciphertexts at n=2^14, L=8, k=40 by the "
+                      "oracle (single-threaded C restatement), %.1f s" % (done, el)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--quick", action="store_true", help="skip the secondary measurements")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: every box here has one MI355X, and the "
+              "multi-GPU path (batch sharding, section on sharding in DESIGN.md) is not "
+              "built in this repository.", file=sys.stderr)
+        return 2
+
+    import shell_gpu as sg
+
+    rng = np.random.default_rng(20261203)
+    qs = primes_for(LOG_N, L)
+    n = 1 << LOG_N
+    log_bs = [LOG_B] * L
+    k = sum((q.bit_length() + LOG_B - 1) // LOG_B for q in qs)
+    poly_words = L * n
+
+    ctx = sg.Context(0)
+    pids = ctx.primes(qs, LOG_N)
+    print("[bench] generating inputs", file=sys.stderr, flush=True)
+    comps = uniform(rng, qs, (BATCH, 3), n)
+    kas = uniform(rng, qs, (k,), n)
+    kbs = uniform(rng, qs, (k,), n)
+    d_comps = ctx.array(comps)
+    d_ka, d_kb = ctx.array(kas), ctx.array(kbs)
+    d_o0, d_o1 = ctx.empty((BATCH, L, n)), ctx.empty((BATCH, L, n))
+
+    def relin_step():
+        ctx.relin_apply(d_comps, 2, 0b111, d_ka, d_kb, k, log_bs, d_o0, d_o1, BATCH, pids)
+
+    print("[bench] warmup", file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        relin_step()
+    ctx.sync()
+    ctx.profile(True)
+    ctx.profile_reset()
+    lat = []
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        relin_step()
+        ctx.sync()
+        lat.append((time.perf_counter() - t0) * 1e3)
+    elapsed = time.perf_counter() - t_start
+    ks_ms, ks_launches = ctx.profile_query("keyswitch")
+    intt_ms, intt_launches = ctx.profile_query("keyswitch_intt")
+    ctx.profile(False)
+    ms_per_step = elapsed * 1e3 / args.steps
+    value = BATCH / (elapsed / args.steps)
+    print("[bench] relin batched: %.3f ms/step, %.0f key-switches/s" % (ms_per_step, value),
+          file=sys.stderr, flush=True)
+
+    # Bit-exactness of two ciphertexts of the batch against the oracle.
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    rns = oracle.Rns(qs, LOG_N)
+    o0, o1 = d_o0.download(), d_o1.download()
+    bit_exact = True
+    for c in (0, BATCH - 1):
+        e0, e1 = rns.relin_apply(comps[c], kas, kbs, log_bs)
+        bit_exact &= bool(np.array_equal(o0[c], e0) and np.array_equal(o1[c], e1))
+
+    # Dominant kernel roofline: algorithmic bytes per ciphertext = coefficient-
+    # form input (L*n*8) + c0, c1 (2*L*n*8) + output (2*L*n*8) + the key
+    # (2*k*L*n*8) shared by the whole launch.
+    ks_avg_ms = ks_ms / max(ks_launches, 1)
+    ks_bytes = BATCH * 5 * poly_words * 8 + 2 * k * poly_words * 8
+    ks_gbs = ks_bytes / (ks_avg_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(ks_gbs, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(ks_gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "keyswitch_kernel<13,4,1,F64Arith>",
+                "avg_launch_ms": round(ks_avg_ms, 4),
+                "algorithmic_bytes_per_launch": ks_bytes}
+
+    out = {
+        "metric": "key-switches/s & NTTs/s at n=2^14,L=8 batched vs plain (bit-exact); p50/p99",
+        "value": round(value, 1), "unit": "key-switches/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u64",
+        "data": "synthetic (seeded uniform residues and key matrices)",
+        "config": {"workload": "BGV relinearization of 512 degree-2 ciphertexts, n=2^14, L=8 "
+                               "primes < 2^50, gadget base 2^10 (k=40), key resident in HBM",
+                   "n": n, "L": L, "log_b": LOG_B, "k": k, "batch": BATCH},
+        "latency_ms": {"p50": round(percentile(lat, 50), 4), "p99": round(percentile(lat, 99), 4)},
+        "bit_exact": bit_exact,
+        "roofline": roofline,
+        "keyswitch_intt_ms_per_step": round(intt_ms / max(intt_launches, 1), 4),
+    }
+
+    if not args.quick:
+        # NTTs/s: forward transform of NTT_BATCH RNS polynomials at (2^14, 8).
+        x = uniform(rng, qs, (NTT_BATCH,), n)
+        d_x = ctx.array(x)
+        for v in (sg.NTT_F64, sg.NTT_INT):
+            ctx.ntt_fwd(d_x, NTT_BATCH, pids, v)
+            ctx.ntt_inv(d_x, NTT_BATCH, pids, v)
+        ctx.sync()
+        res = {}
+        for name, v in (("f64", sg.NTT_F64), ("int", sg.NTT_INT)):
+            ctx.profile(True)
+            ctx.profile_reset()
+            reps = 10
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                ctx.ntt_fwd(d_x, NTT_BATCH, pids, v)
+            ctx.sync()
+            el = time.perf_counter() - t0
+            f_ms, f_n = ctx.profile_query("ntt_fwd")
+            for _ in range(reps):
+                ctx.ntt_inv(d_x, NTT_BATCH, pids, v)
+            ctx.sync()
+            i_ms, i_n = ctx.profile_query("ntt_inv")
+            ctx.profile(False)
+            avg = f_ms / f_n
+            by = NTT_BATCH * 2 * poly_words * 8
+            res[name] = {"ntts_per_s": round(NTT_BATCH / (avg * 1e-3), 1),
+                         "fwd_avg_launch_ms": round(avg, 4),
+                         "inv_avg_launch_ms": round(i_ms / i_n, 4),
+                         "hbm_gbs": round(by / (avg * 1e-3) / 1e9, 1),
+                         "hbm_frac": round(by / (avg * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                         "wall_ntts_per_s": round(NTT_BATCH * reps / el, 1)}
+        out["ntts_per_s"] = res["f64"]["ntts_per_s"]
+        out["ntt"] = res
+        # round trip stays bit-exact
+        d_x2 = ctx.array(x[:4])
+        ctx.ntt_fwd(d_x2, 4, pids)
+        ctx.ntt_inv(d_x2, 4, pids)
+        out["ntt_round_trip_identity"] = bool(np.array_equal(d_x2.download(), x[:4]))
+        del d_x
+
+        # Plain per-call form: one API call per ciphertext, synchronous.
+        calls = 64
+        one0, one1 = ctx.empty((1, L, n)), ctx.empty((1, L, n))
+        plain_lat = []
+        for c in range(calls + 4):
+            t0 = time.perf_counter()
+            sub = d_comps.view(c * 3 * poly_words, (1, 3, L, n))
+            ctx.relin_apply(sub, 2, 0b111, d_ka, d_kb, k, log_bs, one0, one1, 1, pids)
+            ctx.sync()
+            if c >= 4:
+                plain_lat.append((time.perf_counter() - t0) * 1e3)
+        plain = 1e3 / np.mean(plain_lat)
+        out["plain_keyswitches_per_s"] = round(plain, 1)
+        out["plain_latency_ms"] = {"p50": round(percentile(plain_lat, 50), 4),
+                                   "p99": round(percentile(plain_lat, 99), 4)}
+        out["batched_vs_plain"] = round(value / plain, 2)
+
+        # Galois substitution x -> x^5 and key switch, then modulus switch.
+        gal_in = ctx.array(comps[:, :2])
+        d_sub = ctx.empty((BATCH, 2, L, n))
+        ctx.substitute(gal_in, d_sub, 5, BATCH * 2, pids)
+        ctx.galois_apply(d_sub, True, d_ka, d_kb, log_bs, d_o0, d_o1, BATCH, pids)
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(3):
+            ctx.substitute(gal_in, d_sub, 5, BATCH * 2, pids)
+            ctx.galois_apply(d_sub, True, d_ka, d_kb, log_bs, d_o0, d_o1, BATCH, pids)
+        ctx.sync()
+        out["galois_keyswitches_per_s"] = round(3 * BATCH / (time.perf_counter() - t0), 1)
+        del gal_in, d_sub
+        ql_inv = rns.ql_inv()
+        ms_in = ctx.array(comps[:, :2])
+        ctx.mod_reduce_lsb(ms_in, 65537, ql_inv, True, BATCH * 2, pids)
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(3):
+            ctx.mod_reduce_lsb(ms_in, 65537, ql_inv, True, BATCH * 2, pids)
+        ctx.sync()
+        out["modswitch_ciphertexts_per_s"] = round(3 * BATCH / (time.perf_counter() - t0), 1)
+
+    if not args.no_cpu:
+        print("[bench] cpu baseline", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(qs, comps[:8], kas, kbs, log_bs)
+    ctx.close()
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -189,6 +189,16 @@ typedef struct shg_limb_job {
 int shg_batch_limbs(shg_context* ctx, const shg_limb_job* host_jobs,
                     int num_jobs, int op);
 
+/* ---- timing -------------------------------------------------------------
+ * Per-kernel timing with HIP events recorded on the context's stream around
+ * each launch (off by default; the reference has no tracing of its own).
+ * Names: "ntt_fwd", "ntt_inv", "pointwise", "substitute", "tensor_product",
+ * "keyswitch_intt", "keyswitch". */
+int shg_profile_enable(shg_context* ctx, int enable);
+int shg_profile_reset(shg_context* ctx);
+int shg_profile_query(shg_context* ctx, const char* kernel, double* total_ms,
+                      long long* launches);
+
 /* ---- autotuner ---------------------------------------------------------- */
 /* Times the NTT variants for (log_n, count*num_limbs) on the device and
  * records the fastest for SHG_NTT_AUTO.  Returns the chosen variant. */

@@ -66,6 +66,14 @@ struct shg_context {
   size_t scratch_size = 0;
   std::map<std::pair<int, int>, int> tuned;  // (log_n, bucket) -> variant
   std::mutex mu;
+  // Optional per-kernel timing: HIP events recorded around launches.
+  bool profiling = false;
+  struct ProfRec {
+    std::string name;
+    hipEvent_t start, stop;
+  };
+  std::vector<ProfRec> prof;
+  std::map<std::string, std::pair<double, long long>> prof_totals;
 };
 
 namespace {
@@ -82,6 +90,44 @@ int ensure_scratch(shg_context* ctx, size_t bytes) {
   if (e != hipSuccess) return fail(SHG_E_NO_MEMORY, "scratch allocation of %zu bytes failed", bytes);
   ctx->scratch_size = bytes;
   return SHG_OK;
+}
+
+// Records a HIP event pair around the launches made during its lifetime when
+// profiling is on.
+struct ProfScope {
+  shg_context* ctx;
+  shg_context::ProfRec rec;
+  bool on;
+  ProfScope(shg_context* c, const char* name) : ctx(c), on(c && c->profiling) {
+    if (!on) return;
+    rec.name = name;
+    if (hipEventCreate(&rec.start) != hipSuccess || hipEventCreate(&rec.stop) != hipSuccess) {
+      on = false;
+      return;
+    }
+    (void)hipEventRecord(rec.start, ctx->stream);
+  }
+  ~ProfScope() {
+    if (!on) return;
+    (void)hipEventRecord(rec.stop, ctx->stream);
+    ctx->prof.push_back(rec);
+  }
+};
+
+void prof_collect(shg_context* ctx) {
+  if (ctx->prof.empty()) return;
+  (void)hipStreamSynchronize(ctx->stream);
+  for (auto& r : ctx->prof) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, r.start, r.stop) == hipSuccess) {
+      auto& t = ctx->prof_totals[r.name];
+      t.first += ms;
+      t.second += 1;
+    }
+    (void)hipEventDestroy(r.start);
+    (void)hipEventDestroy(r.stop);
+  }
+  ctx->prof.clear();
 }
 
 // Validates a prime list and returns log_n and the LimbPrimes table.
@@ -290,6 +336,10 @@ int shg_prime_create(shg_context* ctx, uint64_t q, int log_n, int* prime) {
   d.qinv = 1.0 / qd;
   d.n_inv_d = (double)d.n_inv_c;
   d.n_inv_dq = (double)d.n_inv_c / qd;
+  d.n_inv_x_c = P.ExportInt(d.n_inv_c);  // n^-1 * 2^-64 mod q
+  d.n_inv_x_cb = (uint64_t)(((rlwe::Uint128)d.n_inv_x_c << 64) / q);
+  d.n_inv_x_d = (double)d.n_inv_x_c;
+  d.n_inv_x_dq = (double)d.n_inv_x_c / qd;
   d.log_n = log_n;
   d.log_q = (int)P.log_modulus;
   d.fw = (const uint64_t*)base;
@@ -358,6 +408,7 @@ static int ntt_common(shg_context* ctx, bool inverse, uint64_t* polys, int count
     return fail(SHG_E_INVALID_ARGUMENT, "batch too large for one launch.");
   const int v = resolve_variant(ctx, variant, primes, num_limbs, log_n, rows);
   if (v < 0) return fail(SHG_E_INVALID_ARGUMENT, "FP64 transform requires moduli below 2^50.");
+  ProfScope prof(ctx, inverse ? "ntt_inv" : "ntt_fwd");
   hipError_t e = shg::launch_ntt(inverse, v, polys, ctx->d_primes, lp, (int)rows, num_limbs, log_n,
                                  shg::kMaxLogM, ctx->stream);
   return e == hipSuccess ? SHG_OK : hip_fail(e, "ntt launch");
@@ -384,6 +435,7 @@ int shg_pointwise(shg_context* ctx, int op, uint64_t* x, const uint64_t* a, size
   if (!aligned_ptr(x)) return fail(SHG_E_INVALID_ARGUMENT, "`this` must not be null.");
   if (op != SHG_OP_NEG && !aligned_ptr(a)) return fail(SHG_E_INVALID_ARGUMENT, "`that` must not be null.");
   if (op == SHG_OP_FMA && !aligned_ptr(b)) return fail(SHG_E_INVALID_ARGUMENT, "`b` must not be null.");
+  ProfScope prof(ctx, "pointwise");
   hipError_t e = shg::launch_pointwise(op, x, a, a_stride, b, b_stride, ctx->d_primes, lp, count,
                                        num_limbs, log_n, ctx->stream);
   return e == hipSuccess ? SHG_OK : hip_fail(e, "pointwise launch");
@@ -434,6 +486,7 @@ int shg_substitute(shg_context* ctx, const uint64_t* in, uint64_t* out, int powe
   if (count <= 0) return count == 0 ? SHG_OK : fail(SHG_E_INVALID_ARGUMENT, "count must be non-negative.");
   if (!aligned_ptr(in) || !aligned_ptr(out) || in == out)
     return fail(SHG_E_INVALID_ARGUMENT, "substitute needs distinct input and output buffers.");
+  ProfScope prof(ctx, "substitute");
   hipError_t e = shg::launch_substitute(in, out, power, count, num_limbs, log_n, ctx->stream);
   return e == hipSuccess ? SHG_OK : hip_fail(e, "substitute launch");
 }
@@ -506,6 +559,7 @@ int shg_tensor_product(shg_context* ctx, const uint64_t* a, int len_a, const uin
   if (count <= 0) return count == 0 ? SHG_OK : fail(SHG_E_INVALID_ARGUMENT, "count must be non-negative.");
   if (!aligned_ptr(a) || !aligned_ptr(b) || !aligned_ptr(out))
     return fail(SHG_E_INVALID_ARGUMENT, "ciphertext data must not be null.");
+  ProfScope prof(ctx, "tensor_product");
   hipError_t e = shg::launch_tensor_product(a, len_a, b, len_b, out, ctx->d_primes, lp, count,
                                             num_limbs, log_n, ctx->stream);
   return e == hipSuccess ? SHG_OK : hip_fail(e, "tensor launch");
@@ -548,15 +602,23 @@ static int keyswitch_common(shg_context* ctx, const uint64_t* comps, int comps_p
   const int v = resolve_variant(ctx, SHG_NTT_AUTO, primes, num_limbs, log_n,
                                 (long long)count * num_limbs);
   hipError_t e = hipSuccess;
+  const bool fused = log_n <= shg::kMaxLogM && log_n >= 4;
   for (int i = 0; i < num_src && e == hipSuccess; ++i) {
     e = hipMemcpy2DAsync(src, poly_words * 8, comps + (size_t)src_comp[i] * poly_words,
                          (size_t)comps_per_ct * poly_words * 8, poly_words * 8, count,
                          hipMemcpyDeviceToDevice, s);
-    if (e == hipSuccess && src_is_ntt[i])
+    // The fused kernel reads exported residues: fold ExportInt into the
+    // inverse transform's scaling (or export in place for coefficient-form
+    // input).  The unfused path decomposes Montgomery representations.
+    if (e == hipSuccess && src_is_ntt[i]) {
+      ProfScope prof(ctx, "keyswitch_intt");
       e = shg::launch_ntt(true, v, src, ctx->d_primes, lp, count * num_limbs, num_limbs, log_n,
-                          shg::kMaxLogM, s);
+                          shg::kMaxLogM, s, fused ? 2 : 1);
+    }
+    else if (e == hipSuccess && fused)
+      e = shg::launch_export(src, ctx->d_primes, lp, count, num_limbs, log_n, s);
     if (e != hipSuccess) break;
-    if (log_n <= shg::kMaxLogM && log_n >= 4) {
+    if (fused) {
       shg::KeySwitchArgs ka{};
       ka.src = src;
       ka.src_stride = poly_words;
@@ -571,6 +633,7 @@ static int keyswitch_common(shg_context* ctx, const uint64_t* comps, int comps_p
       ka.out1 = out_c1;
       ka.out_stride = poly_words;
       ka.accumulate = i > 0;
+      ProfScope prof(ctx, "keyswitch");
       e = shg::launch_keyswitch(ka, gd, v, ctx->d_primes, lp, count, num_limbs, log_n, s);
     } else {
       // Unfused form for ring degrees outside the fused kernel's range:
@@ -702,6 +765,33 @@ int shg_lazy_fma(shg_context* ctx, const uint64_t* as, size_t as_stride, const u
   hipError_t e = shg::launch_lazy_fma(as, as_stride, bs, bs_stride, num_terms, out, ctx->d_primes,
                                       lp, num_limbs, log_n, ctx->stream);
   return e == hipSuccess ? SHG_OK : hip_fail(e, "lazy fma launch");
+}
+
+int shg_profile_enable(shg_context* ctx, int enable) {
+  if (!ctx) return fail(SHG_E_INVALID_ARGUMENT, "context must not be null.");
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  prof_collect(ctx);
+  ctx->profiling = enable != 0;
+  return SHG_OK;
+}
+
+int shg_profile_reset(shg_context* ctx) {
+  if (!ctx) return fail(SHG_E_INVALID_ARGUMENT, "context must not be null.");
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  prof_collect(ctx);
+  ctx->prof_totals.clear();
+  return SHG_OK;
+}
+
+int shg_profile_query(shg_context* ctx, const char* kernel, double* total_ms, long long* launches) {
+  if (!ctx || !kernel || !total_ms || !launches)
+    return fail(SHG_E_INVALID_ARGUMENT, "null argument.");
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  prof_collect(ctx);
+  auto it = ctx->prof_totals.find(kernel);
+  *total_ms = it == ctx->prof_totals.end() ? 0.0 : it->second.first;
+  *launches = it == ctx->prof_totals.end() ? 0 : it->second.second;
+  return SHG_OK;
 }
 
 int shg_batch_limbs(shg_context* ctx, const shg_limb_job* host_jobs, int num_jobs, int op) {

@@ -33,6 +33,10 @@ struct PrimeDev {
   double qinv;             // fl(1/q)
   double n_inv_d;          // value of n^-1 as double
   double n_inv_dq;         // fl(n^-1 / q)
+  uint64_t n_inv_x_c;      // value of n^-1 * 2^-64 (INTT that also exports)
+  uint64_t n_inv_x_cb;     // its Shoup constant
+  double n_inv_x_d;        // n^-1 * 2^-64 as double
+  double n_inv_x_dq;       // fl(n^-1 * 2^-64 / q)
   int log_n;
   int log_q;
   const uint64_t* fw;      // [2n] forward (c, cb), bit-reversed psi powers

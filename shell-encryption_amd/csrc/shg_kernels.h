@@ -13,11 +13,16 @@ namespace shg {
 // (136 KiB with padding) fits the 160 KiB of LDS per CU.
 constexpr int kMaxLogM = 14;
 
+// inv_mode (inverse only): 1 = times n^-1 (InverseNumberTheoreticTransform),
+// 2 = times n^-1 * 2^-64 (the inverse transform followed by ExportInt).
 hipError_t launch_ntt(bool inverse, int variant, uint64_t* data, const PrimeDev* primes,
                       const LimbPrimes& lp, int rows, int num_limbs, int logn, int max_logm,
-                      hipStream_t s);
+                      hipStream_t s, int inv_mode = 1);
 hipError_t launch_scale_ninv(uint64_t* data, const PrimeDev* primes, const LimbPrimes& lp,
-                             int rows, int num_limbs, int logn, hipStream_t s);
+                             int rows, int num_limbs, int logn, int mode, hipStream_t s);
+// x <- ExportInt(x) (montgomery.h:156-164) over count polynomials.
+hipError_t launch_export(uint64_t* x, const PrimeDev* primes, const LimbPrimes& lp, int count,
+                         int num_limbs, int logn, hipStream_t s);
 
 hipError_t launch_pointwise(int op, uint64_t* x, const uint64_t* a, size_t a_stride,
                             const uint64_t* b, size_t b_stride, const PrimeDev* primes,
@@ -70,10 +75,10 @@ hipError_t launch_gadget_decompose(const uint64_t* a, uint64_t* digits, const Ga
                                    int num_limbs, int logn, hipStream_t s);
 
 // Fused key switch: for each (ciphertext, output limb), for every gadget
-// digit, extract + import + forward NTT in LDS and multiply-accumulate with
-// the key; acc0 starts from c0 (or zero), acc1 from c1 (or zero).
+// digit, extract + forward NTT in LDS and multiply-accumulate with the key;
+// acc0 starts from c0 (or zero), acc1 from c1 (or zero).
 struct KeySwitchArgs {
-  const uint64_t* src;       // coefficient-form input component(s), count x L x n
+  const uint64_t* src;       // exported (ExportInt) coefficient-form input, count x L x n
   size_t src_stride;         // words between ciphertexts in src
   const uint64_t* c0;        // NTT-form c0 (added), or null
   size_t c0_stride;

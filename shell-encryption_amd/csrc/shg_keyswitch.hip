@@ -7,11 +7,13 @@
 // where d_j = g^-1(c) are the unsigned base-2^b digits of the exported
 // residues (gadget.h:52-79), lifted to every limb (rns/rns_gadget.cc:125-151).
 //
-// One workgroup owns one (ciphertext, output limb l).  For each digit j it
-// extracts the digit from the source limb straight into VGPRs (no digit
-// polynomial is ever written to HBM), imports it modulo q_l, runs the whole
-// forward NTT in LDS/VGPRs, and multiplies the transform into two per-thread
-// accumulators that stay in VGPRs across all k digits.  Workgroups are
+// One workgroup owns one (ciphertext, output limb l) (two for n = 2^14).  The
+// source component arrives already exported (the inverse transform that
+// produced it folds ExportInt into its n^-1 scaling).  For each digit j the
+// kernel extracts the digit from the source limb straight into VGPRs (no
+// digit polynomial is ever written to HBM), runs the whole forward NTT in
+// LDS/VGPRs, and multiplies the transform into two per-thread accumulators
+// that stay in VGPRs across all k digits.  Workgroups are
 // ordered limb-major so the ciphertexts resident at one time read the same
 // key rows, which the L2 and Infinity Cache then serve.
 #include <hip/hip_runtime.h>
@@ -22,86 +24,104 @@
 
 namespace shg {
 
-template <int LOGM, int LOGE, class A>
+// LOGN = LOGM + HB.  With HB = 1 (n = 2^14) the limb is split between two
+// workgroups: each loads both halves of the digit, applies the first
+// Cooley-Tukey stage for its own half only, and runs the remaining stages on
+// a 2^13-word slice (68 KiB of LDS, so two workgroups share a CU).
+template <int LOGM, int LOGE, int HB, class A>
 __global__ void __launch_bounds__((1 << LOGM) >> LOGE)
     keyswitch_kernel(KeySwitchArgs ka, GadgetDesc gd, const PrimeDev* __restrict__ primes,
                      LimbPrimes lp, int count, int num_limbs) {
   using V = typename A::V;
+  using Acc = typename A::Acc;
   constexpr int E = 1 << LOGE;
   constexpr int T = (1 << LOGM) >> LOGE;
-  constexpr int N = 1 << LOGM;
+  constexpr int M = 1 << LOGM;
+  constexpr int LOGN = LOGM + HB;
   extern __shared__ __align__(16) unsigned char smem[];
   V* lds = reinterpret_cast<V*>(smem);
-  const int l = blockIdx.x / count;
-  const int ct = blockIdx.x - l * count;
+  const int h = HB ? (int)(blockIdx.x & ((1u << HB) - 1)) : 0;
+  const int lc = (int)(blockIdx.x >> HB);  // l * count + ct: limb-major order
+  const int l = lc / count;
+  const int ct = lc - l * count;
   const int t = threadIdx.x;
   const PrimeDev& P = primes[lp.id[l]];
   const A ar(P);
-  const size_t limb_off = (size_t)l << LOGM;
-  const size_t poly_words = (size_t)num_limbs << LOGM;
+  const size_t limb_off = ((size_t)l << LOGN) + ((size_t)h << LOGM);
+  const size_t poly_words = (size_t)num_limbs << LOGN;
 
-  uint64_t acc0[E], acc1[E];
+  Acc acc0[E], acc1[E];
   uint64_t* o0 = ka.out0 + ct * ka.out_stride + limb_off;
   uint64_t* o1 = ka.out1 + ct * ka.out_stride + limb_off;
-  if (ka.accumulate) {
+  {
+    const uint64_t* i0 = ka.accumulate ? o0 : (ka.c0 ? ka.c0 + ct * ka.c0_stride + limb_off : nullptr);
+    const uint64_t* i1 = ka.accumulate ? o1 : (ka.c1 ? ka.c1 + ct * ka.c1_stride + limb_off : nullptr);
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-      acc0[e] = o0[e * T + t];
-      acc1[e] = o1[e * T + t];
-    }
-  } else {
-    const uint64_t* c0 = ka.c0 ? ka.c0 + ct * ka.c0_stride + limb_off : nullptr;
-    const uint64_t* c1 = ka.c1 ? ka.c1 + ct * ka.c1_stride + limb_off : nullptr;
-#pragma unroll
-    for (int e = 0; e < E; ++e) {
-      acc0[e] = c0 ? c0[e * T + t] : 0;
-      acc1[e] = c1 ? c1[e * T + t] : 0;
+      acc0[e] = ar.acc_in(i0 ? i0[e * T + t] : 0);
+      acc1[e] = ar.acc_in(i1 ? i1[e * T + t] : 0);
     }
   }
 
   const uint64_t* src_ct = ka.src + ct * ka.src_stride;
   for (int j = 0; j < gd.k; ++j) {
-    const int si = gd.src_limb[j];
-    const PrimeDev& Ps = primes[lp.id[si]];
-    const uint64_t* src = src_ct + ((size_t)si << LOGM);
+    const uint64_t* src = src_ct + ((size_t)gd.src_limb[j] << LOGN);
     const int sh = gd.shift[j];
     const uint64_t mask = (1ull << gd.log_b[j]) - 1;
     V x[E];
+    if constexpr (HB == 0) {
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-      const uint64_t v = mont_export(src[e * T + t], Ps);
-      x[e] = ar.load(mont_import((v >> sh) & mask, P));
+      for (int e = 0; e < E; ++e) x[e] = ar.digit_in((src[e * T + t] >> sh) & mask, P);
+    } else {
+      const auto w = ar.fwd_tw(1);
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        V u = ar.digit_in((src[e * T + t] >> sh) & mask, P);
+        V v = ar.digit_in((src[M + e * T + t] >> sh) & mask, P);
+        ar.ct(u, v, w);
+        x[e] = h ? v : u;
+        if (ar.fwd_reduce_every == 1) ar.fwd_reduce(x[e]);
+      }
     }
-    FwdPasses<LOGM, LOGE, 0, A>::run(x, lds, t, 0, 0, ar);
+    FwdPasses<LOGM, LOGE, 0, A>::run(x, lds, t, h, HB, ar);
     const uint64_t* kb = ka.key_b + (size_t)(ka.key_base + j) * poly_words + limb_off;
     const uint64_t* kav = ka.key_a + (size_t)(ka.key_base + j) * poly_words + limb_off;
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int p = e * T + t;
-      const uint64_t d = ar.fwd_out(lds[lpad(p)]);
-      acc0[e] = mont_fma(acc0[e], d, kb[p], P);
-      acc1[e] = mont_fma(acc1[e], d, kav[p], P);
+      const auto d = ar.prep(lds[lpad(p)]);
+      ar.mac(acc0[e], d, kb[p], P);
+      ar.mac(acc1[e], d, kav[p], P);
+      // Keep key loads from being hoisted far ahead of their use: the
+      // accumulators already hold 4*E VGPRs.
+      if ((e & 3) == 3) __builtin_amdgcn_sched_barrier(0);
+    }
+    if (((j + 1) % A::kAccReduceEvery) == 0) {
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        ar.acc_reduce(acc0[e]);
+        ar.acc_reduce(acc1[e]);
+      }
     }
     __syncthreads();
   }
 #pragma unroll
   for (int e = 0; e < E; ++e) {
-    o0[e * T + t] = acc0[e];
-    o1[e * T + t] = acc1[e];
+    o0[e * T + t] = ar.acc_out(acc0[e]);
+    o1[e * T + t] = ar.acc_out(acc1[e]);
   }
-  (void)N;
 }
 
-template <int LOGM, int LOGE, class A>
+template <int LOGM, int LOGE, class A, int HB = 0>
 static hipError_t launch_ks(const KeySwitchArgs& ka, const GadgetDesc& gd, const PrimeDev* primes,
                             const LimbPrimes& lp, int count, int num_limbs, hipStream_t s) {
   constexpr int T = (1 << LOGM) >> LOGE;
   const size_t shmem = (size_t)lds_words<LOGM>() * 8;
-  auto k = keyswitch_kernel<LOGM, LOGE, A>;
+  auto k = keyswitch_kernel<LOGM, LOGE, HB, A>;
   if (shmem > 65536)
     (void)hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
-  hipLaunchKernelGGL(k, dim3((unsigned)(count * num_limbs)), dim3(T), shmem, s, ka, gd, primes, lp,
-                     count, num_limbs);
+  hipLaunchKernelGGL(k, dim3((unsigned)(count * num_limbs) << HB), dim3(T), shmem, s, ka, gd,
+                     primes, lp, count, num_limbs);
   return hipGetLastError();
 }
 
@@ -120,7 +140,7 @@ static hipError_t dispatch_ks(int logn, const KeySwitchArgs& ka, const GadgetDes
     case 11: return launch_ks<11, 4, A>(ka, gd, primes, lp, count, num_limbs, s);
     case 12: return launch_ks<12, 4, A>(ka, gd, primes, lp, count, num_limbs, s);
     case 13: return launch_ks<13, 4, A>(ka, gd, primes, lp, count, num_limbs, s);
-    case 14: return launch_ks<14, 4, A>(ka, gd, primes, lp, count, num_limbs, s);
+    case 14: return launch_ks<13, 4, A, 1>(ka, gd, primes, lp, count, num_limbs, s);
     default: return hipErrorInvalidValue;
   }
 }

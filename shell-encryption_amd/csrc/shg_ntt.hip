@@ -81,7 +81,7 @@ __global__ void __launch_bounds__((1 << LOGM) >> LOGE)
 #pragma unroll
     for (int e = 0; e < (1 << R); ++e) {
       const V v = x[gg * (1 << R) + e];
-      g[(e << I0) | (gg * T + t)] = scale ? ar.inv_out(v) : ar.fwd_out(v);
+      g[(e << I0) | (gg * T + t)] = ar.inv_out(v, scale);
     }
   }
 }
@@ -182,7 +182,7 @@ static hipError_t dispatch_fused(int logm, bool inverse, uint64_t* data, const P
     case 11: return launch_fused<11, 4, A>(inverse, data, primes, lp, rows, num_limbs, logn, scale, s);
     case 12: return launch_fused<12, 4, A>(inverse, data, primes, lp, rows, num_limbs, logn, scale, s);
     case 13: return launch_fused<13, 4, A>(inverse, data, primes, lp, rows, num_limbs, logn, scale, s);
-    case 14: return launch_fused<14, 4, A>(inverse, data, primes, lp, rows, num_limbs, logn, scale, s);
+    case 14: return launch_fused<14, 5, A>(inverse, data, primes, lp, rows, num_limbs, logn, scale, s);
     default: return hipErrorInvalidValue;
   }
 }
@@ -190,7 +190,7 @@ static hipError_t dispatch_fused(int logm, bool inverse, uint64_t* data, const P
 template <class A>
 static hipError_t run_ntt(bool inverse, uint64_t* data, const PrimeDev* primes,
                           const LimbPrimes& lp, int rows, int num_limbs, int logn, int max_logm,
-                          hipStream_t s) {
+                          int inv_mode, hipStream_t s) {
   const int logm = logn < max_logm ? logn : max_logm;
   hipError_t e = hipSuccess;
   if (!inverse) {
@@ -199,21 +199,22 @@ static hipError_t run_ntt(bool inverse, uint64_t* data, const PrimeDev* primes,
     if (e == hipSuccess) e = dispatch_fused<A>(logm, false, data, primes, lp, rows, num_limbs, logn, 1, s);
   } else {
     e = dispatch_fused<A>(logm, true, data, primes, lp, rows, num_limbs, logn,
-                          logn == logm ? 1 : 0, s);
+                          logn == logm ? inv_mode : 0, s);
     for (int st = logm; st < logn && e == hipSuccess; ++st) {
       e = launch_stream<A>(true, data, primes, lp, rows, num_limbs, logn, st, s);
     }
-    if (e == hipSuccess && logn > logm) e = launch_scale_ninv(data, primes, lp, rows, num_limbs, logn, s);
+    if (e == hipSuccess && logn > logm)
+      e = launch_scale_ninv(data, primes, lp, rows, num_limbs, logn, inv_mode, s);
   }
   return e;
 }
 
 hipError_t launch_ntt(bool inverse, int variant, uint64_t* data, const PrimeDev* primes,
                       const LimbPrimes& lp, int rows, int num_limbs, int logn, int max_logm,
-                      hipStream_t s) {
+                      hipStream_t s, int inv_mode) {
   if (variant == SHG_NTT_F64)
-    return run_ntt<F64Arith>(inverse, data, primes, lp, rows, num_limbs, logn, max_logm, s);
-  return run_ntt<IntArith>(inverse, data, primes, lp, rows, num_limbs, logn, max_logm, s);
+    return run_ntt<F64Arith>(inverse, data, primes, lp, rows, num_limbs, logn, max_logm, inv_mode, s);
+  return run_ntt<IntArith>(inverse, data, primes, lp, rows, num_limbs, logn, max_logm, inv_mode, s);
 }
 
 }  // namespace shg

@@ -42,14 +42,14 @@ struct IntArith {
   uint64_t q, q2, bn;
   const uint64_t* __restrict__ fw;
   const uint64_t* __restrict__ iw;
-  uint64_t ninv_c, ninv_cb;
+  uint64_t ninv_c, ninv_cb, ninvx_c, ninvx_cb;
   // Lazy growth is < 2q per CT stage, so up to 15 stages stay below 2^64 for
   // q < 2^58; wider moduli (up to the reference's 2^62 limit) are reduced
   // after every stage.
   int fwd_reduce_every, inv_reduce_every;
   __device__ __forceinline__ explicit IntArith(const PrimeDev& P)
       : q(P.q), q2(2 * P.q), bn(P.bn), fw(P.fw), iw(P.iw), ninv_c(P.n_inv_c),
-        ninv_cb(P.n_inv_cb), fwd_reduce_every(P.q < (1ull << 58) ? 64 : 1),
+        ninv_cb(P.n_inv_cb), ninvx_c(P.n_inv_x_c), ninvx_cb(P.n_inv_x_cb), fwd_reduce_every(P.q < (1ull << 58) ? 64 : 1),
         inv_reduce_every(64) {}
   struct Tw {
     uint64_t c, cb;
@@ -73,18 +73,39 @@ struct IntArith {
   __device__ __forceinline__ void fwd_reduce(V& x) const { x = reduce_full(x, q, bn); }
   __device__ __forceinline__ void inv_reduce(V&) const {}
   __device__ __forceinline__ uint64_t fwd_out(V x) const { return reduce_full(x, q, bn); }
-  __device__ __forceinline__ uint64_t inv_out(V x) const {
-    return csub(shoup_lazy(x, ninv_c, ninv_cb, q), q);
+  // mode 1: times n^-1 (InverseNumberTheoreticTransform); mode 2: times
+  // n^-1 * 2^-64, i.e. the inverse transform followed by ExportInt.
+  __device__ __forceinline__ uint64_t inv_out(V x, int mode) const {
+    if (mode == 0) return fwd_out(x);
+    return mode == 1 ? csub(shoup_lazy(x, ninv_c, ninv_cb, q), q)
+                     : csub(shoup_lazy(x, ninvx_c, ninvx_cb, q), q);
   }
+
+  // Key-switch accumulation (montgomery.h:446-465): digits enter the
+  // transform as Montgomery representations and the key product is a REDC.
+  using Acc = uint64_t;
+  using Prep = uint64_t;
+  static constexpr int kAccReduceEvery = 1 << 30;
+  __device__ __forceinline__ V digit_in(uint64_t d, const PrimeDev& P) const {
+    return mont_import(d, P);
+  }
+  __device__ __forceinline__ Acc acc_in(uint64_t rep) const { return rep; }
+  __device__ __forceinline__ Prep prep(V y) const { return fwd_out(y); }
+  __device__ __forceinline__ void mac(Acc& a, Prep d, uint64_t key, const PrimeDev& P) const {
+    a = mont_fma(a, d, key, P);
+  }
+  __device__ __forceinline__ void acc_reduce(Acc&) const {}
+  __device__ __forceinline__ uint64_t acc_out(Acc a) const { return a; }
 };
 
 struct F64Arith {
   using V = double;
-  double q, qinv, ninv, ninvq;
+  double q, qinv, ninv, ninvq, ninvx, ninvxq;
   const double* __restrict__ fw;
   const double* __restrict__ iw;
   __device__ __forceinline__ explicit F64Arith(const PrimeDev& P)
-      : q(P.qd), qinv(P.qinv), ninv(P.n_inv_d), ninvq(P.n_inv_dq), fw(P.fwd), iw(P.iwd) {}
+      : q(P.qd), qinv(P.qinv), ninv(P.n_inv_d), ninvq(P.n_inv_dq), ninvx(P.n_inv_x_d),
+        ninvxq(P.n_inv_x_dq), fw(P.fwd), iw(P.iwd) {}
   struct Tw {
     double w, wq;
   };
@@ -107,8 +128,32 @@ struct F64Arith {
   __device__ __forceinline__ uint64_t fwd_out(V x) const {
     return f64_to_u64(f64_canon(f64_reduce(x, q, qinv), q));
   }
-  __device__ __forceinline__ uint64_t inv_out(V x) const {
-    return f64_to_u64(f64_canon(f64_reduce(f64_mulmod(x, ninv, ninvq, q), q, qinv), q));
+  __device__ __forceinline__ uint64_t inv_out(V x, int mode) const {
+    if (mode == 0) return fwd_out(x);
+    const double y = mode == 1 ? f64_mulmod(x, ninv, ninvq, q) : f64_mulmod(x, ninvx, ninvxq, q);
+    return f64_to_u64(f64_canon(f64_reduce(y, q, qinv), q));
+  }
+
+  // Key-switch accumulation.  The digit value d (not its Montgomery form)
+  // enters the transform, and the key's Montgomery representation k is used
+  // as a plain residue: NTT(d) * k = NTT(d * 2^64) * k * 2^-64 mod q, which
+  // is the reference's FusedMulAddInPlace of the imported digit.  Terms are
+  // bounded by 0.7 q, so the accumulator is reduced every eight digits.
+  using Acc = double;
+  using Prep = double;
+  static constexpr int kAccReduceEvery = 8;
+  __device__ __forceinline__ V digit_in(uint64_t d, const PrimeDev&) const {
+    return u64_to_f64(d);
+  }
+  __device__ __forceinline__ Acc acc_in(uint64_t rep) const { return u64_to_f64(rep); }
+  __device__ __forceinline__ Prep prep(V y) const { return f64_reduce(y, q, qinv); }
+  __device__ __forceinline__ void mac(Acc& a, Prep d, uint64_t key, const PrimeDev&) const {
+    const double w = u64_to_f64(key);
+    a += f64_mulmod(d, w, w * qinv, q);
+  }
+  __device__ __forceinline__ void acc_reduce(Acc& a) const { a = f64_reduce(a, q, qinv); }
+  __device__ __forceinline__ uint64_t acc_out(Acc a) const {
+    return f64_to_u64(f64_canon(f64_reduce(a, q, qinv), q));
   }
   // Bounds (q < 2^50, |x| <= q/2 after a reduction): four CT stages keep
   // |x| < 5.4 q < 2^52.5; two GS stages keep |x| < 2^52.
@@ -154,6 +199,8 @@ __device__ __forceinline__ void fwd_pass(typename A::V (&x)[1 << LOGE], int t, i
 #pragma unroll
       for (int i = 0; i < E; ++i) ar.fwd_reduce(x[i]);
     }
+    // One stage of twiddles in flight at a time bounds VGPR pressure.
+    __builtin_amdgcn_sched_barrier(0);
   }
 }
 
@@ -237,6 +284,7 @@ __device__ __forceinline__ void inv_pass(typename A::V (&x)[1 << LOGE], int t, i
 #pragma unroll
       for (int k = 0; k < E; ++k) ar.inv_reduce(x[k]);
     }
+    __builtin_amdgcn_sched_barrier(0);
   }
 }
 

@@ -18,19 +18,38 @@ static inline unsigned grid_for(long long items, int block) {
 // dft_transformations.h:195-199) for limbs whose transform ran in pieces.
 // ---------------------------------------------------------------------------
 __global__ void scale_ninv_kernel(uint64_t* __restrict__ data, const PrimeDev* __restrict__ primes,
-                                  LimbPrimes lp, int num_limbs, int logn, long long total) {
+                                  LimbPrimes lp, int num_limbs, int logn, int mode,
+                                  long long total) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= total) return;
   const long long row = i >> logn;
   const PrimeDev& P = primes[lp.id[row % num_limbs]];
-  data[i] = csub(shoup_lazy(data[i], P.n_inv_c, P.n_inv_cb, P.q), P.q);
+  const uint64_t c = mode == 2 ? P.n_inv_x_c : P.n_inv_c;
+  const uint64_t cb = mode == 2 ? P.n_inv_x_cb : P.n_inv_cb;
+  data[i] = csub(shoup_lazy(data[i], c, cb, P.q), P.q);
 }
 
 hipError_t launch_scale_ninv(uint64_t* data, const PrimeDev* primes, const LimbPrimes& lp, int rows,
-                             int num_limbs, int logn, hipStream_t s) {
+                             int num_limbs, int logn, int mode, hipStream_t s) {
   const long long total = (long long)rows << logn;
   hipLaunchKernelGGL(scale_ninv_kernel, dim3(grid_for(total, 256)), dim3(256), 0, s, data, primes,
-                     lp, num_limbs, logn, total);
+                     lp, num_limbs, logn, mode, total);
+  return hipGetLastError();
+}
+
+__global__ void export_kernel(uint64_t* __restrict__ x, const PrimeDev* __restrict__ primes,
+                              LimbPrimes lp, int num_limbs, int logn, long long total) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  const int limb = (int)((i >> logn) % num_limbs);
+  x[i] = mont_export(x[i], primes[lp.id[limb]]);
+}
+
+hipError_t launch_export(uint64_t* x, const PrimeDev* primes, const LimbPrimes& lp, int count,
+                         int num_limbs, int logn, hipStream_t s) {
+  const long long total = ((long long)count * num_limbs) << logn;
+  hipLaunchKernelGGL(export_kernel, dim3(grid_for(total, 256)), dim3(256), 0, s, x, primes, lp,
+                     num_limbs, logn, total);
   return hipGetLastError();
 }
 

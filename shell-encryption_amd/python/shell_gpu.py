@@ -67,6 +67,10 @@ def lib():
         "shg_lazy_fma": [VP, VP, sz, VP, sz, i, VP, i, INTP],
         "shg_batch_limbs": [VP, ctypes.POINTER(LimbJob), i, i],
         "shg_autotune_ntt": [VP, i, i, INTP],
+        "shg_profile_enable": [VP, i],
+        "shg_profile_reset": [VP],
+        "shg_profile_query": [VP, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
+                              ctypes.POINTER(ctypes.c_longlong)],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -107,6 +111,18 @@ class DeviceArray:
     def offset(self, words):
         return VP(self.ptr + 8 * int(words))
 
+    def view(self, offset_words, shape):
+        """A non-owning view of part of this buffer."""
+        v = DeviceArray.__new__(DeviceArray)
+        v.ctx, v.shape = self.ctx, tuple(shape)
+        v.size = int(np.prod(v.shape))
+        v.nbytes = v.size * 8
+        assert offset_words + v.size <= self.size
+        v.ptr = self.ptr + 8 * int(offset_words)
+        v.owner = self
+        v.is_view = True
+        return v
+
     def upload(self, arr):
         a = np.ascontiguousarray(arr, dtype=np.uint64)
         assert a.size == self.size, (a.shape, self.shape)
@@ -119,13 +135,16 @@ class DeviceArray:
         return out
 
     def free(self):
+        if getattr(self, "is_view", False):
+            self.ptr = 0
+            return
         if self.ptr:
             lib().shg_free(self.ctx.h, self.p)
             self.ptr = 0
 
     def __del__(self):
         try:
-            if self.ptr and self.ctx.h:
+            if not getattr(self, "is_view", False) and self.ptr and self.ctx.h:
                 lib().shg_free(self.ctx.h, VP(self.ptr))
                 self.ptr = 0
         except Exception:
@@ -240,6 +259,18 @@ class Context:
     def batch_limbs(self, jobs, op):
         arr = (LimbJob * len(jobs))(*[LimbJob(d, o, p, ln) for (d, o, p, ln) in jobs])
         _check(lib().shg_batch_limbs(self.h, arr, len(jobs), int(op)))
+
+    def profile(self, enable=True):
+        _check(lib().shg_profile_enable(self.h, int(bool(enable))))
+
+    def profile_reset(self):
+        _check(lib().shg_profile_reset(self.h))
+
+    def profile_query(self, kernel):
+        ms = ctypes.c_double()
+        n = ctypes.c_longlong()
+        _check(lib().shg_profile_query(self.h, kernel.encode(), ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
 
     def autotune_ntt(self, log_n, limbs):
         c = ctypes.c_int()
