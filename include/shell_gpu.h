@@ -77,6 +77,8 @@ int shg_alloc(shg_context* ctx, size_t bytes, void** dptr);
 int shg_free(shg_context* ctx, void* dptr);
 int shg_upload(shg_context* ctx, void* dptr, const void* host, size_t bytes);
 int shg_download(shg_context* ctx, void* host, const void* dptr, size_t bytes);
+/* Asynchronous device-to-device copy on the context's stream. */
+int shg_copy(shg_context* ctx, void* dst, const void* src, size_t bytes);
 
 /* ---- primes: MontgomeryIntParams<Uint64>::Create (montgomery.cc:44-59) +
  *      InitializeNttParameters (ntt_parameters.h:183-235) ----------------- */

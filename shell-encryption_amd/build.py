@@ -59,10 +59,17 @@ def build_lib(jobs=8):
 
 
 def build_cpp_tests():
+    """The C++ API test binary; it links the CPU oracle as its checker."""
     src = os.path.join(CPP, "tests", "rns_api_test.cc")
     out = os.path.join(CPP, "tests", "rns_api_test")
-    _run([HIPCC, "-O2", "-std=c++17", "-I" + INCLUDE, "-I" + CPP, src, "-o", out,
-          "-L" + HERE, "-lshellgpu", "-Wl,-rpath," + HERE])
+    oracle_dir = os.path.join(ROOT, "oracle")
+    oracle_obj = os.path.join(CPP, "tests", "shell_oracle_test.o")
+    _run(["gcc", "-O2", "-fPIC", "-std=gnu11", "-c", os.path.join(oracle_dir, "shell_oracle.c"),
+          "-o", oracle_obj])
+    # Host-only C++ over the C-ABI: the system compiler is enough.
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-I" + INCLUDE, "-I" + CPP, "-I" + oracle_dir, src,
+          oracle_obj, "-o", out, "-L" + HERE, "-lshellgpu", "-Wl,-rpath," + HERE])
+    os.remove(oracle_obj)
     return out
 
 

@@ -1,0 +1,354 @@
+// GPU tests of the rlwe C++ API (the drop-in boundary): each case mirrors a
+// reference test (cited) and checks the device results bit for bit against
+// the CPU oracle (oracle/shell_oracle.c, linked into this test binary only).
+//
+// Run: rns_api_test  (needs an MI355X; exits non-zero on the first failure)
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "rlwe/rns_context.h"
+#include "rlwe/rns_relinearization_key.h"
+#include "shell_oracle.h"
+
+using ModularInt = rlwe::MontgomeryInt<uint64_t>;
+using Poly = rlwe::RnsPolynomial<ModularInt>;
+using Ct = rlwe::RnsBgvCiphertext<ModularInt>;
+
+static int g_checks = 0;
+#define CHECK(cond)                                                            \
+  do {                                                                         \
+    ++g_checks;                                                                \
+    if (!(cond)) {                                                             \
+      std::fprintf(stderr, "%s:%d: CHECK failed: %s\n", __FILE__, __LINE__, #cond); \
+      std::exit(1);                                                            \
+    }                                                                          \
+  } while (0)
+#define CHECK_OK(expr)                                                          \
+  do {                                                                          \
+    auto _st = (expr);                                                          \
+    if (!_st.ok()) {                                                            \
+      std::fprintf(stderr, "%s:%d: not ok: %s\n", __FILE__, __LINE__, _st.message().c_str()); \
+      std::exit(1);                                                             \
+    }                                                                           \
+  } while (0)
+#define ASSIGN_OR_DIE(lhs, expr)                                                 \
+  auto lhs##_or = (expr);                                                        \
+  if (!lhs##_or.ok()) {                                                          \
+    std::fprintf(stderr, "%s:%d: not ok: %s\n", __FILE__, __LINE__,              \
+                 lhs##_or.status().message().c_str());                          \
+    std::exit(1);                                                                \
+  }                                                                              \
+  auto lhs = std::move(lhs##_or).value()
+
+static bool IsPrime(uint64_t n) {
+  if (n < 2) return false;
+  for (uint64_t p : {2ull, 3ull, 5ull, 7ull, 11ull, 13ull, 17ull, 19ull, 23ull, 29ull, 31ull, 37ull})
+    if (n % p == 0) return n == p;
+  uint64_t d = n - 1;
+  int s = 0;
+  while (d % 2 == 0) { d /= 2; ++s; }
+  auto mulmod = [n](uint64_t a, uint64_t b) { return (uint64_t)((unsigned __int128)a * b % n); };
+  for (uint64_t a : {2ull, 3ull, 5ull, 7ull, 11ull, 13ull, 17ull, 19ull, 23ull, 29ull, 31ull, 37ull}) {
+    uint64_t x = 1, b = a, e = d;
+    while (e) { if (e & 1) x = mulmod(x, b); b = mulmod(b, b); e >>= 1; }
+    if (x == 1 || x == n - 1) continue;
+    bool comp = true;
+    for (int r = 1; r < s; ++r) { x = mulmod(x, x); if (x == n - 1) { comp = false; break; } }
+    if (comp) return false;
+  }
+  return true;
+}
+
+// The `count` largest primes below 2^50 with q = 1 mod 2^(log_n+1).
+static std::vector<uint64_t> NttPrimes(int log_n, int count) {
+  const uint64_t step = 1ull << (log_n + 1);
+  uint64_t q = ((1ull << 50) - 1) / step * step + 1;
+  std::vector<uint64_t> out;
+  while ((int)out.size() < count) {
+    if (IsPrime(q)) out.push_back(q);
+    q -= step;
+  }
+  return out;
+}
+
+struct OracleChain {
+  std::vector<so_prime*> primes;
+  explicit OracleChain(const std::vector<uint64_t>& qs, int log_n) {
+    for (uint64_t q : qs) {
+      int st = 0;
+      primes.push_back(so_prime_create(q, log_n, &st));
+    }
+  }
+  ~OracleChain() {
+    for (auto* p : primes) so_prime_destroy(p);
+  }
+  so_prime* const* pp() const { return primes.data(); }
+};
+
+static std::vector<uint64_t> Flat(const Poly& p) {
+  std::vector<uint64_t> out;
+  for (const auto& v : p.Coeffs())
+    for (const auto& x : v) out.push_back(x.GetMontgomeryRepresentation());
+  return out;
+}
+
+static Poly FromFlat(const std::vector<uint64_t>& flat, int L, int n, bool ntt) {
+  std::vector<std::vector<ModularInt>> cv(L);
+  for (int i = 0; i < L; ++i)
+    for (int j = 0; j < n; ++j) cv[i].emplace_back(flat[(size_t)i * n + j]);
+  return Poly::Create(std::move(cv), ntt).value();
+}
+
+static std::vector<uint64_t> Uniform(const OracleChain& oc, int L, uint64_t seed) {
+  const size_t n = (size_t)1 << oc.primes[0]->log_n;
+  std::vector<uint64_t> x((size_t)L * n);
+  so_fill_uniform(oc.pp(), L, seed, x.data());
+  return x;
+}
+
+// montgomery_test.cc:342-534 (host MontgomeryInt against the oracle).
+static void TestMontgomery() {
+  for (uint64_t q : NttPrimes(12, 2)) {
+    auto params = rlwe::MontgomeryIntParams<uint64_t>::Create(q).value();
+    so_mont m;
+    so_mont_init(q, &m);
+    CHECK(params->inv_modulus == m.inv_modulus && params->inv_r == m.inv_r);
+    std::mt19937_64 rng(q);
+    for (int i = 0; i < 1000; ++i) {
+      uint64_t a = rng() % q, b = rng() % q;
+      ModularInt x(a), y(b);
+      CHECK(x.Mul(y, params.get()).GetMontgomeryRepresentation() == so_mul(&m, a, b));
+      CHECK(x.Add(y, params.get()).GetMontgomeryRepresentation() == so_add(&m, a, b));
+      CHECK(ModularInt::ImportInt(a, params.get()).value().GetMontgomeryRepresentation() ==
+            so_import(&m, a));
+    }
+  }
+  CHECK(!rlwe::MontgomeryIntParams<uint64_t>::Create(1ull << 62).ok());
+}
+
+// ntt_parameters_test.cc:71-286 (tables against the oracle).
+static void TestNttParameters() {
+  const int log_n = 10;
+  for (uint64_t q : NttPrimes(log_n, 2)) {
+    auto params = rlwe::MontgomeryIntParams<uint64_t>::Create(q).value();
+    auto ntt = rlwe::InitializeNttParameters<ModularInt>(log_n, params.get()).value();
+    int st = 0;
+    so_prime* pr = so_prime_create(q, log_n, &st);
+    for (int i = 0; i < (1 << log_n); ++i) {
+      CHECK(std::get<0>(ntt.psis_bitrev_constant[i]) == pr->psis_bitrev_c[i]);
+      CHECK(std::get<1>(ntt.psis_bitrev_constant[i]) == pr->psis_bitrev_cb[i]);
+      CHECK(std::get<0>(ntt.psis_inv_bitrev_constant[i]) == pr->psis_inv_bitrev_c[i]);
+      CHECK(ntt.bitrevs[i] == pr->bitrevs[i]);
+    }
+    CHECK(ntt.n_inv_ptr->GetMontgomeryRepresentation() == pr->n_inv);
+    so_prime_destroy(pr);
+  }
+  auto params = rlwe::MontgomeryIntParams<uint64_t>::Create(97).value();
+  CHECK(!rlwe::InitializeNttParameters<ModularInt>(6, params.get()).ok());  // 97 != 1 mod 128
+  CHECK(!rlwe::InitializeNttParameters<ModularInt>(16, params.get()).ok());
+}
+
+// rns/rns_polynomial_test.cc:475-557,920-1109
+static void TestRnsPolynomial() {
+  const int log_n = 11, L = 3, n = 1 << log_n;
+  auto qs = NttPrimes(log_n, L);
+  auto ctx = rlwe::RnsContext<ModularInt>::CreateForBgv(log_n, qs, 65537).value();
+  auto moduli = ctx.MainPrimeModuli();
+  OracleChain oc(qs, log_n);
+  auto x = Uniform(oc, L, 5), y = Uniform(oc, L, 6), z = Uniform(oc, L, 7);
+  Poly p = FromFlat(x, L, n, false);
+  // NTT / INTT against the oracle, and the round trip.
+  CHECK_OK(p.ConvertToNttForm(moduli));
+  auto ex = x;
+  so_poly_ntt_fwd(oc.pp(), L, ex.data());
+  CHECK(Flat(p) == ex);
+  auto st = p.ConvertToNttForm(moduli);
+  CHECK(!st.ok() && st.message() == "Polynomial already in NTT form");
+  CHECK(Flat(p) == ex);  // unchanged on failure
+  std::vector<const rlwe::PrimeModulus<ModularInt>*> two(moduli.begin(), moduli.begin() + 2);
+  st = p.NegateInPlace(two);
+  CHECK(!st.ok() && st.message() == "`moduli` must contain 3 RNS moduli.");
+  CHECK_OK(p.ConvertToCoeffForm(moduli));
+  CHECK(Flat(p) == x);
+  CHECK_OK(p.ConvertToNttForm(moduli));
+  // Pointwise operations.
+  Poly a = FromFlat(y, L, n, true), b = FromFlat(z, L, n, true);
+  Poly s = p.Add(a, moduli).value();
+  auto es = ex;
+  so_poly_pointwise(oc.pp(), L, SO_ADD, es.data(), y.data(), nullptr);
+  CHECK(Flat(s) == es);
+  CHECK_OK(s.FusedMulAddInPlace(a, b, moduli));
+  so_poly_pointwise(oc.pp(), L, SO_FMA, es.data(), y.data(), z.data());
+  CHECK(Flat(s) == es);
+  CHECK_OK(s.MulInPlace(a, moduli));
+  so_poly_pointwise(oc.pp(), L, SO_MUL, es.data(), y.data(), nullptr);
+  CHECK(Flat(s) == es);
+  CHECK_OK(s.MulInPlace(uint64_t{12345}, moduli));
+  so_poly_mul_int(oc.pp(), L, es.data(), 12345);
+  CHECK(Flat(s) == es);
+  CHECK_OK(s.NegateInPlace(moduli));
+  so_poly_pointwise(oc.pp(), L, SO_NEG, es.data(), nullptr, nullptr);
+  CHECK(Flat(s) == es);
+  Poly zero = Poly::CreateZero(log_n, moduli).value();
+  CHECK_OK(zero.NegateInPlace(moduli));
+  CHECK(zero.Coeffs()[0][0].GetMontgomeryRepresentation() == qs[0]);  // Negate(0) = q
+  // a * 1 = a
+  Poly one = Poly::CreateOne(log_n, moduli).value();
+  CHECK(a.Mul(one, moduli).value() == a);
+  // Substitute: power checks and values.
+  CHECK(!p.Substitute(2, moduli).ok());
+  CHECK(!p.Substitute(2 * n + 1, moduli).ok());
+  CHECK(p.Substitute(1, moduli).value() == p);
+  Poly sub = p.Substitute(5, moduli).value();
+  std::vector<uint64_t> esub(ex.size());
+  CHECK(so_poly_substitute(oc.pp(), L, 5, ex.data(), esub.data()) == 0);
+  CHECK(Flat(sub) == esub);
+  Poly coeff = FromFlat(x, L, n, false);
+  CHECK(!coeff.Substitute(5, moduli).ok());
+  CHECK(!coeff.MulInPlace(a, moduli).ok());
+  std::printf("  RnsPolynomial: ok\n");
+}
+
+// rns/rns_polynomial_test.cc:1150-1332, rns/rns_gadget_test.cc:174-242
+static void TestModReduceAndGadget() {
+  const int log_n = 10, L = 4, n = 1 << log_n;
+  auto qs = NttPrimes(log_n, L);
+  auto ctx = rlwe::RnsContext<ModularInt>::CreateForBgv(log_n, qs, 65537).value();
+  auto moduli = ctx.MainPrimeModuli();
+  OracleChain oc(qs, log_n);
+  auto x = Uniform(oc, L, 11);
+  for (bool ntt : {true, false}) {
+    Poly p = FromFlat(x, L, n, ntt);
+    auto ql_inv = ctx.MainPrimeModulusInverseResidues()[L - 1].Prefix(L - 1);
+    CHECK_OK(p.ModReduceLsb(65537, ql_inv, moduli));
+    CHECK(p.NumModuli() == L - 1);
+    std::vector<uint64_t> qi;
+    for (auto& v : ql_inv.zs) qi.push_back(v.GetMontgomeryRepresentation());
+    auto ex = x;
+    CHECK(so_poly_mod_reduce_lsb(oc.pp(), L, 65537, qi.data(), ntt ? 1 : 0, ex.data()) == 0);
+    ex.resize((size_t)(L - 1) * n);
+    CHECK(Flat(p) == ex);
+  }
+  std::vector<size_t> log_bs(L, 10);
+  auto q_hats = ctx.MainPrimeModulusComplements(L - 1).value();
+  auto q_hat_invs = ctx.MainPrimeModulusCrtFactors(L - 1).value();
+  auto gadget = rlwe::RnsGadget<ModularInt>::Create(log_n, log_bs, q_hats, q_hat_invs, moduli).value();
+  CHECK(gadget.Dimension() == 20);
+  Poly a = FromFlat(x, L, n, false);
+  auto digits = gadget.Decompose(a, moduli).value();
+  std::vector<uint64_t> ed((size_t)gadget.Dimension() * L * n);
+  int lb[4] = {10, 10, 10, 10};
+  so_gadget_decompose(oc.pp(), L, lb, x.data(), ed.data());
+  for (int j = 0; j < gadget.Dimension(); ++j) {
+    auto f = Flat(digits[j]);
+    CHECK(std::equal(f.begin(), f.end(), ed.begin() + (size_t)j * L * n));
+  }
+  // <g, g^-1(a)> = a
+  Poly acc = Poly::CreateZero(log_n, moduli, false).value();
+  for (int j = 0; j < gadget.Dimension(); ++j) {
+    Poly t = digits[j];
+    CHECK_OK(t.MulInPlace(gadget.Component(j), moduli));
+    CHECK_OK(acc.AddInPlace(t, moduli));
+  }
+  CHECK(Flat(acc) == x);
+  std::printf("  ModReduceLsb and RnsGadget: ok\n");
+}
+
+// rns/rns_relinearization_key_test.cc:424-500, rns/rns_galois_key_test.cc:401-462,
+// rns/rns_bgv_ciphertext_test.cc:293-329,660-713, given the key matrices.
+static void TestKeySwitching(int log_n, int L, int batch) {
+  const int n = 1 << log_n;
+  auto qs = NttPrimes(log_n, L);
+  auto ctx = rlwe::RnsContext<ModularInt>::CreateForBgv(log_n, qs, 65537).value();
+  auto moduli = ctx.MainPrimeModuli();
+  OracleChain oc(qs, log_n);
+  std::vector<size_t> log_bs(L, 10);
+  auto gadget = rlwe::RnsGadget<ModularInt>::Create(
+      log_n, log_bs, ctx.MainPrimeModulusComplements(L - 1).value(),
+      ctx.MainPrimeModulusCrtFactors(L - 1).value(), moduli).value();
+  const int k = gadget.Dimension();
+  auto error_params = rlwe::RnsErrorParams<ModularInt>::Create(log_n, moduli, 17, 4.0).value();
+  std::vector<uint64_t> kas((size_t)k * L * n), kbs((size_t)k * L * n);
+  std::vector<Poly> pa, pb;
+  for (int j = 0; j < k; ++j) {
+    auto va = Uniform(oc, L, 100 + j), vb = Uniform(oc, L, 200 + j);
+    std::copy(va.begin(), va.end(), kas.begin() + (size_t)j * L * n);
+    std::copy(vb.begin(), vb.end(), kbs.begin() + (size_t)j * L * n);
+    pa.push_back(FromFlat(va, L, n, true));
+    pb.push_back(FromFlat(vb, L, n, true));
+  }
+  auto relin = rlwe::RnsRelinKey<ModularInt>::CreateFromKeyComponents(pa, pb, 2, &gadget, moduli).value();
+  auto galois = rlwe::RnsGaloisKey<ModularInt>::CreateFromKeyComponents(pa, pb, 5, &gadget, moduli).value();
+  std::vector<Ct> cts;
+  std::vector<std::vector<uint64_t>> raw;
+  for (int c = 0; c < batch; ++c) {
+    std::vector<Poly> comps;
+    std::vector<uint64_t> all;
+    for (int i = 0; i < 2; ++i) {
+      auto v = Uniform(oc, L, 1000 + 10 * c + i);
+      all.insert(all.end(), v.begin(), v.end());
+      comps.push_back(FromFlat(v, L, n, true));
+    }
+    raw.push_back(all);
+    cts.emplace_back(std::move(comps), moduli, 1, 1.0, &error_params);
+  }
+  // Tensor product, then relinearization (single and batched).
+  ASSIGN_OR_DIE(prod, cts[0] * cts[batch - 1]);
+  CHECK(prod.Degree() == 2);
+  std::vector<uint64_t> etp((size_t)3 * L * n);
+  so_tensor_product(oc.pp(), L, 2, raw[0].data(), 2, raw[batch - 1].data(), etp.data());
+  for (int i = 0; i < 3; ++i) {
+    auto f = Flat(prod.Components()[i]);
+    CHECK(std::equal(f.begin(), f.end(), etp.begin() + (size_t)i * L * n));
+  }
+  ASSIGN_OR_DIE(rel, relin.ApplyTo(prod));
+  std::vector<uint64_t> e0((size_t)L * n), e1((size_t)L * n);
+  int lb[64];
+  for (int i = 0; i < L; ++i) lb[i] = 10;
+  CHECK(so_relin_apply(oc.pp(), L, lb, 2, etp.data(), 7, kas.data(), kbs.data(), e0.data(), e1.data()) == 0);
+  CHECK(Flat(rel.Components()[0]) == e0 && Flat(rel.Components()[1]) == e1);
+  CHECK(rel.Degree() == 1 && rel.PowerOfS() == 1);
+  CHECK(rel.Error() == prod.Error() + error_params.BoundOnGadgetBasedKeySwitching(1, 10, k));
+  std::vector<Ct> prods;
+  for (int c = 0; c < batch; ++c) prods.push_back((cts[c] * cts[(c + 1) % batch]).value());
+  std::vector<const Ct*> ptrs;
+  for (auto& c : prods) ptrs.push_back(&c);
+  ASSIGN_OR_DIE(rels, relin.ApplyToBatch(ptrs));
+  for (int c = 0; c < batch; ++c) {
+    ASSIGN_OR_DIE(one, relin.ApplyTo(prods[c]));
+    CHECK(one.Components()[0] == rels[c].Components()[0]);
+    CHECK(one.Components()[1] == rels[c].Components()[1]);
+  }
+  // Galois: substitution then key switch.
+  ASSIGN_OR_DIE(sub, cts[0].Substitute(5));
+  CHECK(sub.PowerOfS() == 5);
+  CHECK(!relin.ApplyTo(sub).ok());  // power of s must be 1
+  ASSIGN_OR_DIE(gal, galois.ApplyTo(sub));
+  std::vector<uint64_t> sraw((size_t)2 * L * n);
+  so_poly_substitute(oc.pp(), L, 5, raw[0].data(), sraw.data());
+  so_poly_substitute(oc.pp(), L, 5, raw[0].data() + (size_t)L * n, sraw.data() + (size_t)L * n);
+  CHECK(so_galois_apply(oc.pp(), L, lb, sraw.data(), sraw.data() + (size_t)L * n, 1, kas.data(),
+                        kbs.data(), e0.data(), e1.data()) == 0);
+  CHECK(Flat(gal.Components()[0]) == e0 && Flat(gal.Components()[1]) == e1);
+  CHECK(!galois.ApplyTo(cts[0]).ok());  // power mismatch
+  // Modulus switch of the relinearized ciphertext.
+  auto ql_inv = ctx.MainPrimeModulusInverseResidues()[L - 1].Prefix(L - 1);
+  CHECK_OK(rel.ModReduce(65537, ql_inv));
+  CHECK(rel.NumModuli() == L - 1);
+  std::printf("  key switching at n=2^%d, L=%d, k=%d, batch %d: ok\n", log_n, L, k, batch);
+}
+
+int main() {
+  TestMontgomery();
+  TestNttParameters();
+  TestRnsPolynomial();
+  TestModReduceAndGadget();
+  TestKeySwitching(10, 3, 4);
+  TestKeySwitching(14, 8, 2);
+  std::printf("rns_api_test: %d checks passed\n", g_checks);
+  return 0;
+}

@@ -279,6 +279,13 @@ int shg_download(shg_context* ctx, void* host, const void* dptr, size_t bytes) {
   return e == hipSuccess ? SHG_OK : hip_fail(e, "download");
 }
 
+int shg_copy(shg_context* ctx, void* dst, const void* src, size_t bytes) {
+  if (!ctx || (bytes && (!dst || !src))) return fail(SHG_E_INVALID_ARGUMENT, "null argument.");
+  if (!bytes || dst == src) return SHG_OK;
+  hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream);
+  return e == hipSuccess ? SHG_OK : hip_fail(e, "copy");
+}
+
 int shg_prime_create(shg_context* ctx, uint64_t q, int log_n, int* prime) {
   if (!ctx || !prime) return fail(SHG_E_INVALID_ARGUMENT, "null argument.");
   std::lock_guard<std::mutex> lock(ctx->mu);
