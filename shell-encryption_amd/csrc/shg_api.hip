@@ -48,7 +48,7 @@ struct PrimeRec {
   int log_n = 0;
   std::unique_ptr<const Params> mp;
   rlwe::NttParameters<ModInt> ntt;
-  void* d_tables = nullptr;  // fw, iw (2n u64 each), fwd, iwd (2n double each)
+  void* d_tables = nullptr;  // fw, iw (2n u64 each: c, cb pairs), fwd, iwd (n doubles each)
   PrimeDev dev{};
 };
 
@@ -169,6 +169,8 @@ int resolve_variant(shg_context* ctx, int variant, const int* primes, int num_li
 }
 
 bool aligned_ptr(const void* p) { return p != nullptr && ((uintptr_t)p % 8) == 0; }
+// Transform and key-switch kernels move residues in 16-byte pairs.
+bool aligned16(const void* p) { return p != nullptr && ((uintptr_t)p % 16) == 0; }
 
 int build_gadget(shg_context* ctx, const int* primes, int num_limbs, const int* log_bs,
                  shg::GadgetDesc* gd) {
@@ -306,26 +308,24 @@ int shg_prime_create(shg_context* ctx, uint64_t q, int log_n, int* prime) {
   rec->ntt = std::move(ntt_or).value();
   const size_t n = (size_t)1 << log_n;
   std::vector<uint64_t> fw(2 * n), iw(2 * n);
-  std::vector<double> fwd(2 * n), iwd(2 * n);
+  std::vector<double> fwd(n), iwd(n);
   const double qd = (double)q;
   for (size_t i = 0; i < n; ++i) {
     fw[2 * i] = std::get<0>(rec->ntt.psis_bitrev_constant[i]);
     fw[2 * i + 1] = std::get<1>(rec->ntt.psis_bitrev_constant[i]);
     iw[2 * i] = std::get<0>(rec->ntt.psis_inv_bitrev_constant[i]);
     iw[2 * i + 1] = std::get<1>(rec->ntt.psis_inv_bitrev_constant[i]);
-    fwd[2 * i] = (double)fw[2 * i];
-    fwd[2 * i + 1] = (double)fw[2 * i] / qd;
-    iwd[2 * i] = (double)iw[2 * i];
-    iwd[2 * i + 1] = (double)iw[2 * i] / qd;
+    fwd[i] = (double)fw[2 * i];
+    iwd[i] = (double)iw[2 * i];
   }
-  const size_t bytes = 4 * 2 * n * 8;
+  const size_t bytes = 6 * n * 8;
   hipError_t e = hipMalloc(&rec->d_tables, bytes);
   if (e != hipSuccess) return fail(SHG_E_NO_MEMORY, "twiddle table allocation failed");
   char* base = (char*)rec->d_tables;
   hipMemcpy(base, fw.data(), 2 * n * 8, hipMemcpyHostToDevice);
   hipMemcpy(base + 2 * n * 8, iw.data(), 2 * n * 8, hipMemcpyHostToDevice);
-  hipMemcpy(base + 4 * n * 8, fwd.data(), 2 * n * 8, hipMemcpyHostToDevice);
-  e = hipMemcpy(base + 6 * n * 8, iwd.data(), 2 * n * 8, hipMemcpyHostToDevice);
+  hipMemcpy(base + 4 * n * 8, fwd.data(), n * 8, hipMemcpyHostToDevice);
+  e = hipMemcpy(base + 5 * n * 8, iwd.data(), n * 8, hipMemcpyHostToDevice);
   if (e != hipSuccess) return hip_fail(e, "twiddle upload");
   PrimeDev& d = rec->dev;
   const Params& P = *rec->mp;
@@ -352,7 +352,7 @@ int shg_prime_create(shg_context* ctx, uint64_t q, int log_n, int* prime) {
   d.fw = (const uint64_t*)base;
   d.iw = (const uint64_t*)(base + 2 * n * 8);
   d.fwd = (const double*)(base + 4 * n * 8);
-  d.iwd = (const double*)(base + 6 * n * 8);
+  d.iwd = (const double*)(base + 5 * n * 8);
   const int id = (int)ctx->primes.size();
   if (id >= ctx->d_capacity) {
     PrimeDev* bigger = nullptr;
@@ -409,7 +409,8 @@ static int ntt_common(shg_context* ctx, bool inverse, uint64_t* polys, int count
   if (rc) return rc;
   if (count < 0) return fail(SHG_E_INVALID_ARGUMENT, "count must be non-negative.");
   if (count == 0) return SHG_OK;
-  if (!aligned_ptr(polys)) return fail(SHG_E_INVALID_ARGUMENT, "polynomial data must not be null.");
+  if (!aligned16(polys))
+    return fail(SHG_E_INVALID_ARGUMENT, "polynomial data must be a non-null 16-byte aligned pointer.");
   const long long rows = (long long)count * num_limbs;
   if ((rows << (log_n > shg::kMaxLogM ? log_n - shg::kMaxLogM : 0)) > 0x7fffffffLL)
     return fail(SHG_E_INVALID_ARGUMENT, "batch too large for one launch.");
@@ -712,9 +713,9 @@ int shg_relin_apply(shg_context* ctx, const uint64_t* comps, int degree, int ntt
     return fail(SHG_E_INVALID_ARGUMENT,
                 "`ciphertext` degree is larger than degree of this relinearization key.");
   if (count <= 0) return count == 0 ? SHG_OK : fail(SHG_E_INVALID_ARGUMENT, "count must be non-negative.");
-  if (!aligned_ptr(comps) || !aligned_ptr(out_c0) || !aligned_ptr(out_c1) ||
-      (degree >= 2 && (!aligned_ptr(key_as) || !aligned_ptr(key_bs))))
-    return fail(SHG_E_INVALID_ARGUMENT, "data must not be null.");
+  if (!aligned16(comps) || !aligned16(out_c0) || !aligned16(out_c1) ||
+      (degree >= 2 && (!aligned16(key_as) || !aligned16(key_bs))))
+    return fail(SHG_E_INVALID_ARGUMENT, "data must be non-null 16-byte aligned pointers.");
   const size_t poly_words = (size_t)num_limbs << log_n;
   if (degree == 1) {
     hipStream_t s = ctx->stream;
@@ -747,9 +748,9 @@ int shg_galois_apply(shg_context* ctx, const uint64_t* comps, int c1_is_ntt, con
   rc = build_gadget(ctx, primes, num_limbs, host_log_bs, &gd);
   if (rc) return rc;
   if (count <= 0) return count == 0 ? SHG_OK : fail(SHG_E_INVALID_ARGUMENT, "count must be non-negative.");
-  if (!aligned_ptr(comps) || !aligned_ptr(out_c0) || !aligned_ptr(out_c1) || !aligned_ptr(key_as) ||
-      !aligned_ptr(key_bs))
-    return fail(SHG_E_INVALID_ARGUMENT, "data must not be null.");
+  if (!aligned16(comps) || !aligned16(out_c0) || !aligned16(out_c1) || !aligned16(key_as) ||
+      !aligned16(key_bs))
+    return fail(SHG_E_INVALID_ARGUMENT, "data must be non-null 16-byte aligned pointers.");
   const int src_comp = 1, src_ntt = c1_is_ntt ? 1 : 0, base = 0;
   return keyswitch_common(ctx, comps, 2, &src_comp, &src_ntt, &base, 1, false, key_as, key_bs,
                           host_log_bs, out_c0, out_c1, count, num_limbs, primes, lp, log_n, gd);

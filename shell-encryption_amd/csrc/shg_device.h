@@ -18,7 +18,7 @@ namespace shg {
 // Per-(q, log_n) constants, resident in device memory.  Twiddle tables are
 // the reference's psis_bitrev_constant / psis_inv_bitrev_constant
 // (ntt_parameters.h:160-179) interleaved as (c, cb) pairs, plus FP64 copies
-// interleaved as (w, w/q).
+// of the values c.
 struct PrimeDev {
   uint64_t q;
   uint64_t inv_modulus;    // q * inv_modulus = -1 mod 2^64 (REDC)
@@ -41,8 +41,8 @@ struct PrimeDev {
   int log_q;
   const uint64_t* fw;      // [2n] forward (c, cb), bit-reversed psi powers
   const uint64_t* iw;      // [2n] inverse (c, cb)
-  const double* fwd;       // [2n] forward (w, w/q) as doubles
-  const double* iwd;       // [2n] inverse (w, w/q) as doubles
+  const double* fwd;       // [n] forward twiddle values as doubles
+  const double* iwd;       // [n] inverse twiddle values as doubles
 };
 
 struct LimbPrimes {

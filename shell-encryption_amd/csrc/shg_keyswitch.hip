@@ -22,6 +22,11 @@
 #include "shg_kernels.h"
 #include "shg_ntt_core.h"
 
+// Per-thread word count (log2) of the 2^13-word key-switch slices.
+#ifndef SHG_KS_LOGE
+#define SHG_KS_LOGE 4
+#endif
+
 namespace shg {
 
 // LOGN = LOGM + HB.  With HB = 1 (n = 2^14) the limb is split between two
@@ -50,6 +55,13 @@ __global__ void __launch_bounds__((1 << LOGM) >> LOGE)
   const size_t limb_off = ((size_t)l << LOGN) + ((size_t)h << LOGM);
   const size_t poly_words = (size_t)num_limbs << LOGN;
 
+  // Thread t accumulates the words the last register pass leaves it:
+  // for each group gamma = gg*T + t, the 2^R consecutive positions
+  // (gamma << R) + e of the slice (FwdLast); they are read and written with
+  // 16-byte accesses.
+  constexpr int R = FwdLast<LOGM, LOGE>::R_LAST;
+  constexpr int G = FwdLast<LOGM, LOGE>::G_LAST;
+  static_assert(R >= 1, "key switching needs at least one register stage");
   Acc acc0[E], acc1[E];
   uint64_t* o0 = ka.out0 + ct * ka.out_stride + limb_off;
   uint64_t* o1 = ka.out1 + ct * ka.out_stride + limb_off;
@@ -57,9 +69,17 @@ __global__ void __launch_bounds__((1 << LOGM) >> LOGE)
     const uint64_t* i0 = ka.accumulate ? o0 : (ka.c0 ? ka.c0 + ct * ka.c0_stride + limb_off : nullptr);
     const uint64_t* i1 = ka.accumulate ? o1 : (ka.c1 ? ka.c1 + ct * ka.c1_stride + limb_off : nullptr);
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-      acc0[e] = ar.acc_in(i0 ? i0[e * T + t] : 0);
-      acc1[e] = ar.acc_in(i1 ? i1[e * T + t] : 0);
+    for (int gg = 0; gg < G; ++gg) {
+      const size_t base = (size_t)(gg * T + t) << R;
+#pragma unroll
+      for (int e = 0; e < (1 << R); e += 2) {
+        u64x2 a = i0 ? *reinterpret_cast<const u64x2*>(i0 + base + e) : u64x2{0, 0};
+        u64x2 b = i1 ? *reinterpret_cast<const u64x2*>(i1 + base + e) : u64x2{0, 0};
+        acc0[gg * (1 << R) + e] = ar.acc_in(a.x);
+        acc0[gg * (1 << R) + e + 1] = ar.acc_in(a.y);
+        acc1[gg * (1 << R) + e] = ar.acc_in(b.x);
+        acc1[gg * (1 << R) + e + 1] = ar.acc_in(b.y);
+      }
     }
   }
 
@@ -83,18 +103,24 @@ __global__ void __launch_bounds__((1 << LOGM) >> LOGE)
         if (ar.fwd_reduce_every == 1) ar.fwd_reduce(x[e]);
       }
     }
-    FwdPasses<LOGM, LOGE, 0, A>::run(x, lds, t, h, HB, ar);
+    FwdPasses<LOGM, LOGE, 0, A, false>::run(x, lds, t, h, HB, ar);
     const uint64_t* kb = ka.key_b + (size_t)(ka.key_base + j) * poly_words + limb_off;
     const uint64_t* kav = ka.key_a + (size_t)(ka.key_base + j) * poly_words + limb_off;
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-      const int p = e * T + t;
-      const auto d = ar.prep(lds[lpad(p)]);
-      ar.mac(acc0[e], d, kb[p], P);
-      ar.mac(acc1[e], d, kav[p], P);
-      // Keep key loads from being hoisted far ahead of their use: the
-      // accumulators already hold 4*E VGPRs.
-      if ((e & 3) == 3) __builtin_amdgcn_sched_barrier(0);
+    for (int gg = 0; gg < G; ++gg) {
+      const size_t base = (size_t)(gg * T + t) << R;
+#pragma unroll
+      for (int e = 0; e < (1 << R); e += 2) {
+        const u64x2 b = *reinterpret_cast<const u64x2*>(kb + base + e);
+        const u64x2 a = *reinterpret_cast<const u64x2*>(kav + base + e);
+        const int r = gg * (1 << R) + e;
+        const auto d0 = ar.prep(x[r]);
+        const auto d1 = ar.prep(x[r + 1]);
+        ar.mac(acc0[r], d0, b.x, P);
+        ar.mac(acc0[r + 1], d1, b.y, P);
+        ar.mac(acc1[r], d0, a.x, P);
+        ar.mac(acc1[r + 1], d1, a.y, P);
+      }
     }
     if (((j + 1) % A::kAccReduceEvery) == 0) {
 #pragma unroll
@@ -103,12 +129,23 @@ __global__ void __launch_bounds__((1 << LOGM) >> LOGE)
         ar.acc_reduce(acc1[e]);
       }
     }
-    __syncthreads();
+    // The next digit's first pass writes LDS only after its registers are
+    // computed; all reads of this digit's last exchange are done by then
+    // (they precede the last pass), so no barrier is needed here.
   }
 #pragma unroll
-  for (int e = 0; e < E; ++e) {
-    o0[e * T + t] = ar.acc_out(acc0[e]);
-    o1[e * T + t] = ar.acc_out(acc1[e]);
+  for (int gg = 0; gg < G; ++gg) {
+    const size_t base = (size_t)(gg * T + t) << R;
+#pragma unroll
+    for (int e = 0; e < (1 << R); e += 2) {
+      u64x2 a, b;
+      a.x = ar.acc_out(acc0[gg * (1 << R) + e]);
+      a.y = ar.acc_out(acc0[gg * (1 << R) + e + 1]);
+      b.x = ar.acc_out(acc1[gg * (1 << R) + e]);
+      b.y = ar.acc_out(acc1[gg * (1 << R) + e + 1]);
+      *reinterpret_cast<u64x2*>(o0 + base + e) = a;
+      *reinterpret_cast<u64x2*>(o1 + base + e) = b;
+    }
   }
 }
 
@@ -140,7 +177,7 @@ static hipError_t dispatch_ks(int logn, const KeySwitchArgs& ka, const GadgetDes
     case 11: return launch_ks<11, 4, A>(ka, gd, primes, lp, count, num_limbs, s);
     case 12: return launch_ks<12, 4, A>(ka, gd, primes, lp, count, num_limbs, s);
     case 13: return launch_ks<13, 4, A>(ka, gd, primes, lp, count, num_limbs, s);
-    case 14: return launch_ks<13, 4, A, 1>(ka, gd, primes, lp, count, num_limbs, s);
+    case 14: return launch_ks<13, SHG_KS_LOGE, A, 1>(ka, gd, primes, lp, count, num_limbs, s);
     default: return hipErrorInvalidValue;
   }
 }

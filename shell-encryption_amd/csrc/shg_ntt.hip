@@ -13,6 +13,18 @@
 #include "shg_kernels.h"
 #include "shg_ntt_core.h"
 
+// Per-thread word count (log2) for 2^14-word slices: 32 words with 512
+// threads (three register passes) by default.
+#ifndef SHG_LOGE_14
+#define SHG_LOGE_14 5
+#endif
+#ifndef SHG_FWD_DIRECT_STORE
+#define SHG_FWD_DIRECT_STORE 0
+#endif
+#ifndef SHG_INV_DIRECT_LOAD
+#define SHG_INV_DIRECT_LOAD 0
+#endif
+
 namespace shg {
 
 // ---------------------------------------------------------------------------
@@ -45,10 +57,34 @@ __global__ void __launch_bounds__((1 << LOGM) >> LOGE)
 #pragma unroll
     for (int e = 0; e < E; ++e) ar.fwd_reduce(x[e]);
   }
-  FwdPasses<LOGM, LOGE, 0, A>::run(x, lds, t, h, hbits, ar);
-  // Coalesced store from LDS.
+#if SHG_FWD_DIRECT_STORE
+  FwdPasses<LOGM, LOGE, 0, A, false>::run(x, lds, t, h, hbits, ar);
+  // The last pass left 2^R consecutive words per group in VGPRs: store them
+  // with 16-byte writes.
+  constexpr int R = FwdLast<LOGM, LOGE>::R_LAST;
+  constexpr int G = FwdLast<LOGM, LOGE>::G_LAST;
+#pragma unroll
+  for (int gg = 0; gg < G; ++gg) {
+    uint64_t* dst = g + ((size_t)(gg * T + t) << R);
+    if constexpr (R >= 1) {
+#pragma unroll
+      for (int e = 0; e < (1 << R); e += 2) {
+        u64x2 v;
+        v.x = ar.fwd_out(x[gg * (1 << R) + e]);
+        v.y = ar.fwd_out(x[gg * (1 << R) + e + 1]);
+        *reinterpret_cast<u64x2*>(dst + e) = v;
+      }
+    } else {
+      dst[0] = ar.fwd_out(x[gg]);
+    }
+  }
+#else
+  // The last pass goes through LDS so that lanes store consecutive words
+  // (measured faster than 16-byte thread-contiguous stores).
+  FwdPasses<LOGM, LOGE, 0, A, true>::run(x, lds, t, h, hbits, ar);
 #pragma unroll
   for (int e = 0; e < E; ++e) g[e * T + t] = ar.fwd_out(lds[lpad(e * T + t)]);
+#endif
 }
 
 template <int LOGM, int LOGE, class A>
@@ -67,11 +103,38 @@ __global__ void __launch_bounds__((1 << LOGM) >> LOGE)
   const A ar(P);
   uint64_t* g = data + ((size_t)row << logn) + ((size_t)h << LOGM);
   const int t = threadIdx.x;
+#if SHG_INV_DIRECT_LOAD
+  // The first pass toggles the low local bits, so each thread loads its
+  // groups' consecutive words straight from HBM with 16-byte reads.
+  V x[E];
+  {
+    constexpr int R0 = LOGM < LOGE ? LOGM : LOGE;
+    constexpr int G0 = E >> R0;
+#pragma unroll
+    for (int gg = 0; gg < G0; ++gg) {
+      const uint64_t* src = g + ((size_t)(gg * T + t) << R0);
+      if constexpr (R0 >= 1) {
+#pragma unroll
+        for (int e = 0; e < (1 << R0); e += 2) {
+          const u64x2 v = *reinterpret_cast<const u64x2*>(src + e);
+          x[gg * (1 << R0) + e] = ar.load(v.x);
+          x[gg * (1 << R0) + e + 1] = ar.load(v.y);
+        }
+      } else {
+        x[gg] = ar.load(src[0]);
+      }
+    }
+  }
+  InvPasses<LOGM, LOGE, 0, A, true>::run(x, lds, t, h, logn, ar);
+#else
+  // Coalesced load through LDS (measured faster than thread-contiguous
+  // 16-byte reads).
 #pragma unroll
   for (int e = 0; e < E; ++e) lds[lpad(e * T + t)] = ar.load(g[e * T + t]);
   __syncthreads();
   V x[E];
-  InvPasses<LOGM, LOGE, 0, A>::run(x, lds, t, h, logn, ar);
+  InvPasses<LOGM, LOGE, 0, A, false>::run(x, lds, t, h, logn, ar);
+#endif
   // The last pass toggles the top local bits: word (e << I0) | gamma.
   constexpr int I0 = InvLast<LOGM, LOGE, 0, A>::value;
   constexpr int R = LOGM - I0;
@@ -182,7 +245,7 @@ static hipError_t dispatch_fused(int logm, bool inverse, uint64_t* data, const P
     case 11: return launch_fused<11, 4, A>(inverse, data, primes, lp, rows, num_limbs, logn, scale, s);
     case 12: return launch_fused<12, 4, A>(inverse, data, primes, lp, rows, num_limbs, logn, scale, s);
     case 13: return launch_fused<13, 4, A>(inverse, data, primes, lp, rows, num_limbs, logn, scale, s);
-    case 14: return launch_fused<14, 5, A>(inverse, data, primes, lp, rows, num_limbs, logn, scale, s);
+    case 14: return launch_fused<14, SHG_LOGE_14, A>(inverse, data, primes, lp, rows, num_limbs, logn, scale, s);
     default: return hipErrorInvalidValue;
   }
 }

@@ -22,6 +22,10 @@
 
 #include "shg_device.h"
 
+#ifndef SHG_STAGE_BARRIER
+#define SHG_STAGE_BARRIER 0
+#endif
+
 namespace shg {
 
 
@@ -54,8 +58,12 @@ struct IntArith {
   struct Tw {
     uint64_t c, cb;
   };
-  __device__ __forceinline__ Tw fwd_tw(int i) const { return Tw{fw[2 * i], fw[2 * i + 1]}; }
-  __device__ __forceinline__ Tw inv_tw(int i) const { return Tw{iw[2 * i], iw[2 * i + 1]}; }
+  using TwRaw = Tw;  // (c, cb) as stored
+  __device__ __forceinline__ TwRaw fwd_tw_raw(int i) const { return Tw{fw[2 * i], fw[2 * i + 1]}; }
+  __device__ __forceinline__ TwRaw inv_tw_raw(int i) const { return Tw{iw[2 * i], iw[2 * i + 1]}; }
+  __device__ __forceinline__ Tw tw(const TwRaw& r) const { return r; }
+  __device__ __forceinline__ Tw fwd_tw(int i) const { return fwd_tw_raw(i); }
+  __device__ __forceinline__ Tw inv_tw(int i) const { return inv_tw_raw(i); }
   __device__ __forceinline__ V load(uint64_t x) const { return x; }
   // Cooley-Tukey: values grow by less than 2q per stage.
   __device__ __forceinline__ void ct(V& u, V& v, const Tw& w) const {
@@ -109,8 +117,14 @@ struct F64Arith {
   struct Tw {
     double w, wq;
   };
-  __device__ __forceinline__ Tw fwd_tw(int i) const { return Tw{fw[2 * i], fw[2 * i + 1]}; }
-  __device__ __forceinline__ Tw inv_tw(int i) const { return Tw{iw[2 * i], iw[2 * i + 1]}; }
+  // Tables hold w only (8 bytes per twiddle); w/q = w * fl(1/q) is formed
+  // on use, within 2^-52 relative error, which the bounds above absorb.
+  using TwRaw = double;
+  __device__ __forceinline__ TwRaw fwd_tw_raw(int i) const { return fw[i]; }
+  __device__ __forceinline__ TwRaw inv_tw_raw(int i) const { return iw[i]; }
+  __device__ __forceinline__ Tw tw(TwRaw r) const { return Tw{r, r * qinv}; }
+  __device__ __forceinline__ Tw fwd_tw(int i) const { return tw(fw[i]); }
+  __device__ __forceinline__ Tw inv_tw(int i) const { return tw(iw[i]); }
   __device__ __forceinline__ V load(uint64_t x) const { return u64_to_f64(x); }
   __device__ __forceinline__ void ct(V& u, V& v, const Tw& w) const {
     const double t = f64_mulmod(v, w.w, w.wq, q);
@@ -166,28 +180,61 @@ struct F64Arith {
 // G = E / 2^R groups of 2^R words; group gamma = gg*T + t.
 // ---------------------------------------------------------------------------
 
+// Twiddles used by one pass, per group gg: stage j needs 2^j of them
+// (blocks bh), stored at [gg][2^j - 1 + bh].
+template <int LOGE, int R>
+struct PassTw {
+  static constexpr int G = (1 << LOGE) >> R;
+  static constexpr int PER = (1 << R) - 1;
+  static constexpr int N = G * PER;
+};
+
 // Forward pass: stages S0..S0+R-1 (global stage s toggles index bit
 // LOGN-1-s).  The slice is sub-block `h` of a limb with LOGN bits; local
 // indices have LOGM bits, so the twiddle block carries h in its top bits.
+// Twiddles are loaded by fwd_pass_tw before the pass's LDS exchange so that
+// their latency overlaps it; in the first pass they are uniform across the
+// workgroup and come through the scalar cache.
 template <int LOGM, int LOGE, int S0, int R, class A>
-__device__ __forceinline__ void fwd_pass(typename A::V (&x)[1 << LOGE], int t, int h,
-                                         int hbits, const A& ar) {
-  constexpr int E = 1 << LOGE;
+__device__ __forceinline__ void fwd_pass_tw(typename A::TwRaw* tw, int t,
+                                            int h, int hbits, const A& ar) {
   constexpr int T = (1 << LOGM) >> LOGE;
-  constexpr int G = E >> R;
+  constexpr int G = PassTw<LOGE, R>::G;
+  constexpr int PER = PassTw<LOGE, R>::PER;
   constexpr int PLO = LOGM - S0 - R;
 #pragma unroll
+  for (int gg = 0; gg < G; ++gg) {
+    const int gam = gg * T + t;
+    const int hi = ((h << S0) | (gam >> PLO));
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+      const int s = S0 + j + hbits;
+#pragma unroll
+      for (int bh = 0; bh < (1 << j); ++bh) {
+        int idx = (1 << s) + ((hi << j) | bh);
+        if constexpr (S0 == 0 && G == 1) idx = __builtin_amdgcn_readfirstlane(idx);
+        tw[gg * PER + (1 << j) - 1 + bh] = ar.fwd_tw_raw(idx);
+      }
+    }
+  }
+}
+
+template <int LOGM, int LOGE, int S0, int R, class A>
+__device__ __forceinline__ void fwd_pass(typename A::V (&x)[1 << LOGE],
+                                         const typename A::TwRaw* tw,
+                                         int hbits, const A& ar) {
+  constexpr int E = 1 << LOGE;
+  constexpr int G = PassTw<LOGE, R>::G;
+  constexpr int PER = PassTw<LOGE, R>::PER;
+#pragma unroll
   for (int j = 0; j < R; ++j) {
-    const int sl = S0 + j;            // local stage
-    const int s = sl + hbits;         // global stage
+    const int s = S0 + j + hbits;  // global stage
     const int eb = R - 1 - j;
 #pragma unroll
     for (int gg = 0; gg < G; ++gg) {
-      const int gam = gg * T + t;
-      const int hi = ((h << S0) | (gam >> PLO));
 #pragma unroll
       for (int bh = 0; bh < (1 << j); ++bh) {
-        const auto w = ar.fwd_tw((1 << s) + ((hi << j) | bh));
+        const auto w = ar.tw(tw[gg * PER + (1 << j) - 1 + bh]);
 #pragma unroll
         for (int el = 0; el < (1 << eb); ++el) {
           const int e = (bh << (eb + 1)) | el;
@@ -199,8 +246,9 @@ __device__ __forceinline__ void fwd_pass(typename A::V (&x)[1 << LOGE], int t, i
 #pragma unroll
       for (int i = 0; i < E; ++i) ar.fwd_reduce(x[i]);
     }
-    // One stage of twiddles in flight at a time bounds VGPR pressure.
+#if SHG_STAGE_BARRIER
     __builtin_amdgcn_sched_barrier(0);
+#endif
   }
 }
 
@@ -236,43 +284,89 @@ __device__ __forceinline__ void fwd_from_lds(V (&x)[1 << LOGE], const V* lds, in
   }
 }
 
-// Runs forward passes S0.. to LOGM-1 (recursive over passes).
-template <int LOGM, int LOGE, int S0, class A>
+// Runs forward passes S0.. to LOGM-1 (recursive over passes).  With
+// LAST_TO_LDS the final pass is written to LDS (natural layout, read back
+// coalesced); otherwise it stays in VGPRs, where thread t holds, for each of
+// its G groups gamma = gg*T + t, the 2^R consecutive words
+// (gamma << R) .. (gamma << R) + 2^R - 1 (see FwdLast).
+template <int LOGM, int LOGE, int S0, class A, bool LAST_TO_LDS = true>
 struct FwdPasses {
   static constexpr int R = (LOGM - S0) < LOGE ? (LOGM - S0) : LOGE;
   __device__ __forceinline__ static void run(typename A::V (&x)[1 << LOGE], typename A::V* lds,
                                              int t, int h, int hbits, const A& ar) {
+    typename A::TwRaw tw[PassTw<LOGE, R>::N];
+    fwd_pass_tw<LOGM, LOGE, S0, R>(tw, t, h, hbits, ar);
     if constexpr (S0 > 0) {
       fwd_from_lds<LOGM, LOGE, S0, R>(x, lds, t);
       __syncthreads();
     }
-    fwd_pass<LOGM, LOGE, S0, R>(x, t, h, hbits, ar);
-    fwd_to_lds<LOGM, LOGE, S0, R>(x, lds, t);
-    __syncthreads();
-    if constexpr (S0 + R < LOGM) FwdPasses<LOGM, LOGE, S0 + R, A>::run(x, lds, t, h, hbits, ar);
+    fwd_pass<LOGM, LOGE, S0, R>(x, tw, hbits, ar);
+    if constexpr (S0 + R < LOGM || LAST_TO_LDS) {
+      fwd_to_lds<LOGM, LOGE, S0, R>(x, lds, t);
+      __syncthreads();
+    }
+    if constexpr (S0 + R < LOGM)
+      FwdPasses<LOGM, LOGE, S0 + R, A, LAST_TO_LDS>::run(x, lds, t, h, hbits, ar);
   }
 };
 
+// Shape of the last forward pass: R_LAST stages, G_LAST groups per thread.
+template <int LOGM, int LOGE, int S0 = 0>
+struct FwdLast {
+  static constexpr int R = (LOGM - S0) < LOGE ? (LOGM - S0) : LOGE;
+  static constexpr int R_LAST = (S0 + R >= LOGM) ? R : FwdLast<LOGM, LOGE, S0 + R>::R_LAST;
+  static constexpr int G_LAST = (1 << LOGE) >> R_LAST;
+};
+template <int LOGM, int LOGE>
+struct FwdLast<LOGM, LOGE, LOGM> {
+  static constexpr int R = 0;
+  static constexpr int R_LAST = 0;
+  static constexpr int G_LAST = 0;
+};
+
+typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
+
 // Inverse pass: stages I0..I0+R-1 (stage i toggles local bit i).  Local
 // stage i of sub-block h is global stage i with block index carrying h.
+// Stage j of the pass needs 2^(R-1-j) twiddles per group, stored at
+// [gg][2^R - 2^(R-j) + bh].
 template <int LOGM, int LOGE, int I0, int R, class A>
-__device__ __forceinline__ void inv_pass(typename A::V (&x)[1 << LOGE], int t, int h, int logn,
+__device__ __forceinline__ void inv_pass_tw(typename A::TwRaw* tw, int t,
+                                            int h, int logn, const A& ar) {
+  constexpr int T = (1 << LOGM) >> LOGE;
+  constexpr int G = PassTw<LOGE, R>::G;
+  constexpr int PER = PassTw<LOGE, R>::PER;
+  const int n = 1 << logn;
+#pragma unroll
+  for (int gg = 0; gg < G; ++gg) {
+    const int gam = gg * T + t;
+    const int hi = (h << (LOGM - I0 - R)) | (gam >> I0);
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+      const int i = I0 + j;
+#pragma unroll
+      for (int bh = 0; bh < (1 << (R - 1 - j)); ++bh)
+        tw[gg * PER + (1 << R) - (1 << (R - j)) + bh] =
+            ar.inv_tw_raw(n - (n >> i) + ((hi << (R - 1 - j)) | bh));
+    }
+  }
+}
+
+template <int LOGM, int LOGE, int I0, int R, class A>
+__device__ __forceinline__ void inv_pass(typename A::V (&x)[1 << LOGE],
+                                         const typename A::TwRaw* tw,
                                          const A& ar) {
   constexpr int E = 1 << LOGE;
-  constexpr int T = (1 << LOGM) >> LOGE;
-  constexpr int G = E >> R;
-  const int n = 1 << logn;
+  constexpr int G = PassTw<LOGE, R>::G;
+  constexpr int PER = PassTw<LOGE, R>::PER;
 #pragma unroll
   for (int j = 0; j < R; ++j) {
     const int i = I0 + j;
 #pragma unroll
     for (int gg = 0; gg < G; ++gg) {
-      const int gam = gg * T + t;
-      const int hi = (h << (LOGM - I0 - R)) | (gam >> I0);
 #pragma unroll
       for (int bh = 0; bh < (1 << (R - 1 - j)); ++bh) {
-        const int block = (hi << (R - 1 - j)) | bh;
-        const auto w = ar.inv_tw(n - (n >> i) + block);
+        const auto w = ar.tw(tw[gg * PER + (1 << R) - (1 << (R - j)) + bh]);
 #pragma unroll
         for (int el = 0; el < (1 << j); ++el) {
           const int e = (bh << (j + 1)) | el;
@@ -284,7 +378,9 @@ __device__ __forceinline__ void inv_pass(typename A::V (&x)[1 << LOGE], int t, i
 #pragma unroll
       for (int k = 0; k < E; ++k) ar.inv_reduce(x[k]);
     }
+#if SHG_STAGE_BARRIER
     __builtin_amdgcn_sched_barrier(0);
+#endif
   }
 }
 
@@ -319,14 +415,21 @@ __device__ __forceinline__ void inv_from_lds(V (&x)[1 << LOGE], const V* lds, in
 }
 
 // Inverse passes from I0; the last pass is left in registers.
-template <int LOGM, int LOGE, int I0, class A>
+// With FIRST_IN_REGS the caller has already loaded the first pass's words
+// into VGPRs: thread t holds, for each group gamma = gg*T + t, the 2^R
+// consecutive words (gamma << R) + e (the first pass toggles the low bits).
+template <int LOGM, int LOGE, int I0, class A, bool FIRST_IN_REGS = false>
 struct InvPasses {
   static constexpr int R = (LOGM - I0) < LOGE ? (LOGM - I0) : LOGE;
   __device__ __forceinline__ static void run(typename A::V (&x)[1 << LOGE], typename A::V* lds,
                                              int t, int h, int logn, const A& ar) {
-    inv_from_lds<LOGM, LOGE, I0, R>(x, lds, t);
-    __syncthreads();
-    inv_pass<LOGM, LOGE, I0, R>(x, t, h, logn, ar);
+    typename A::TwRaw tw[PassTw<LOGE, R>::N];
+    inv_pass_tw<LOGM, LOGE, I0, R>(tw, t, h, logn, ar);
+    if constexpr (!(FIRST_IN_REGS && I0 == 0)) {
+      inv_from_lds<LOGM, LOGE, I0, R>(x, lds, t);
+      __syncthreads();
+    }
+    inv_pass<LOGM, LOGE, I0, R>(x, tw, ar);
     if constexpr (I0 + R < LOGM) {
       inv_to_lds<LOGM, LOGE, I0, R>(x, lds, t);
       __syncthreads();
