@@ -100,10 +100,10 @@ __global__ void __launch_bounds__((1 << LOGM) >> LOGE)
         V v = ar.digit_in((src[M + e * T + t] >> sh) & mask, P);
         ar.ct(u, v, w);
         x[e] = h ? v : u;
-        if (ar.fwd_reduce_every == 1) ar.fwd_reduce(x[e]);
+        if (ar.fwd_reduce_at(0)) ar.fwd_reduce(x[e]);
       }
     }
-    FwdPasses<LOGM, LOGE, 0, A, false>::run(x, lds, t, h, HB, ar);
+    FwdPasses<LOGM, LOGE, 0, A, false, HB>::run(x, lds, t, h, ar);
     const uint64_t* kb = ka.key_b + (size_t)(ka.key_base + j) * poly_words + limb_off;
     const uint64_t* kav = ka.key_a + (size_t)(ka.key_base + j) * poly_words + limb_off;
 #pragma unroll

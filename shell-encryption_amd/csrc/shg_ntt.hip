@@ -13,10 +13,11 @@
 #include "shg_kernels.h"
 #include "shg_ntt_core.h"
 
-// Per-thread word count (log2) for 2^14-word slices: 32 words with 512
-// threads (three register passes) by default.
+// Per-thread word count (log2) for 2^14-word slices: 16 words with 1024
+// threads (four register passes) by default; 32 words with 512 threads
+// measured 2% slower forward and 7% slower inverse on MI355X.
 #ifndef SHG_LOGE_14
-#define SHG_LOGE_14 5
+#define SHG_LOGE_14 4
 #endif
 #ifndef SHG_FWD_DIRECT_STORE
 #define SHG_FWD_DIRECT_STORE 0
@@ -28,22 +29,24 @@
 namespace shg {
 
 // ---------------------------------------------------------------------------
-// Kernels.  blockIdx.x = row * (2^hbits) + h where row indexes limbs
+// Kernels.  blockIdx.x = row * 2^HB + h where row indexes limbs
 // (count * num_limbs of them) and h the slice within the limb.
 // ---------------------------------------------------------------------------
 
-template <int LOGM, int LOGE, class A>
+// HB = log n - LOGM: the number of leading stages already applied by
+// streaming passes (limbs larger than one LDS slice).
+template <int LOGM, int LOGE, int HB, class A>
 __global__ void __launch_bounds__((1 << LOGM) >> LOGE)
     ntt_fwd_kernel(uint64_t* __restrict__ data, const PrimeDev* __restrict__ primes,
-                   LimbPrimes lp, int num_limbs, int logn) {
+                   LimbPrimes lp, int num_limbs) {
   using V = typename A::V;
   constexpr int E = 1 << LOGE;
   constexpr int T = (1 << LOGM) >> LOGE;
+  constexpr int logn = LOGM + HB;
   extern __shared__ __align__(16) unsigned char smem[];
   V* lds = reinterpret_cast<V*>(smem);
-  const int hbits = logn - LOGM;
-  const int h = blockIdx.x & ((1 << hbits) - 1);
-  const int row = blockIdx.x >> hbits;
+  const int h = blockIdx.x & ((1 << HB) - 1);
+  const int row = blockIdx.x >> HB;
   const PrimeDev& P = primes[lp.id[row % num_limbs]];
   const A ar(P);
   uint64_t* g = data + ((size_t)row << logn) + ((size_t)h << LOGM);
@@ -52,13 +55,8 @@ __global__ void __launch_bounds__((1 << LOGM) >> LOGE)
   // First pass reads straight from HBM: element e of thread t is word e*T+t.
 #pragma unroll
   for (int e = 0; e < E; ++e) x[e] = ar.load(g[e * T + t]);
-  if (hbits > 0) {
-    // Streaming stages already applied growth; bring values back in range.
-#pragma unroll
-    for (int e = 0; e < E; ++e) ar.fwd_reduce(x[e]);
-  }
 #if SHG_FWD_DIRECT_STORE
-  FwdPasses<LOGM, LOGE, 0, A, false>::run(x, lds, t, h, hbits, ar);
+  FwdPasses<LOGM, LOGE, 0, A, false, HB>::run(x, lds, t, h, ar);
   // The last pass left 2^R consecutive words per group in VGPRs: store them
   // with 16-byte writes.
   constexpr int R = FwdLast<LOGM, LOGE>::R_LAST;
@@ -81,24 +79,25 @@ __global__ void __launch_bounds__((1 << LOGM) >> LOGE)
 #else
   // The last pass goes through LDS so that lanes store consecutive words
   // (measured faster than 16-byte thread-contiguous stores).
-  FwdPasses<LOGM, LOGE, 0, A, true>::run(x, lds, t, h, hbits, ar);
+  FwdPasses<LOGM, LOGE, 0, A, true, HB>::run(x, lds, t, h, ar);
 #pragma unroll
   for (int e = 0; e < E; ++e) g[e * T + t] = ar.fwd_out(lds[lpad(e * T + t)]);
 #endif
 }
 
-template <int LOGM, int LOGE, class A>
+// HB = log n - LOGM: the number of trailing stages left to streaming passes.
+template <int LOGM, int LOGE, int HB, class A>
 __global__ void __launch_bounds__((1 << LOGM) >> LOGE)
     ntt_inv_kernel(uint64_t* __restrict__ data, const PrimeDev* __restrict__ primes,
-                   LimbPrimes lp, int num_limbs, int logn, int scale) {
+                   LimbPrimes lp, int num_limbs, int scale) {
   using V = typename A::V;
   constexpr int E = 1 << LOGE;
   constexpr int T = (1 << LOGM) >> LOGE;
+  constexpr int logn = LOGM + HB;
   extern __shared__ __align__(16) unsigned char smem[];
   V* lds = reinterpret_cast<V*>(smem);
-  const int hbits = logn - LOGM;
-  const int h = blockIdx.x & ((1 << hbits) - 1);
-  const int row = blockIdx.x >> hbits;
+  const int h = blockIdx.x & ((1 << HB) - 1);
+  const int row = blockIdx.x >> HB;
   const PrimeDev& P = primes[lp.id[row % num_limbs]];
   const A ar(P);
   uint64_t* g = data + ((size_t)row << logn) + ((size_t)h << LOGM);
@@ -125,7 +124,7 @@ __global__ void __launch_bounds__((1 << LOGM) >> LOGE)
       }
     }
   }
-  InvPasses<LOGM, LOGE, 0, A, true>::run(x, lds, t, h, logn, ar);
+  InvPasses<LOGM, LOGE, 0, A, true, logn>::run(x, lds, t, h, ar);
 #else
   // Coalesced load through LDS (measured faster than thread-contiguous
   // 16-byte reads).
@@ -133,7 +132,7 @@ __global__ void __launch_bounds__((1 << LOGM) >> LOGE)
   for (int e = 0; e < E; ++e) lds[lpad(e * T + t)] = ar.load(g[e * T + t]);
   __syncthreads();
   V x[E];
-  InvPasses<LOGM, LOGE, 0, A, false>::run(x, lds, t, h, logn, ar);
+  InvPasses<LOGM, LOGE, 0, A, false, logn>::run(x, lds, t, h, ar);
 #endif
   // The last pass toggles the top local bits: word (e << I0) | gamma.
   constexpr int I0 = InvLast<LOGM, LOGE, 0, A>::value;
@@ -189,26 +188,40 @@ __global__ void ntt_stream_stage(uint64_t* __restrict__ data, const PrimeDev* __
 // Launchers
 // ---------------------------------------------------------------------------
 
+template <int LOGM, int LOGE, int HB, class A>
+static hipError_t launch_fused_hb(bool inverse, uint64_t* data, const PrimeDev* primes,
+                                  const LimbPrimes& lp, int rows, int num_limbs, int scale,
+                                  hipStream_t stream) {
+  constexpr int T = (1 << LOGM) >> LOGE;
+  const size_t shmem = (size_t)lds_words<LOGM>() * 8;
+  const unsigned grid = (unsigned)rows << HB;
+  if (!inverse) {
+    auto k = ntt_fwd_kernel<LOGM, LOGE, HB, A>;
+    if (shmem > 65536)
+      (void)hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
+    hipLaunchKernelGGL(k, dim3(grid), dim3(T), shmem, stream, data, primes, lp, num_limbs);
+  } else {
+    auto k = ntt_inv_kernel<LOGM, LOGE, HB, A>;
+    if (shmem > 65536)
+      (void)hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
+    hipLaunchKernelGGL(k, dim3(grid), dim3(T), shmem, stream, data, primes, lp, num_limbs, scale);
+  }
+  return hipGetLastError();
+}
+
 template <int LOGM, int LOGE, class A>
 static hipError_t launch_fused(bool inverse, uint64_t* data, const PrimeDev* primes,
                                const LimbPrimes& lp, int rows, int num_limbs, int logn,
                                int scale, hipStream_t stream) {
-  constexpr int T = (1 << LOGM) >> LOGE;
-  const size_t shmem = (size_t)lds_words<LOGM>() * 8;
-  const unsigned grid = (unsigned)rows << (logn - LOGM);
-  if (!inverse) {
-    auto k = ntt_fwd_kernel<LOGM, LOGE, A>;
-    if (shmem > 65536)
-      (void)hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
-    hipLaunchKernelGGL(k, dim3(grid), dim3(T), shmem, stream, data, primes, lp, num_limbs, logn);
-  } else {
-    auto k = ntt_inv_kernel<LOGM, LOGE, A>;
-    if (shmem > 65536)
-      (void)hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
-    hipLaunchKernelGGL(k, dim3(grid), dim3(T), shmem, stream, data, primes, lp, num_limbs, logn,
-                       scale);
+  if (logn == LOGM)
+    return launch_fused_hb<LOGM, LOGE, 0, A>(inverse, data, primes, lp, rows, num_limbs, scale,
+                                             stream);
+  if constexpr (LOGM == kMaxLogM) {
+    if (logn == LOGM + 1)
+      return launch_fused_hb<LOGM, LOGE, 1, A>(inverse, data, primes, lp, rows, num_limbs, scale,
+                                               stream);
   }
-  return hipGetLastError();
+  return hipErrorInvalidValue;
 }
 
 template <class A>

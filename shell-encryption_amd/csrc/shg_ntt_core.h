@@ -31,6 +31,24 @@ namespace shg {
 
 // One padding word per 16 keeps strided u64 LDS accesses spread over banks.
 __device__ __forceinline__ int lpad(int i) { return i + (i >> 4); }
+__host__ __device__ constexpr int cpad(int i) { return i + (i >> 4); }
+
+// Padded LDS address of word (base + (e << PLO)) where base's bits below PLO
+// come from the thread (lo) and e has R bits.  When the padding is linear
+// over the two parts (PLO >= 4, or lo == 0 and base a multiple of 16) the
+// address is one per-thread base plus a compile-time offset per e.
+template <int PLO, int R>
+struct LdsMap {
+  static constexpr bool kLinear = (PLO >= 4) || (PLO == 0 && R >= 4);
+  __device__ __forceinline__ static int base(int hi, int lo) {
+    const int b = (hi << (PLO + R)) + lo;
+    return kLinear ? lpad(b) : b;
+  }
+  __device__ __forceinline__ static int addr(int pbase, int e) {
+    if constexpr (kLinear) return pbase + cpad(e << PLO);
+    return lpad(pbase + (e << PLO));
+  }
+};
 
 template <int LOGM>
 constexpr int lds_words() {
@@ -49,12 +67,14 @@ struct IntArith {
   uint64_t ninv_c, ninv_cb, ninvx_c, ninvx_cb;
   // Lazy growth is < 2q per CT stage, so up to 15 stages stay below 2^64 for
   // q < 2^58; wider moduli (up to the reference's 2^62 limit) are reduced
-  // after every stage.
-  int fwd_reduce_every, inv_reduce_every;
+  // after every stage.  GS outputs stay in [0, 2q).
+  bool reduce_each;
   __device__ __forceinline__ explicit IntArith(const PrimeDev& P)
       : q(P.q), q2(2 * P.q), bn(P.bn), fw(P.fw), iw(P.iw), ninv_c(P.n_inv_c),
-        ninv_cb(P.n_inv_cb), ninvx_c(P.n_inv_x_c), ninvx_cb(P.n_inv_x_cb), fwd_reduce_every(P.q < (1ull << 58) ? 64 : 1),
-        inv_reduce_every(64) {}
+        ninv_cb(P.n_inv_cb), ninvx_c(P.n_inv_x_c), ninvx_cb(P.n_inv_x_cb),
+        reduce_each(P.q >= (1ull << 58)) {}
+  __device__ __forceinline__ bool fwd_reduce_at(int) const { return reduce_each; }
+  __device__ __forceinline__ bool inv_reduce_at(int) const { return false; }
   struct Tw {
     uint64_t c, cb;
   };
@@ -169,10 +189,20 @@ struct F64Arith {
   __device__ __forceinline__ uint64_t acc_out(Acc a) const {
     return f64_to_u64(f64_canon(f64_reduce(a, q, qinv), q));
   }
-  // Bounds (q < 2^50, |x| <= q/2 after a reduction): four CT stages keep
-  // |x| < 5.4 q < 2^52.5; two GS stages keep |x| < 2^52.
-  static constexpr int fwd_reduce_every = 4;
-  static constexpr int inv_reduce_every = 2;
+  // Reduction schedule (s, i: 0-based global stage after which to reduce).
+  // For q < 2^50, f64_mulmod(x, w) with w < q returns |t| <= q/2 +
+  // 1.5 * 2^-52 q |x| <= q/2 + 0.375 |x| (three roundings in
+  // x * fl(w * fl(1/q))).  A CT stage maps a bound B to 1.375 B + q/2:
+  // from canonical inputs (B = q) four stages give B < 7.1 q, and from a
+  // reduction (B = q/2 + 2^-49 q) five stages give B < 7.7 q; both stay below
+  // 8 q < 2^53, so every sum and product term is an exact integer.  A GS
+  // stage at most doubles B: three stages from B = q, four from B = q/2.
+  __device__ static constexpr bool fwd_reduce_at(int s) {
+    return s == 3 || (s > 3 && (s - 3) % 5 == 0);
+  }
+  __device__ static constexpr bool inv_reduce_at(int i) {
+    return i == 2 || (i > 2 && (i - 2) % 4 == 0);
+  }
 };
 
 // ---------------------------------------------------------------------------
@@ -195,9 +225,8 @@ struct PassTw {
 // Twiddles are loaded by fwd_pass_tw before the pass's LDS exchange so that
 // their latency overlaps it; in the first pass they are uniform across the
 // workgroup and come through the scalar cache.
-template <int LOGM, int LOGE, int S0, int R, class A>
-__device__ __forceinline__ void fwd_pass_tw(typename A::TwRaw* tw, int t,
-                                            int h, int hbits, const A& ar) {
+template <int LOGM, int LOGE, int S0, int R, int HB, class A>
+__device__ __forceinline__ void fwd_pass_tw(typename A::TwRaw* tw, int t, int h, const A& ar) {
   constexpr int T = (1 << LOGM) >> LOGE;
   constexpr int G = PassTw<LOGE, R>::G;
   constexpr int PER = PassTw<LOGE, R>::PER;
@@ -205,13 +234,13 @@ __device__ __forceinline__ void fwd_pass_tw(typename A::TwRaw* tw, int t,
 #pragma unroll
   for (int gg = 0; gg < G; ++gg) {
     const int gam = gg * T + t;
-    const int hi = ((h << S0) | (gam >> PLO));
+    const int hi = (h << S0) + (gam >> PLO);
 #pragma unroll
     for (int j = 0; j < R; ++j) {
-      const int s = S0 + j + hbits;
+      const int s = S0 + j + HB;
 #pragma unroll
       for (int bh = 0; bh < (1 << j); ++bh) {
-        int idx = (1 << s) + ((hi << j) | bh);
+        int idx = (1 << s) + (hi << j) + bh;
         if constexpr (S0 == 0 && G == 1) idx = __builtin_amdgcn_readfirstlane(idx);
         tw[gg * PER + (1 << j) - 1 + bh] = ar.fwd_tw_raw(idx);
       }
@@ -219,16 +248,15 @@ __device__ __forceinline__ void fwd_pass_tw(typename A::TwRaw* tw, int t,
   }
 }
 
-template <int LOGM, int LOGE, int S0, int R, class A>
+template <int LOGM, int LOGE, int S0, int R, int HB, class A>
 __device__ __forceinline__ void fwd_pass(typename A::V (&x)[1 << LOGE],
-                                         const typename A::TwRaw* tw,
-                                         int hbits, const A& ar) {
+                                         const typename A::TwRaw* tw, const A& ar) {
   constexpr int E = 1 << LOGE;
   constexpr int G = PassTw<LOGE, R>::G;
   constexpr int PER = PassTw<LOGE, R>::PER;
 #pragma unroll
   for (int j = 0; j < R; ++j) {
-    const int s = S0 + j + hbits;  // global stage
+    const int s = S0 + j + HB;  // global stage
     const int eb = R - 1 - j;
 #pragma unroll
     for (int gg = 0; gg < G; ++gg) {
@@ -242,7 +270,7 @@ __device__ __forceinline__ void fwd_pass(typename A::V (&x)[1 << LOGE],
         }
       }
     }
-    if (((s + 1) % ar.fwd_reduce_every) == 0) {
+    if (ar.fwd_reduce_at(s)) {
 #pragma unroll
       for (int i = 0; i < E; ++i) ar.fwd_reduce(x[i]);
     }
@@ -260,11 +288,9 @@ __device__ __forceinline__ void fwd_to_lds(const V (&x)[1 << LOGE], V* lds, int 
 #pragma unroll
   for (int gg = 0; gg < G; ++gg) {
     const int gam = gg * T + t;
-    const int lo = gam & ((1 << PLO) - 1);
-    const int hi = gam >> PLO;
+    const int pb = LdsMap<PLO, R>::base(gam >> PLO, gam & ((1 << PLO) - 1));
 #pragma unroll
-    for (int e = 0; e < (1 << R); ++e)
-      lds[lpad((hi << (PLO + R)) | (e << PLO) | lo)] = x[gg * (1 << R) + e];
+    for (int e = 0; e < (1 << R); ++e) lds[LdsMap<PLO, R>::addr(pb, e)] = x[gg * (1 << R) + e];
   }
 }
 
@@ -276,11 +302,9 @@ __device__ __forceinline__ void fwd_from_lds(V (&x)[1 << LOGE], const V* lds, in
 #pragma unroll
   for (int gg = 0; gg < G; ++gg) {
     const int gam = gg * T + t;
-    const int lo = gam & ((1 << PLO) - 1);
-    const int hi = gam >> PLO;
+    const int pb = LdsMap<PLO, R>::base(gam >> PLO, gam & ((1 << PLO) - 1));
 #pragma unroll
-    for (int e = 0; e < (1 << R); ++e)
-      x[gg * (1 << R) + e] = lds[lpad((hi << (PLO + R)) | (e << PLO) | lo)];
+    for (int e = 0; e < (1 << R); ++e) x[gg * (1 << R) + e] = lds[LdsMap<PLO, R>::addr(pb, e)];
   }
 }
 
@@ -289,24 +313,24 @@ __device__ __forceinline__ void fwd_from_lds(V (&x)[1 << LOGE], const V* lds, in
 // coalesced); otherwise it stays in VGPRs, where thread t holds, for each of
 // its G groups gamma = gg*T + t, the 2^R consecutive words
 // (gamma << R) .. (gamma << R) + 2^R - 1 (see FwdLast).
-template <int LOGM, int LOGE, int S0, class A, bool LAST_TO_LDS = true>
+template <int LOGM, int LOGE, int S0, class A, bool LAST_TO_LDS = true, int HB = 0>
 struct FwdPasses {
   static constexpr int R = (LOGM - S0) < LOGE ? (LOGM - S0) : LOGE;
   __device__ __forceinline__ static void run(typename A::V (&x)[1 << LOGE], typename A::V* lds,
-                                             int t, int h, int hbits, const A& ar) {
+                                             int t, int h, const A& ar) {
     typename A::TwRaw tw[PassTw<LOGE, R>::N];
-    fwd_pass_tw<LOGM, LOGE, S0, R>(tw, t, h, hbits, ar);
+    fwd_pass_tw<LOGM, LOGE, S0, R, HB>(tw, t, h, ar);
     if constexpr (S0 > 0) {
       fwd_from_lds<LOGM, LOGE, S0, R>(x, lds, t);
       __syncthreads();
     }
-    fwd_pass<LOGM, LOGE, S0, R>(x, tw, hbits, ar);
+    fwd_pass<LOGM, LOGE, S0, R, HB>(x, tw, ar);
     if constexpr (S0 + R < LOGM || LAST_TO_LDS) {
       fwd_to_lds<LOGM, LOGE, S0, R>(x, lds, t);
       __syncthreads();
     }
     if constexpr (S0 + R < LOGM)
-      FwdPasses<LOGM, LOGE, S0 + R, A, LAST_TO_LDS>::run(x, lds, t, h, hbits, ar);
+      FwdPasses<LOGM, LOGE, S0 + R, A, LAST_TO_LDS, HB>::run(x, lds, t, h, ar);
   }
 };
 
@@ -330,24 +354,23 @@ typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
 // stage i of sub-block h is global stage i with block index carrying h.
 // Stage j of the pass needs 2^(R-1-j) twiddles per group, stored at
 // [gg][2^R - 2^(R-j) + bh].
-template <int LOGM, int LOGE, int I0, int R, class A>
-__device__ __forceinline__ void inv_pass_tw(typename A::TwRaw* tw, int t,
-                                            int h, int logn, const A& ar) {
+template <int LOGM, int LOGE, int I0, int R, int LOGN, class A>
+__device__ __forceinline__ void inv_pass_tw(typename A::TwRaw* tw, int t, int h, const A& ar) {
   constexpr int T = (1 << LOGM) >> LOGE;
   constexpr int G = PassTw<LOGE, R>::G;
   constexpr int PER = PassTw<LOGE, R>::PER;
-  const int n = 1 << logn;
+  constexpr int n = 1 << LOGN;
 #pragma unroll
   for (int gg = 0; gg < G; ++gg) {
     const int gam = gg * T + t;
-    const int hi = (h << (LOGM - I0 - R)) | (gam >> I0);
+    const int hi = (h << (LOGM - I0 - R)) + (gam >> I0);
 #pragma unroll
     for (int j = 0; j < R; ++j) {
       const int i = I0 + j;
 #pragma unroll
       for (int bh = 0; bh < (1 << (R - 1 - j)); ++bh)
         tw[gg * PER + (1 << R) - (1 << (R - j)) + bh] =
-            ar.inv_tw_raw(n - (n >> i) + ((hi << (R - 1 - j)) | bh));
+            ar.inv_tw_raw(n - (n >> i) + (hi << (R - 1 - j)) + bh);
     }
   }
 }
@@ -374,7 +397,7 @@ __device__ __forceinline__ void inv_pass(typename A::V (&x)[1 << LOGE],
         }
       }
     }
-    if (((i + 1) % ar.inv_reduce_every) == 0) {
+    if (ar.inv_reduce_at(i)) {
 #pragma unroll
       for (int k = 0; k < E; ++k) ar.inv_reduce(x[k]);
     }
@@ -391,11 +414,9 @@ __device__ __forceinline__ void inv_to_lds(const V (&x)[1 << LOGE], V* lds, int 
 #pragma unroll
   for (int gg = 0; gg < G; ++gg) {
     const int gam = gg * T + t;
-    const int lo = gam & ((1 << I0) - 1);
-    const int hi = gam >> I0;
+    const int pb = LdsMap<I0, R>::base(gam >> I0, gam & ((1 << I0) - 1));
 #pragma unroll
-    for (int e = 0; e < (1 << R); ++e)
-      lds[lpad((hi << (I0 + R)) | (e << I0) | lo)] = x[gg * (1 << R) + e];
+    for (int e = 0; e < (1 << R); ++e) lds[LdsMap<I0, R>::addr(pb, e)] = x[gg * (1 << R) + e];
   }
 }
 
@@ -406,11 +427,9 @@ __device__ __forceinline__ void inv_from_lds(V (&x)[1 << LOGE], const V* lds, in
 #pragma unroll
   for (int gg = 0; gg < G; ++gg) {
     const int gam = gg * T + t;
-    const int lo = gam & ((1 << I0) - 1);
-    const int hi = gam >> I0;
+    const int pb = LdsMap<I0, R>::base(gam >> I0, gam & ((1 << I0) - 1));
 #pragma unroll
-    for (int e = 0; e < (1 << R); ++e)
-      x[gg * (1 << R) + e] = lds[lpad((hi << (I0 + R)) | (e << I0) | lo)];
+    for (int e = 0; e < (1 << R); ++e) x[gg * (1 << R) + e] = lds[LdsMap<I0, R>::addr(pb, e)];
   }
 }
 
@@ -418,13 +437,13 @@ __device__ __forceinline__ void inv_from_lds(V (&x)[1 << LOGE], const V* lds, in
 // With FIRST_IN_REGS the caller has already loaded the first pass's words
 // into VGPRs: thread t holds, for each group gamma = gg*T + t, the 2^R
 // consecutive words (gamma << R) + e (the first pass toggles the low bits).
-template <int LOGM, int LOGE, int I0, class A, bool FIRST_IN_REGS = false>
+template <int LOGM, int LOGE, int I0, class A, bool FIRST_IN_REGS = false, int LOGN = LOGM>
 struct InvPasses {
   static constexpr int R = (LOGM - I0) < LOGE ? (LOGM - I0) : LOGE;
   __device__ __forceinline__ static void run(typename A::V (&x)[1 << LOGE], typename A::V* lds,
-                                             int t, int h, int logn, const A& ar) {
+                                             int t, int h, const A& ar) {
     typename A::TwRaw tw[PassTw<LOGE, R>::N];
-    inv_pass_tw<LOGM, LOGE, I0, R>(tw, t, h, logn, ar);
+    inv_pass_tw<LOGM, LOGE, I0, R, LOGN>(tw, t, h, ar);
     if constexpr (!(FIRST_IN_REGS && I0 == 0)) {
       inv_from_lds<LOGM, LOGE, I0, R>(x, lds, t);
       __syncthreads();
@@ -433,7 +452,7 @@ struct InvPasses {
     if constexpr (I0 + R < LOGM) {
       inv_to_lds<LOGM, LOGE, I0, R>(x, lds, t);
       __syncthreads();
-      InvPasses<LOGM, LOGE, I0 + R, A>::run(x, lds, t, h, logn, ar);
+      InvPasses<LOGM, LOGE, I0 + R, A, false, LOGN>::run(x, lds, t, h, ar);
     }
   }
 };

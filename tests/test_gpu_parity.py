@@ -69,6 +69,38 @@ def test_ntt_round_trip_bit_exact(ctx, variant, log_n, L, count):
     assert np.array_equal(back, xr)
 
 
+@pytest.mark.parametrize("variant", VARIANTS)
+@pytest.mark.parametrize("log_n", [11, 14, 15])
+def test_ntt_extreme_inputs(ctx, variant, log_n):
+    """Constant and alternating extreme residues (moduli just below 2^50)
+    push the lazy-integer and FP64 growth bounds hardest; forward and
+    inverse transforms of each pattern must match the oracle."""
+    qs, rns = _rns(log_n, 2)
+    pids = ctx.primes(qs, log_n)
+    n = rns.n
+    pats = []
+    for pat in range(4):
+        x = np.zeros((2, n), dtype=np.uint64)
+        for i, q in enumerate(qs):
+            if pat == 0:
+                x[i] = q - 1
+            elif pat == 1:
+                x[i] = q
+            elif pat == 2:
+                x[i, ::2] = q - 1
+            else:
+                x[i, ::2] = 1
+                x[i, 1::2] = q - 1
+        pats.append(x)
+    x = np.stack(pats)
+    d = ctx.array(x)
+    ctx.ntt_fwd(d, len(pats), pids, variant)
+    assert np.array_equal(d.download(), np.stack([rns.ntt_fwd(v) for v in x]))
+    d = ctx.array(x)
+    ctx.ntt_inv(d, len(pats), pids, variant)
+    assert np.array_equal(d.download(), np.stack([rns.ntt_inv(v) for v in x]))
+
+
 def test_ntt_wide_modulus_int_only(ctx):
     """60-bit primes take the integer path (FP64 needs q < 2^50)."""
     qs, rns = _rns(12, 2, log_bound=60)
