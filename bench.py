@@ -24,6 +24,10 @@ sys.path.insert(0, os.path.join(ROOT, "shell-encryption_amd", "python"))
 sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the key-switch kernel, measured by rocprofv3
+# FETCH_SIZE + WRITE_SIZE and scaled by the same counters' reading for a
+# streaming kernel of known size (bench.py --copy-probe).
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "keyswitch_relin_n16384_L8_b512_pmc.md")
 LOG_N = 14
 L = 8
 LOG_B = 10
@@ -66,6 +70,32 @@ def cpu_baseline(qs, comps, kas, kbs, log_bs, budget_s=12.0):
                       "oracle (single-threaded C restatement), %.1f s" % (done, el)}
 
 
+def measured_traffic():
+    """traffic_bytes_per_launch from the committed counter summary, or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
+def copy_probe():
+    """A streaming kernel of known size for scaling HBM byte counters:
+    bitwise_not over 2^25 int64 words reads and writes 256 MiB each."""
+    import torch
+    a = torch.arange(1 << 25, dtype=torch.int64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        torch.bitwise_not(a, out=b)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe": "bitwise_not int64", "read_bytes": a.numel() * 8,
+                      "write_bytes": b.numel() * 8, "launches": 5}))
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -73,7 +103,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="skip the secondary measurements")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run the known-size streaming kernel used to scale byte counters")
     args = ap.parse_args()
+    if args.copy_probe:
+        return copy_probe()
     if args.gpus != 1:
         print("bench.py measures one GPU only: every box here has one MI355X, and the "
               "multi-GPU path (batch sharding, section on sharding in DESIGN.md) is not "
@@ -141,7 +175,7 @@ def main():
     ks_bytes = BATCH * 5 * poly_words * 8 + 2 * k * poly_words * 8
     ks_gbs = ks_bytes / (ks_avg_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(ks_gbs, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(ks_gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(ks_gbs / HBM_PEAK_GBS, 4), "traffic": measured_traffic(),
                 "kernel": "keyswitch_kernel<13,4,1,F64Arith>",
                 "avg_launch_ms": round(ks_avg_ms, 4),
                 "algorithmic_bytes_per_launch": ks_bytes}

@@ -1,5 +1,8 @@
 """Bit-exact parity of the HIP path (through the C-ABI) against the CPU
 oracle on identical seeded inputs.  Every output word is compared."""
+import json
+import os
+
 import numpy as np
 import pytest
 
@@ -315,3 +318,26 @@ def test_errors_leave_outputs_untouched(ctx):
     with pytest.raises(sg.ShgError):
         ctx.ntt_fwd(d, 1, [pids[0], 999])
     assert np.array_equal(d.download(), x)
+
+
+_GOLDEN = json.load(open(os.path.join(os.path.dirname(__file__), "golden",
+                                      "reference_moduli.json")))
+
+
+@pytest.mark.parametrize("entry", _GOLDEN["moduli"], ids=lambda e: "%d-%d" % (e["q"], e["log_n"]))
+def test_ntt_reference_moduli(ctx, entry):
+    """Every (q, log n) pair of the reference's own tests, through both
+    transform variants that accept it (FP64 needs q < 2^50)."""
+    q, log_n = entry["q"], entry["log_n"]
+    rns = oracle.Rns([q], log_n)
+    pids = ctx.primes([q], log_n)
+    x = rns.uniform(q % 997, 3)
+    exp = np.stack([rns.ntt_fwd(v) for v in x])
+    for variant in VARIANTS:
+        if variant == sg.NTT_F64 and q >= 2**50:
+            continue
+        d = ctx.array(x)
+        ctx.ntt_fwd(d, 3, pids, variant)
+        assert np.array_equal(d.download(), exp)
+        ctx.ntt_inv(d, 3, pids, variant)
+        assert np.array_equal(d.download(), x)

@@ -408,3 +408,43 @@ def test_relin_key_index_quirk_degree3():
             e0 = rns.pointwise("fma", e0, d, kbs[idx])
             e1 = rns.pointwise("fma", e1, d, kas[idx])
     assert np.array_equal(c0, e0) and np.array_equal(c1, e1)
+
+
+# ---------------------------------------------------------------------------
+# The reference's own test moduli and Montgomery constant (tests/golden/
+# reference_moduli.json, copied from shell_encryption/constants.h and
+# shell_encryption/rns/testing/parameters.h).
+# ---------------------------------------------------------------------------
+
+import json
+import os
+
+GOLDEN = json.load(open(os.path.join(os.path.dirname(__file__), "golden",
+                                     "reference_moduli.json")))
+
+
+def test_reference_montgomery_kat():
+    for e in GOLDEN["montgomery_kat"]:
+        assert oracle.Mont(e["q"]).s.inv_modulus == e["inv_modulus"], e["source"]
+
+
+@pytest.mark.parametrize("entry", GOLDEN["moduli"], ids=lambda e: "%d-%d" % (e["q"], e["log_n"]))
+def test_reference_moduli_ntt(entry):
+    """Tables follow the psi rule and sampled outputs are the evaluations at
+    psi^(2*brv(i)+1), for every (q, log n) pair the reference tests use."""
+    q, log_n = entry["q"], entry["log_n"]
+    p = oracle.Prime(q, log_n)
+    n = 1 << log_n
+    t, psi = psi_by_rule(q, log_n)
+    assert p.tables()["psi"] * pow(R, -1, q) % q == psi
+    x = np.random.default_rng(q % 1000).integers(0, q, size=n, dtype=np.uint64)
+    y = p.ntt_fwd(x)
+    for i in (0, 1, n // 2 - 1, n // 2, n - 1):
+        e = 2 * brv(i, log_n) + 1
+        w = pow(psi, e, q)
+        acc, wp = 0, 1
+        for j in range(n):
+            acc += int(x[j]) * wp
+            wp = wp * w % q
+        assert int(y[i]) == acc % q
+    assert np.array_equal(p.ntt_inv(y), x)
