@@ -175,7 +175,8 @@ bool aligned16(const void* p) { return p != nullptr && ((uintptr_t)p % 16) == 0;
 int build_gadget(shg_context* ctx, const int* primes, int num_limbs, const int* log_bs,
                  shg::GadgetDesc* gd) {
   if (!log_bs) return fail(SHG_E_INVALID_ARGUMENT, "`log_bs` must contain %d elements.", num_limbs);
-  int k = 0;
+  int k = 0, max_b = 0;
+  uint64_t min_q = ~0ull;
   for (int i = 0; i < num_limbs; ++i) {
     const int b = log_bs[i];
     const unsigned lq = ctx->primes[primes[i]]->mp->log_modulus;
@@ -188,8 +189,15 @@ int build_gadget(shg_context* ctx, const int* primes, int num_limbs, const int* 
       gd->log_b[k] = (unsigned char)b;
       ++k;
     }
+    gd->limb_digits[i] = (unsigned char)dims;
+    gd->limb_log_b[i] = (unsigned char)b;
+    max_b = b > max_b ? b : max_b;
+    const uint64_t q = ctx->primes[primes[i]]->q;
+    min_q = q < min_q ? q : min_q;
   }
   gd->k = k;
+  gd->max_log_b = max_b;
+  gd->small_digits = max_b + 6 < 64 && (1ull << (max_b + 6)) <= min_q;
   return SHG_OK;
 }
 

@@ -45,6 +45,12 @@ struct PrimeDev {
   const double* iwd;       // [n] inverse twiddle values as doubles
 };
 
+// Pointer into global memory.  Table pointers read from a PrimeDev would
+// otherwise be generic (flat) pointers, whose loads also count against the
+// LDS wait counter and so serialise with the LDS exchanges.
+template <class T>
+using gptr = const T __attribute__((address_space(1)))*;
+
 struct LimbPrimes {
   int id[SHG_MAX_LIMBS];
 };
@@ -114,18 +120,14 @@ __device__ __forceinline__ uint64_t mod_sub(uint64_t a, uint64_t b, uint64_t q) 
 // |t| <= q * (0.5 + |x| * 2^-52).
 // ---------------------------------------------------------------------------
 
-__device__ __forceinline__ double u64_to_f64(uint64_t x) {
-  const double hi = (double)(uint32_t)(x >> 32);
-  const double lo = (double)(uint32_t)x;
-  return fma(hi, 4294967296.0, lo);
+// x < 2^52 -> double, exactly: 2^52 + x has the bits 0x433 << 52 | x.
+__device__ __forceinline__ double u52_to_f64(uint64_t x) {
+  return __longlong_as_double((long long)(x | 0x4330000000000000ull)) - 4503599627370496.0;
 }
 
-// x in [0, 2^53) integral -> uint64.
-__device__ __forceinline__ uint64_t f64_to_u64(double x) {
-  const double hd = trunc(x * (1.0 / 4294967296.0));
-  const uint32_t hi = (uint32_t)hd;
-  const uint32_t lo = (uint32_t)fma(hd, -4294967296.0, x);
-  return ((uint64_t)hi << 32) | lo;
+// x integral in [0, 2^52) -> uint64: the mantissa of 2^52 + x.
+__device__ __forceinline__ uint64_t f64_to_u52(double x) {
+  return (uint64_t)__double_as_longlong(x + 4503599627370496.0) & 0x000FFFFFFFFFFFFFull;
 }
 
 __device__ __forceinline__ double f64_mulmod(double x, double w, double wq, double q) {

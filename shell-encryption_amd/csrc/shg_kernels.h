@@ -69,6 +69,14 @@ struct GadgetDesc {
   unsigned char src_limb[256];
   unsigned char shift[256];
   unsigned char log_b[256];
+  // Per source limb: digit count and base log (digits of limb i are
+  // consecutive in j).
+  unsigned char limb_digits[SHG_MAX_LIMBS];
+  unsigned char limb_log_b[SHG_MAX_LIMBS];
+  int max_log_b;
+  // 2^max_log_b * 64 <= every modulus: digits enter the transform far below
+  // q, which lets the FP64 key switch defer its first reduction.
+  int small_digits;
 };
 hipError_t launch_gadget_decompose(const uint64_t* a, uint64_t* digits, const GadgetDesc& gd,
                                    const PrimeDev* primes, const LimbPrimes& lp, int count,

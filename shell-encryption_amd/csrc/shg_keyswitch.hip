@@ -26,13 +26,43 @@
 #ifndef SHG_KS_LOGE
 #define SHG_KS_LOGE 4
 #endif
+#ifndef SHG_KS_XCD_REMAP
+#define SHG_KS_XCD_REMAP 1
+#endif
 
 namespace shg {
+
+// FP64 schedule for key switching when every digit is below q/64 (bases up
+// to 2^43 for 50-bit moduli).  With the per-stage bound B -> 1.375 B + q/2
+// (see F64Arith), six stages from digits below q/64 reach 7.8 q, so the
+// first reduction waits until after stage 5 and the second follows five
+// stages later (7.7 q).  When at most three stages follow the last reduction
+// the transform output stays below 3.5 q, each key product below
+// q/2 + 0.375 * 3.5 q < 1.85 q, and three of them added to a reduced or
+// canonical accumulator (<= q) stay below 8 q: the output is then used
+// unreduced and the accumulator is reduced every three digits.
+struct F64KsArith : F64Arith {
+  using F64Arith::F64Arith;
+  __device__ static constexpr bool fwd_reduce_at(int s) { return s == 5 || s == 10; }
+  __host__ __device__ static constexpr int last_reduce(int logn) {
+    return logn - 1 >= 10 ? 10 : (logn - 1 >= 5 ? 5 : -1);
+  }
+  __host__ __device__ static constexpr bool ks_prep(int logn) {
+    return logn - 1 - last_reduce(logn) > 3;
+  }
+  __host__ __device__ static constexpr int ks_acc_every(int logn) { return ks_prep(logn) ? 8 : 3; }
+};
 
 // LOGN = LOGM + HB.  With HB = 1 (n = 2^14) the limb is split between two
 // workgroups: each loads both halves of the digit, applies the first
 // Cooley-Tukey stage for its own half only, and runs the remaining stages on
 // a 2^13-word slice (68 KiB of LDS, so two workgroups share a CU).
+//
+// Workgroup order: logical index (l * count + ct) * 2^HB + h, limb-major.
+// Hardware hands workgroup b to XCD b mod 8; the logical index is remapped
+// so that each XCD runs one contiguous range (one output limb at the bench
+// sizes): its CUs then share that limb's key rows and both halves of a
+// ciphertext's source in their own L2.
 template <int LOGM, int LOGE, int HB, class A>
 __global__ void __launch_bounds__((1 << LOGM) >> LOGE)
     keyswitch_kernel(KeySwitchArgs ka, GadgetDesc gd, const PrimeDev* __restrict__ primes,
@@ -43,10 +73,16 @@ __global__ void __launch_bounds__((1 << LOGM) >> LOGE)
   constexpr int T = (1 << LOGM) >> LOGE;
   constexpr int M = 1 << LOGM;
   constexpr int LOGN = LOGM + HB;
+  constexpr bool kPrep = A::ks_prep(LOGN);
+  constexpr int kAccEvery = A::ks_acc_every(LOGN);
   extern __shared__ __align__(16) unsigned char smem[];
   V* lds = reinterpret_cast<V*>(smem);
-  const int h = HB ? (int)(blockIdx.x & ((1u << HB) - 1)) : 0;
-  const int lc = (int)(blockIdx.x >> HB);  // l * count + ct: limb-major order
+  unsigned b = blockIdx.x;
+#if SHG_KS_XCD_REMAP
+  if ((gridDim.x & 7) == 0) b = (b & 7) * (gridDim.x >> 3) + (b >> 3);
+#endif
+  const int h = HB ? (int)(b & ((1u << HB) - 1)) : 0;
+  const int lc = (int)(b >> HB);  // l * count + ct: limb-major order
   const int l = lc / count;
   const int ct = lc - l * count;
   const int t = threadIdx.x;
@@ -74,77 +110,100 @@ __global__ void __launch_bounds__((1 << LOGM) >> LOGE)
 #pragma unroll
       for (int e = 0; e < (1 << R); e += 2) {
         u64x2 a = i0 ? *reinterpret_cast<const u64x2*>(i0 + base + e) : u64x2{0, 0};
-        u64x2 b = i1 ? *reinterpret_cast<const u64x2*>(i1 + base + e) : u64x2{0, 0};
+        u64x2 b2 = i1 ? *reinterpret_cast<const u64x2*>(i1 + base + e) : u64x2{0, 0};
         acc0[gg * (1 << R) + e] = ar.acc_in(a.x);
         acc0[gg * (1 << R) + e + 1] = ar.acc_in(a.y);
-        acc1[gg * (1 << R) + e] = ar.acc_in(b.x);
-        acc1[gg * (1 << R) + e + 1] = ar.acc_in(b.y);
+        acc1[gg * (1 << R) + e] = ar.acc_in(b2.x);
+        acc1[gg * (1 << R) + e + 1] = ar.acc_in(b2.y);
       }
     }
   }
 
   const uint64_t* src_ct = ka.src + ct * ka.src_stride;
-  for (int j = 0; j < gd.k; ++j) {
-    const uint64_t* src = src_ct + ((size_t)gd.src_limb[j] << LOGN);
-    const int sh = gd.shift[j];
-    const uint64_t mask = (1ull << gd.log_b[j]) - 1;
-    V x[E];
-    if constexpr (HB == 0) {
+  const bool digits32 = gd.max_log_b <= 32;
+  int j = 0;
+  // Digits of source limb i are consecutive: j runs over (i, dd).
+  for (int i = 0; i < num_limbs; ++i) {
+    const uint64_t* src = src_ct + ((size_t)i << LOGN);
+    const int lb = gd.limb_log_b[i];
+    const int nd = gd.limb_digits[i];
+    const uint64_t mask = (1ull << lb) - 1;
+    for (int dd = 0; dd < nd; ++dd, ++j) {
+      const int sh = dd * lb;
+      V x[E];
+      // Digit extraction (one uniform branch per digit on the base width),
+      // then for HB = 1 the first Cooley-Tukey stage on this half.
+      auto load_digits = [&](auto conv) {
+        if constexpr (HB == 0) {
 #pragma unroll
-      for (int e = 0; e < E; ++e) x[e] = ar.digit_in((src[e * T + t] >> sh) & mask, P);
-    } else {
-      const auto w = ar.fwd_tw(1);
+          for (int e = 0; e < E; ++e) x[e] = conv(src[e * T + t] >> sh);
+        } else {
+          const auto w = ar.fwd_tw(1);
 #pragma unroll
-      for (int e = 0; e < E; ++e) {
-        V u = ar.digit_in((src[e * T + t] >> sh) & mask, P);
-        V v = ar.digit_in((src[M + e * T + t] >> sh) & mask, P);
-        ar.ct(u, v, w);
-        x[e] = h ? v : u;
-        if (ar.fwd_reduce_at(0)) ar.fwd_reduce(x[e]);
+          for (int e = 0; e < E; ++e) {
+            V u = conv(src[e * T + t] >> sh);
+            V v = conv(src[M + e * T + t] >> sh);
+            ar.ct(u, v, w);
+            x[e] = h ? v : u;
+            if (ar.fwd_reduce_at(0)) ar.fwd_reduce(x[e]);
+          }
+        }
+      };
+      if (digits32) {
+        const uint32_t m32 = (uint32_t)mask;
+        load_digits([&](uint64_t s64) { return ar.digit_in32((uint32_t)s64 & m32, P); });
+      } else {
+        load_digits([&](uint64_t s64) { return ar.digit_in(s64 & mask, P); });
       }
-    }
-    FwdPasses<LOGM, LOGE, 0, A, false, HB>::run(x, lds, t, h, ar);
-    const uint64_t* kb = ka.key_b + (size_t)(ka.key_base + j) * poly_words + limb_off;
-    const uint64_t* kav = ka.key_a + (size_t)(ka.key_base + j) * poly_words + limb_off;
+      FwdPasses<LOGM, LOGE, 0, A, false, HB>::run(x, lds, t, h, ar);
+      const uint64_t* kb = ka.key_b + (size_t)(ka.key_base + j) * poly_words + limb_off;
+      const uint64_t* kav = ka.key_a + (size_t)(ka.key_base + j) * poly_words + limb_off;
 #pragma unroll
-    for (int gg = 0; gg < G; ++gg) {
-      const size_t base = (size_t)(gg * T + t) << R;
+      for (int gg = 0; gg < G; ++gg) {
+        const size_t base = (size_t)(gg * T + t) << R;
 #pragma unroll
-      for (int e = 0; e < (1 << R); e += 2) {
-        const u64x2 b = *reinterpret_cast<const u64x2*>(kb + base + e);
-        const u64x2 a = *reinterpret_cast<const u64x2*>(kav + base + e);
-        const int r = gg * (1 << R) + e;
-        const auto d0 = ar.prep(x[r]);
-        const auto d1 = ar.prep(x[r + 1]);
-        ar.mac(acc0[r], d0, b.x, P);
-        ar.mac(acc0[r + 1], d1, b.y, P);
-        ar.mac(acc1[r], d0, a.x, P);
-        ar.mac(acc1[r + 1], d1, a.y, P);
+        for (int e = 0; e < (1 << R); e += 2) {
+          const u64x2 kbv = *reinterpret_cast<const u64x2*>(kb + base + e);
+          const u64x2 kav2 = *reinterpret_cast<const u64x2*>(kav + base + e);
+          const int r = gg * (1 << R) + e;
+          typename A::Prep d0, d1;
+          if constexpr (kPrep) {
+            d0 = ar.prep(x[r]);
+            d1 = ar.prep(x[r + 1]);
+          } else {
+            d0 = x[r];
+            d1 = x[r + 1];
+          }
+          ar.mac(acc0[r], d0, kbv.x, P);
+          ar.mac(acc0[r + 1], d1, kbv.y, P);
+          ar.mac(acc1[r], d0, kav2.x, P);
+          ar.mac(acc1[r + 1], d1, kav2.y, P);
+        }
       }
-    }
-    if (((j + 1) % A::kAccReduceEvery) == 0) {
+      if (((j + 1) % kAccEvery) == 0) {
 #pragma unroll
-      for (int e = 0; e < E; ++e) {
-        ar.acc_reduce(acc0[e]);
-        ar.acc_reduce(acc1[e]);
+        for (int e = 0; e < E; ++e) {
+          ar.acc_reduce(acc0[e]);
+          ar.acc_reduce(acc1[e]);
+        }
       }
+      // The next digit's first pass writes LDS only after its registers are
+      // computed; all reads of this digit's last exchange are done by then
+      // (they precede the last pass), so no barrier is needed here.
     }
-    // The next digit's first pass writes LDS only after its registers are
-    // computed; all reads of this digit's last exchange are done by then
-    // (they precede the last pass), so no barrier is needed here.
   }
 #pragma unroll
   for (int gg = 0; gg < G; ++gg) {
     const size_t base = (size_t)(gg * T + t) << R;
 #pragma unroll
     for (int e = 0; e < (1 << R); e += 2) {
-      u64x2 a, b;
+      u64x2 a, b2;
       a.x = ar.acc_out(acc0[gg * (1 << R) + e]);
       a.y = ar.acc_out(acc0[gg * (1 << R) + e + 1]);
-      b.x = ar.acc_out(acc1[gg * (1 << R) + e]);
-      b.y = ar.acc_out(acc1[gg * (1 << R) + e + 1]);
+      b2.x = ar.acc_out(acc1[gg * (1 << R) + e]);
+      b2.y = ar.acc_out(acc1[gg * (1 << R) + e + 1]);
       *reinterpret_cast<u64x2*>(o0 + base + e) = a;
-      *reinterpret_cast<u64x2*>(o1 + base + e) = b;
+      *reinterpret_cast<u64x2*>(o1 + base + e) = b2;
     }
   }
 }
@@ -185,8 +244,11 @@ static hipError_t dispatch_ks(int logn, const KeySwitchArgs& ka, const GadgetDes
 hipError_t launch_keyswitch(const KeySwitchArgs& ka, const GadgetDesc& gd, int variant,
                             const PrimeDev* primes, const LimbPrimes& lp, int count, int num_limbs,
                             int logn, hipStream_t s) {
-  if (variant == SHG_NTT_F64)
+  if (variant == SHG_NTT_F64) {
+    if (gd.small_digits)
+      return dispatch_ks<F64KsArith>(logn, ka, gd, primes, lp, count, num_limbs, s);
     return dispatch_ks<F64Arith>(logn, ka, gd, primes, lp, count, num_limbs, s);
+  }
   return dispatch_ks<IntArith>(logn, ka, gd, primes, lp, count, num_limbs, s);
 }
 

@@ -62,15 +62,16 @@ constexpr int lds_words() {
 struct IntArith {
   using V = uint64_t;
   uint64_t q, q2, bn;
-  const uint64_t* __restrict__ fw;
-  const uint64_t* __restrict__ iw;
+  gptr<uint64_t> fw;
+  gptr<uint64_t> iw;
   uint64_t ninv_c, ninv_cb, ninvx_c, ninvx_cb;
   // Lazy growth is < 2q per CT stage, so up to 15 stages stay below 2^64 for
   // q < 2^58; wider moduli (up to the reference's 2^62 limit) are reduced
   // after every stage.  GS outputs stay in [0, 2q).
   bool reduce_each;
   __device__ __forceinline__ explicit IntArith(const PrimeDev& P)
-      : q(P.q), q2(2 * P.q), bn(P.bn), fw(P.fw), iw(P.iw), ninv_c(P.n_inv_c),
+      : q(P.q), q2(2 * P.q), bn(P.bn), fw((gptr<uint64_t>)P.fw), iw((gptr<uint64_t>)P.iw),
+        ninv_c(P.n_inv_c),
         ninv_cb(P.n_inv_cb), ninvx_c(P.n_inv_x_c), ninvx_cb(P.n_inv_x_cb),
         reduce_each(P.q >= (1ull << 58)) {}
   __device__ __forceinline__ bool fwd_reduce_at(int) const { return reduce_each; }
@@ -113,8 +114,14 @@ struct IntArith {
   // transform as Montgomery representations and the key product is a REDC.
   using Acc = uint64_t;
   using Prep = uint64_t;
-  static constexpr int kAccReduceEvery = 1 << 30;
+  // Transform outputs are reduced before the MAC; REDC accumulators stay
+  // canonical.
+  __host__ __device__ static constexpr bool ks_prep(int) { return true; }
+  __host__ __device__ static constexpr int ks_acc_every(int) { return 1 << 30; }
   __device__ __forceinline__ V digit_in(uint64_t d, const PrimeDev& P) const {
+    return mont_import(d, P);
+  }
+  __device__ __forceinline__ V digit_in32(uint32_t d, const PrimeDev& P) const {
     return mont_import(d, P);
   }
   __device__ __forceinline__ Acc acc_in(uint64_t rep) const { return rep; }
@@ -129,11 +136,11 @@ struct IntArith {
 struct F64Arith {
   using V = double;
   double q, qinv, ninv, ninvq, ninvx, ninvxq;
-  const double* __restrict__ fw;
-  const double* __restrict__ iw;
+  gptr<double> fw;
+  gptr<double> iw;
   __device__ __forceinline__ explicit F64Arith(const PrimeDev& P)
       : q(P.qd), qinv(P.qinv), ninv(P.n_inv_d), ninvq(P.n_inv_dq), ninvx(P.n_inv_x_d),
-        ninvxq(P.n_inv_x_dq), fw(P.fwd), iw(P.iwd) {}
+        ninvxq(P.n_inv_x_dq), fw((gptr<double>)P.fwd), iw((gptr<double>)P.iwd) {}
   struct Tw {
     double w, wq;
   };
@@ -145,7 +152,8 @@ struct F64Arith {
   __device__ __forceinline__ Tw tw(TwRaw r) const { return Tw{r, r * qinv}; }
   __device__ __forceinline__ Tw fwd_tw(int i) const { return tw(fw[i]); }
   __device__ __forceinline__ Tw inv_tw(int i) const { return tw(iw[i]); }
-  __device__ __forceinline__ V load(uint64_t x) const { return u64_to_f64(x); }
+  // Inputs are canonical (<= q < 2^50).
+  __device__ __forceinline__ V load(uint64_t x) const { return u52_to_f64(x); }
   __device__ __forceinline__ void ct(V& u, V& v, const Tw& w) const {
     const double t = f64_mulmod(v, w.w, w.wq, q);
     v = u - t;
@@ -160,34 +168,38 @@ struct F64Arith {
   __device__ __forceinline__ void fwd_reduce(V& x) const { x = f64_reduce(x, q, qinv); }
   __device__ __forceinline__ void inv_reduce(V& x) const { x = f64_reduce(x, q, qinv); }
   __device__ __forceinline__ uint64_t fwd_out(V x) const {
-    return f64_to_u64(f64_canon(f64_reduce(x, q, qinv), q));
+    return f64_to_u52(f64_canon(f64_reduce(x, q, qinv), q));
   }
   __device__ __forceinline__ uint64_t inv_out(V x, int mode) const {
     if (mode == 0) return fwd_out(x);
     const double y = mode == 1 ? f64_mulmod(x, ninv, ninvq, q) : f64_mulmod(x, ninvx, ninvxq, q);
-    return f64_to_u64(f64_canon(f64_reduce(y, q, qinv), q));
+    return f64_to_u52(f64_canon(f64_reduce(y, q, qinv), q));
   }
 
   // Key-switch accumulation.  The digit value d (not its Montgomery form)
   // enters the transform, and the key's Montgomery representation k is used
   // as a plain residue: NTT(d) * k = NTT(d * 2^64) * k * 2^-64 mod q, which
-  // is the reference's FusedMulAddInPlace of the imported digit.  Terms are
-  // bounded by 0.7 q, so the accumulator is reduced every eight digits.
+  // is the reference's FusedMulAddInPlace of the imported digit.  Reduced
+  // transform outputs (|d| <= q/2) give terms below 0.7 q, so the
+  // accumulator is reduced every eight digits.
   using Acc = double;
   using Prep = double;
-  static constexpr int kAccReduceEvery = 8;
+  __host__ __device__ static constexpr bool ks_prep(int) { return true; }
+  __host__ __device__ static constexpr int ks_acc_every(int) { return 8; }
+  // Digits are at most the exported residue, so below 2^50.
   __device__ __forceinline__ V digit_in(uint64_t d, const PrimeDev&) const {
-    return u64_to_f64(d);
+    return u52_to_f64(d);
   }
-  __device__ __forceinline__ Acc acc_in(uint64_t rep) const { return u64_to_f64(rep); }
+  __device__ __forceinline__ V digit_in32(uint32_t d, const PrimeDev&) const { return (double)d; }
+  __device__ __forceinline__ Acc acc_in(uint64_t rep) const { return u52_to_f64(rep); }
   __device__ __forceinline__ Prep prep(V y) const { return f64_reduce(y, q, qinv); }
   __device__ __forceinline__ void mac(Acc& a, Prep d, uint64_t key, const PrimeDev&) const {
-    const double w = u64_to_f64(key);
+    const double w = u52_to_f64(key);
     a += f64_mulmod(d, w, w * qinv, q);
   }
   __device__ __forceinline__ void acc_reduce(Acc& a) const { a = f64_reduce(a, q, qinv); }
   __device__ __forceinline__ uint64_t acc_out(Acc a) const {
-    return f64_to_u64(f64_canon(f64_reduce(a, q, qinv), q));
+    return f64_to_u52(f64_canon(f64_reduce(a, q, qinv), q));
   }
   // Reduction schedule (s, i: 0-based global stage after which to reduce).
   // For q < 2^50, f64_mulmod(x, w) with w < q returns |t| <= q/2 +
