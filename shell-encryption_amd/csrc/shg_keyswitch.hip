@@ -29,6 +29,10 @@
 #ifndef SHG_KS_XCD_REMAP
 #define SHG_KS_XCD_REMAP 1
 #endif
+#ifndef SHG_KS_DB
+#define SHG_KS_DB 1
+#endif
+
 
 namespace shg {
 
@@ -56,7 +60,8 @@ struct F64KsArith : F64Arith {
 // LOGN = LOGM + HB.  With HB = 1 (n = 2^14) the limb is split between two
 // workgroups: each loads both halves of the digit, applies the first
 // Cooley-Tukey stage for its own half only, and runs the remaining stages on
-// a 2^13-word slice (68 KiB of LDS, so two workgroups share a CU).
+// a 2^13-word slice, which keeps the accumulators of 16 words per thread in
+// VGPRs without spilling.
 //
 // Workgroup order: logical index (l * count + ct) * 2^HB + h, limb-major.
 // Hardware hands workgroup b to XCD b mod 8; the logical index is remapped
@@ -75,8 +80,12 @@ __global__ void __launch_bounds__((1 << LOGM) >> LOGE)
   constexpr int LOGN = LOGM + HB;
   constexpr bool kPrep = A::ks_prep(LOGN);
   constexpr int kAccEvery = A::ks_acc_every(LOGN);
+  // Double-buffered LDS exchanges (two slices; one workgroup per CU is
+  // resident anyway, its VGPRs limit it).
+  using Passes = FwdPasses<LOGM, LOGE, 0, A, false, HB, SHG_KS_DB != 0>;
   extern __shared__ __align__(16) unsigned char smem[];
-  V* lds = reinterpret_cast<V*>(smem);
+  V* lds0 = reinterpret_cast<V*>(smem);
+  V* lds1 = lds0 + lds_words<LOGM>();
   unsigned b = blockIdx.x;
 #if SHG_KS_XCD_REMAP
   if ((gridDim.x & 7) == 0) b = (b & 7) * (gridDim.x >> 3) + (b >> 3);
@@ -155,16 +164,21 @@ __global__ void __launch_bounds__((1 << LOGM) >> LOGE)
       } else {
         load_digits([&](uint64_t s64) { return ar.digit_in(s64 & mask, P); });
       }
-      FwdPasses<LOGM, LOGE, 0, A, false, HB>::run(x, lds, t, h, ar);
+      Passes::run(x, lds0, lds1, t, h, ar);
+      if constexpr (SHG_KS_DB && (Passes::kExchanges & 1)) {
+        V* tmp = lds0;
+        lds0 = lds1;
+        lds1 = tmp;
+      }
       const uint64_t* kb = ka.key_b + (size_t)(ka.key_base + j) * poly_words + limb_off;
-      const uint64_t* kav = ka.key_a + (size_t)(ka.key_base + j) * poly_words + limb_off;
+      const uint64_t* kap = ka.key_a + (size_t)(ka.key_base + j) * poly_words + limb_off;
 #pragma unroll
       for (int gg = 0; gg < G; ++gg) {
         const size_t base = (size_t)(gg * T + t) << R;
 #pragma unroll
         for (int e = 0; e < (1 << R); e += 2) {
           const u64x2 kbv = *reinterpret_cast<const u64x2*>(kb + base + e);
-          const u64x2 kav2 = *reinterpret_cast<const u64x2*>(kav + base + e);
+          const u64x2 kav = *reinterpret_cast<const u64x2*>(kap + base + e);
           const int r = gg * (1 << R) + e;
           typename A::Prep d0, d1;
           if constexpr (kPrep) {
@@ -176,8 +190,8 @@ __global__ void __launch_bounds__((1 << LOGM) >> LOGE)
           }
           ar.mac(acc0[r], d0, kbv.x, P);
           ar.mac(acc0[r + 1], d1, kbv.y, P);
-          ar.mac(acc1[r], d0, kav2.x, P);
-          ar.mac(acc1[r + 1], d1, kav2.y, P);
+          ar.mac(acc1[r], d0, kav.x, P);
+          ar.mac(acc1[r + 1], d1, kav.y, P);
         }
       }
       if (((j + 1) % kAccEvery) == 0) {
@@ -187,9 +201,8 @@ __global__ void __launch_bounds__((1 << LOGM) >> LOGE)
           ar.acc_reduce(acc1[e]);
         }
       }
-      // The next digit's first pass writes LDS only after its registers are
-      // computed; all reads of this digit's last exchange are done by then
-      // (they precede the last pass), so no barrier is needed here.
+      // The buffer the next digit writes first was last read before the
+      // barrier of this digit's final exchange, so no barrier is needed.
     }
   }
 #pragma unroll
@@ -212,7 +225,7 @@ template <int LOGM, int LOGE, class A, int HB = 0>
 static hipError_t launch_ks(const KeySwitchArgs& ka, const GadgetDesc& gd, const PrimeDev* primes,
                             const LimbPrimes& lp, int count, int num_limbs, hipStream_t s) {
   constexpr int T = (1 << LOGM) >> LOGE;
-  const size_t shmem = (size_t)lds_words<LOGM>() * 8;
+  const size_t shmem = (size_t)lds_words<LOGM>() * 8 * (SHG_KS_DB ? 2 : 1);
   auto k = keyswitch_kernel<LOGM, LOGE, HB, A>;
   if (shmem > 65536)
     (void)hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);

@@ -20,6 +20,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "shg_device.h"
 
 #ifndef SHG_STAGE_BARRIER
@@ -325,25 +327,50 @@ __device__ __forceinline__ void fwd_from_lds(V (&x)[1 << LOGE], const V* lds, in
 // coalesced); otherwise it stays in VGPRs, where thread t holds, for each of
 // its G groups gamma = gg*T + t, the 2^R consecutive words
 // (gamma << R) .. (gamma << R) + 2^R - 1 (see FwdLast).
-template <int LOGM, int LOGE, int S0, class A, bool LAST_TO_LDS = true, int HB = 0>
+struct NoHook {
+  template <class P>
+  __device__ __forceinline__ void operator()(P) const {}
+};
+
+// With DB (double buffering) consecutive exchanges alternate between lds0
+// and lds1, so a pass may write its exchange as soon as it has computed: the
+// barrier that separates the previous exchange's reads from new writes to
+// the same buffer is the one after the intervening exchange's writes.  The
+// caller must continue the alternation across calls (swap the buffers after
+// a call with an odd number of exchanges).  hook(integral_constant<S0>) runs
+// at the start of every pass, before its twiddle loads, for callers that
+// issue their own loads early.
+template <int LOGM, int LOGE, int S0, class A, bool LAST_TO_LDS = true, int HB = 0,
+          bool DB = false>
 struct FwdPasses {
   static constexpr int R = (LOGM - S0) < LOGE ? (LOGM - S0) : LOGE;
-  __device__ __forceinline__ static void run(typename A::V (&x)[1 << LOGE], typename A::V* lds,
-                                             int t, int h, const A& ar) {
+  static constexpr int P = S0 / LOGE;  // pass index (all passes but the last have R = LOGE)
+  template <class Hook = NoHook>
+  __device__ __forceinline__ static void run(typename A::V (&x)[1 << LOGE], typename A::V* lds0,
+                                             typename A::V* lds1, int t, int h, const A& ar,
+                                             const Hook& hook = Hook()) {
+    hook(std::integral_constant<int, S0>());
     typename A::TwRaw tw[PassTw<LOGE, R>::N];
     fwd_pass_tw<LOGM, LOGE, S0, R, HB>(tw, t, h, ar);
     if constexpr (S0 > 0) {
-      fwd_from_lds<LOGM, LOGE, S0, R>(x, lds, t);
-      __syncthreads();
+      fwd_from_lds<LOGM, LOGE, S0, R>(x, (DB && ((P - 1) & 1)) ? lds1 : lds0, t);
+      if constexpr (!DB) __syncthreads();
     }
     fwd_pass<LOGM, LOGE, S0, R, HB>(x, tw, ar);
     if constexpr (S0 + R < LOGM || LAST_TO_LDS) {
-      fwd_to_lds<LOGM, LOGE, S0, R>(x, lds, t);
+      fwd_to_lds<LOGM, LOGE, S0, R>(x, (DB && (P & 1)) ? lds1 : lds0, t);
       __syncthreads();
     }
     if constexpr (S0 + R < LOGM)
-      FwdPasses<LOGM, LOGE, S0 + R, A, LAST_TO_LDS, HB>::run(x, lds, t, h, ar);
+      FwdPasses<LOGM, LOGE, S0 + R, A, LAST_TO_LDS, HB, DB>::run(x, lds0, lds1, t, h, ar, hook);
   }
+  __device__ __forceinline__ static void run(typename A::V (&x)[1 << LOGE], typename A::V* lds,
+                                             int t, int h, const A& ar) {
+    run(x, lds, lds, t, h, ar);
+  }
+  // LDS exchanges made by one run.
+  static constexpr int kPasses = (LOGM - S0 + LOGE - 1) / LOGE;
+  static constexpr int kExchanges = kPasses - 1 + (LAST_TO_LDS ? 1 : 0);
 };
 
 // Shape of the last forward pass: R_LAST stages, G_LAST groups per thread.
