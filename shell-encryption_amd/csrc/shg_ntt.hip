@@ -19,132 +19,137 @@
 #ifndef SHG_LOGE_14
 #define SHG_LOGE_14 4
 #endif
-#ifndef SHG_FWD_DIRECT_STORE
-#define SHG_FWD_DIRECT_STORE 0
-#endif
-#ifndef SHG_INV_DIRECT_LOAD
-#define SHG_INV_DIRECT_LOAD 0
-#endif
 
 namespace shg {
 
 // ---------------------------------------------------------------------------
-// Kernels.  blockIdx.x = row * 2^HB + h where row indexes limbs
-// (count * num_limbs of them) and h the slice within the limb.
+// Kernels.  Work item i = row * 2^HB + h, where row indexes limbs
+// (count * num_limbs of them) and h the slice within the limb.  The grid is
+// at most the number of resident workgroups; each workgroup loops over items
+// i = blockIdx.x, blockIdx.x + gridDim.x, ... and loads the next item's
+// words while it transforms the current one.  Vector loads complete in
+// issue order, so the prefetch is issued after the first pass's twiddle
+// loads: it then overlaps the first pass and its exchange, and the stores
+// of the previous item drain meanwhile.  The grid is a multiple of the
+// items per limb period (num_limbs * 2^HB) when it can be, so a workgroup
+// keeps one prime; with 8 limbs each XCD (workgroup b runs on XCD b mod 8)
+// then serves one prime's twiddles from its own L2.
 // ---------------------------------------------------------------------------
+
+template <int LOGM, int HB>
+__device__ __forceinline__ uint64_t* slice_ptr(uint64_t* data, int item) {
+  constexpr int logn = LOGM + HB;
+  return data + ((size_t)(item >> HB) << logn) + ((size_t)(item & ((1 << HB) - 1)) << LOGM);
+}
 
 // HB = log n - LOGM: the number of leading stages already applied by
 // streaming passes (limbs larger than one LDS slice).
-template <int LOGM, int LOGE, int HB, class A>
+template <int LOGM, int LOGE, int HB, class A, bool PF>
 __global__ void __launch_bounds__((1 << LOGM) >> LOGE)
     ntt_fwd_kernel(uint64_t* __restrict__ data, const PrimeDev* __restrict__ primes,
-                   LimbPrimes lp, int num_limbs) {
+                   LimbPrimes lp, int num_limbs, int items) {
   using V = typename A::V;
   constexpr int E = 1 << LOGE;
   constexpr int T = (1 << LOGM) >> LOGE;
-  constexpr int logn = LOGM + HB;
   extern __shared__ __align__(16) unsigned char smem[];
   V* lds = reinterpret_cast<V*>(smem);
-  const int h = blockIdx.x & ((1 << HB) - 1);
-  const int row = blockIdx.x >> HB;
-  const PrimeDev& P = primes[lp.id[row % num_limbs]];
-  const A ar(P);
-  uint64_t* g = data + ((size_t)row << logn) + ((size_t)h << LOGM);
   const int t = threadIdx.x;
-  V x[E];
-  // First pass reads straight from HBM: element e of thread t is word e*T+t.
+  int item = blockIdx.x;
+  uint64_t pf[E];  // the words of the item to transform next
+  if (PF && item < items) {
+    const uint64_t* g = slice_ptr<LOGM, HB>(data, item);
 #pragma unroll
-  for (int e = 0; e < E; ++e) x[e] = ar.load(g[e * T + t]);
-#if SHG_FWD_DIRECT_STORE
-  FwdPasses<LOGM, LOGE, 0, A, false, HB>::run(x, lds, t, h, ar);
-  // The last pass left 2^R consecutive words per group in VGPRs: store them
-  // with 16-byte writes.
-  constexpr int R = FwdLast<LOGM, LOGE>::R_LAST;
-  constexpr int G = FwdLast<LOGM, LOGE>::G_LAST;
-#pragma unroll
-  for (int gg = 0; gg < G; ++gg) {
-    uint64_t* dst = g + ((size_t)(gg * T + t) << R);
-    if constexpr (R >= 1) {
-#pragma unroll
-      for (int e = 0; e < (1 << R); e += 2) {
-        u64x2 v;
-        v.x = ar.fwd_out(x[gg * (1 << R) + e]);
-        v.y = ar.fwd_out(x[gg * (1 << R) + e + 1]);
-        *reinterpret_cast<u64x2*>(dst + e) = v;
-      }
-    } else {
-      dst[0] = ar.fwd_out(x[gg]);
-    }
+    for (int e = 0; e < E; ++e) pf[e] = g[e * T + t];
   }
-#else
-  // The last pass goes through LDS so that lanes store consecutive words
-  // (measured faster than 16-byte thread-contiguous stores).
-  FwdPasses<LOGM, LOGE, 0, A, true, HB>::run(x, lds, t, h, ar);
+  for (; item < items; item += gridDim.x) {
+    const int h = item & ((1 << HB) - 1);
+    const A ar(primes[lp.id[(item >> HB) % num_limbs]]);
+    uint64_t* g = slice_ptr<LOGM, HB>(data, item);
+    // Element e of thread t is word e*T + t of the slice.
+    if constexpr (!PF) {
 #pragma unroll
-  for (int e = 0; e < E; ++e) g[e * T + t] = ar.fwd_out(lds[lpad(e * T + t)]);
-#endif
+      for (int e = 0; e < E; ++e) pf[e] = g[e * T + t];
+    }
+    V x[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) x[e] = ar.load(pf[e]);
+    const int next = item + gridDim.x;
+    auto prefetch = [&](auto s0) {
+      if constexpr (PF && decltype(s0)::value == 0) {
+        if (next < items) {
+          const uint64_t* gn = slice_ptr<LOGM, HB>(data, next);
+#pragma unroll
+          for (int e = 0; e < E; ++e) pf[e] = gn[e * T + t];
+        }
+      }
+    };
+    // The last pass goes through LDS so that lanes store consecutive words.
+    FwdPasses<LOGM, LOGE, 0, A, true, HB>::run(x, lds, lds, t, h, ar, prefetch);
+#pragma unroll
+    for (int e = 0; e < E; ++e) g[e * T + t] = ar.fwd_out(lds[lpad(e * T + t)]);
+    // The next item's first exchange overwrites what was just read.
+    __syncthreads();
+  }
 }
 
 // HB = log n - LOGM: the number of trailing stages left to streaming passes.
-template <int LOGM, int LOGE, int HB, class A>
+template <int LOGM, int LOGE, int HB, class A, bool PF>
 __global__ void __launch_bounds__((1 << LOGM) >> LOGE)
     ntt_inv_kernel(uint64_t* __restrict__ data, const PrimeDev* __restrict__ primes,
-                   LimbPrimes lp, int num_limbs, int scale) {
+                   LimbPrimes lp, int num_limbs, int items, int scale) {
   using V = typename A::V;
   constexpr int E = 1 << LOGE;
   constexpr int T = (1 << LOGM) >> LOGE;
   constexpr int logn = LOGM + HB;
   extern __shared__ __align__(16) unsigned char smem[];
   V* lds = reinterpret_cast<V*>(smem);
-  const int h = blockIdx.x & ((1 << HB) - 1);
-  const int row = blockIdx.x >> HB;
-  const PrimeDev& P = primes[lp.id[row % num_limbs]];
-  const A ar(P);
-  uint64_t* g = data + ((size_t)row << logn) + ((size_t)h << LOGM);
   const int t = threadIdx.x;
-#if SHG_INV_DIRECT_LOAD
-  // The first pass toggles the low local bits, so each thread loads its
-  // groups' consecutive words straight from HBM with 16-byte reads.
-  V x[E];
-  {
-    constexpr int R0 = LOGM < LOGE ? LOGM : LOGE;
-    constexpr int G0 = E >> R0;
+  int item = blockIdx.x;
+  uint64_t pf[E];
+  if (PF && item < items) {
+    const uint64_t* g = slice_ptr<LOGM, HB>(data, item);
 #pragma unroll
-    for (int gg = 0; gg < G0; ++gg) {
-      const uint64_t* src = g + ((size_t)(gg * T + t) << R0);
-      if constexpr (R0 >= 1) {
+    for (int e = 0; e < E; ++e) pf[e] = g[e * T + t];
+  }
+  for (; item < items; item += gridDim.x) {
+    const int h = item & ((1 << HB) - 1);
+    const A ar(primes[lp.id[(item >> HB) % num_limbs]]);
+    uint64_t* g = slice_ptr<LOGM, HB>(data, item);
+    const int next = item + gridDim.x;
+    // Stage the current words in LDS (the first pass reads them back in its
+    // own layout); the next item's loads follow the first pass's twiddles.
+    if constexpr (!PF) {
 #pragma unroll
-        for (int e = 0; e < (1 << R0); e += 2) {
-          const u64x2 v = *reinterpret_cast<const u64x2*>(src + e);
-          x[gg * (1 << R0) + e] = ar.load(v.x);
-          x[gg * (1 << R0) + e + 1] = ar.load(v.y);
+      for (int e = 0; e < E; ++e) pf[e] = g[e * T + t];
+    }
+#pragma unroll
+    for (int e = 0; e < E; ++e) lds[lpad(e * T + t)] = ar.load(pf[e]);
+    __syncthreads();
+    auto prefetch = [&](auto i0) {
+      if constexpr (PF && decltype(i0)::value == 0) {
+        if (next < items) {
+          const uint64_t* gn = slice_ptr<LOGM, HB>(data, next);
+#pragma unroll
+          for (int e = 0; e < E; ++e) pf[e] = gn[e * T + t];
         }
-      } else {
-        x[gg] = ar.load(src[0]);
+      }
+    };
+    V x[E];
+    InvPasses<LOGM, LOGE, 0, A, false, logn>::run(x, lds, t, h, ar, prefetch);
+    // The last pass toggles the top local bits: word (e << I0) | gamma.
+    constexpr int I0 = InvLast<LOGM, LOGE, 0, A>::value;
+    constexpr int R = LOGM - I0;
+    constexpr int G = E >> R;
+#pragma unroll
+    for (int gg = 0; gg < G; ++gg) {
+#pragma unroll
+      for (int e = 0; e < (1 << R); ++e) {
+        const V v = x[gg * (1 << R) + e];
+        g[(e << I0) | (gg * T + t)] = ar.inv_out(v, scale);
       }
     }
-  }
-  InvPasses<LOGM, LOGE, 0, A, true, logn>::run(x, lds, t, h, ar);
-#else
-  // Coalesced load through LDS (measured faster than thread-contiguous
-  // 16-byte reads).
-#pragma unroll
-  for (int e = 0; e < E; ++e) lds[lpad(e * T + t)] = ar.load(g[e * T + t]);
-  __syncthreads();
-  V x[E];
-  InvPasses<LOGM, LOGE, 0, A, false, logn>::run(x, lds, t, h, ar);
-#endif
-  // The last pass toggles the top local bits: word (e << I0) | gamma.
-  constexpr int I0 = InvLast<LOGM, LOGE, 0, A>::value;
-  constexpr int R = LOGM - I0;
-  constexpr int G = E >> R;
-#pragma unroll
-  for (int gg = 0; gg < G; ++gg) {
-#pragma unroll
-    for (int e = 0; e < (1 << R); ++e) {
-      const V v = x[gg * (1 << R) + e];
-      g[(e << I0) | (gg * T + t)] = ar.inv_out(v, scale);
-    }
+    // The last pass read its exchange before its own barrier, so the next
+    // item may write LDS without another one.
   }
 }
 
@@ -188,23 +193,53 @@ __global__ void ntt_stream_stage(uint64_t* __restrict__ data, const PrimeDev* __
 // Launchers
 // ---------------------------------------------------------------------------
 
+// Prefetching the next item costs 2E VGPRs.  At E = 16 (1024 threads, 128
+// VGPRs) that spills and measured slower (fwd 0.80 vs 0.63 ms per 8192 limbs
+// of 2^14), so it is off by default; the integer policy never has room.
+#ifndef SHG_NTT_PREFETCH
+#define SHG_NTT_PREFETCH 0
+#endif
+template <class A>
+constexpr bool kPrefetch = SHG_NTT_PREFETCH && std::is_same<A, F64Arith>::value;
+
+// Workgroups resident at once on the current device for kernel k (at least
+// one per CU), capped by the item count and rounded down to a multiple of
+// `period` when that leaves at least one period.
+static int persistent_grid(const void* k, int threads, size_t shmem, int items, int period) {
+  int dev = 0, cus = 0, per_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+    return items;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, threads, shmem) != hipSuccess ||
+      per_cu <= 0)
+    per_cu = 1;
+  long long cap = (long long)cus * per_cu;
+  if (cap >= items) return items;
+  if (period > 0 && cap >= period) cap -= cap % period;
+  return (int)cap;
+}
+
 template <int LOGM, int LOGE, int HB, class A>
 static hipError_t launch_fused_hb(bool inverse, uint64_t* data, const PrimeDev* primes,
                                   const LimbPrimes& lp, int rows, int num_limbs, int scale,
                                   hipStream_t stream) {
   constexpr int T = (1 << LOGM) >> LOGE;
   const size_t shmem = (size_t)lds_words<LOGM>() * 8;
-  const unsigned grid = (unsigned)rows << HB;
+  const int items = rows << HB;
+  const int period = num_limbs << HB;
   if (!inverse) {
-    auto k = ntt_fwd_kernel<LOGM, LOGE, HB, A>;
+    auto k = ntt_fwd_kernel<LOGM, LOGE, HB, A, kPrefetch<A>>;
     if (shmem > 65536)
       (void)hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
-    hipLaunchKernelGGL(k, dim3(grid), dim3(T), shmem, stream, data, primes, lp, num_limbs);
+    const int grid = persistent_grid((const void*)k, T, shmem, items, period);
+    hipLaunchKernelGGL(k, dim3(grid), dim3(T), shmem, stream, data, primes, lp, num_limbs, items);
   } else {
-    auto k = ntt_inv_kernel<LOGM, LOGE, HB, A>;
+    auto k = ntt_inv_kernel<LOGM, LOGE, HB, A, kPrefetch<A>>;
     if (shmem > 65536)
       (void)hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
-    hipLaunchKernelGGL(k, dim3(grid), dim3(T), shmem, stream, data, primes, lp, num_limbs, scale);
+    const int grid = persistent_grid((const void*)k, T, shmem, items, period);
+    hipLaunchKernelGGL(k, dim3(grid), dim3(T), shmem, stream, data, primes, lp, num_limbs, items,
+                       scale);
   }
   return hipGetLastError();
 }

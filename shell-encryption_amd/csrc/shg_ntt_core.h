@@ -338,8 +338,10 @@ struct NoHook {
 // the same buffer is the one after the intervening exchange's writes.  The
 // caller must continue the alternation across calls (swap the buffers after
 // a call with an odd number of exchanges).  hook(integral_constant<S0>) runs
-// at the start of every pass, before its twiddle loads, for callers that
-// issue their own loads early.
+// in every pass right after its twiddle loads are issued and before its LDS
+// reads, for callers that issue loads of their own there (vector loads
+// complete in issue order, so a load issued after the twiddles does not
+// delay this pass's twiddles).
 template <int LOGM, int LOGE, int S0, class A, bool LAST_TO_LDS = true, int HB = 0,
           bool DB = false>
 struct FwdPasses {
@@ -349,9 +351,9 @@ struct FwdPasses {
   __device__ __forceinline__ static void run(typename A::V (&x)[1 << LOGE], typename A::V* lds0,
                                              typename A::V* lds1, int t, int h, const A& ar,
                                              const Hook& hook = Hook()) {
-    hook(std::integral_constant<int, S0>());
     typename A::TwRaw tw[PassTw<LOGE, R>::N];
     fwd_pass_tw<LOGM, LOGE, S0, R, HB>(tw, t, h, ar);
+    hook(std::integral_constant<int, S0>());
     if constexpr (S0 > 0) {
       fwd_from_lds<LOGM, LOGE, S0, R>(x, (DB && ((P - 1) & 1)) ? lds1 : lds0, t);
       if constexpr (!DB) __syncthreads();
@@ -476,13 +478,18 @@ __device__ __forceinline__ void inv_from_lds(V (&x)[1 << LOGE], const V* lds, in
 // With FIRST_IN_REGS the caller has already loaded the first pass's words
 // into VGPRs: thread t holds, for each group gamma = gg*T + t, the 2^R
 // consecutive words (gamma << R) + e (the first pass toggles the low bits).
+// hook(integral_constant<I0>) runs as in FwdPasses: after the pass's
+// twiddle loads, before its LDS reads.
 template <int LOGM, int LOGE, int I0, class A, bool FIRST_IN_REGS = false, int LOGN = LOGM>
 struct InvPasses {
   static constexpr int R = (LOGM - I0) < LOGE ? (LOGM - I0) : LOGE;
+  template <class Hook = NoHook>
   __device__ __forceinline__ static void run(typename A::V (&x)[1 << LOGE], typename A::V* lds,
-                                             int t, int h, const A& ar) {
+                                             int t, int h, const A& ar,
+                                             const Hook& hook = Hook()) {
     typename A::TwRaw tw[PassTw<LOGE, R>::N];
     inv_pass_tw<LOGM, LOGE, I0, R, LOGN>(tw, t, h, ar);
+    hook(std::integral_constant<int, I0>());
     if constexpr (!(FIRST_IN_REGS && I0 == 0)) {
       inv_from_lds<LOGM, LOGE, I0, R>(x, lds, t);
       __syncthreads();
@@ -491,7 +498,7 @@ struct InvPasses {
     if constexpr (I0 + R < LOGM) {
       inv_to_lds<LOGM, LOGE, I0, R>(x, lds, t);
       __syncthreads();
-      InvPasses<LOGM, LOGE, I0 + R, A, false, LOGN>::run(x, lds, t, h, ar);
+      InvPasses<LOGM, LOGE, I0 + R, A, false, LOGN>::run(x, lds, t, h, ar, hook);
     }
   }
 };
