@@ -177,19 +177,47 @@ int shg_lazy_fma(shg_context* ctx, const uint64_t* as, size_t as_stride,
                  const uint64_t* bs, size_t bs_stride, int num_terms,
                  uint64_t* out, int num_limbs, const int* primes);
 
-/* ---- batched uneven work (persistent launch) --------------------------- */
-/* One job = one limb of n = 2^log_n words at `data` (device), reduced modulo
- * `prime` (whose log_n must match). */
+/* ---- batched uneven work (persistent launch) ---------------------------
+ * No counterpart in the reference, which transforms one RnsPolynomial limb
+ * at a time (rns/rns_polynomial.cc:171-207): these run the per-limb
+ * operations of many polynomials of different ring degree and level in one
+ * launch.  One job = one limb of n = 2^log_n words at `data` (device,
+ * 16-byte aligned), 2^11 <= n <= 2^15, reduced modulo `prime` (whose log_n
+ * must match).  Jobs must not overlap. */
 typedef struct shg_limb_job {
   uint64_t* data;
-  const uint64_t* other; /* second operand for pointwise jobs, else NULL */
+  const uint64_t* other; /* second operand for multiply jobs, else NULL */
   int prime;
   int log_n;
 } shg_limb_job;
-/* Runs `op` (0 = forward NTT, 1 = inverse NTT, 2 = pointwise multiply
- * data <- data * other) over all jobs in one persistent launch. */
+#define SHG_BATCH_NTT_FWD 0 /* ForwardNumberTheoreticTransform */
+#define SHG_BATCH_NTT_INV 1 /* InverseNumberTheoreticTransform */
+#define SHG_BATCH_MUL 2     /* MulInPlace: data <- data * other (REDC) */
+typedef struct shg_batch_plan shg_batch_plan;
+/* Validates the jobs, uploads the job table (sorted by decreasing size) and
+ * sizes the persistent grid.  variant as for shg_ntt_fwd (AUTO takes the
+ * autotuner's choice for the plan's largest ring). */
+int shg_batch_plan_create(shg_context* ctx, const shg_limb_job* host_jobs,
+                          int num_jobs, int variant, shg_batch_plan** out);
+int shg_batch_plan_destroy(shg_context* ctx, shg_batch_plan* plan);
+/* One kernel launch on the context's stream, nothing else: may be captured
+ * in a graph (shg_graph_begin). */
+int shg_batch_run(shg_context* ctx, const shg_batch_plan* plan, int op);
+/* Plan, run and wait in one call. */
 int shg_batch_limbs(shg_context* ctx, const shg_limb_job* host_jobs,
                     int num_jobs, int op);
+
+/* ---- HIP graphs of op chains --------------------------------------------
+ * Captures every launch the context makes between begin and end into a
+ * graph (stream capture on the context's stream); launch replays it.  Only
+ * entry points that launch work without host synchronisation or allocation
+ * may be called while capturing: the transforms, pointwise ops, substitute,
+ * tensor product, lazy FMA and shg_batch_run. */
+typedef struct shg_graph shg_graph;
+int shg_graph_begin(shg_context* ctx);
+int shg_graph_end(shg_context* ctx, shg_graph** out);
+int shg_graph_launch(shg_context* ctx, const shg_graph* graph);
+int shg_graph_destroy(shg_context* ctx, shg_graph* graph);
 
 /* ---- timing -------------------------------------------------------------
  * Per-kernel timing with HIP events recorded on the context's stream around

@@ -810,8 +810,167 @@ int shg_profile_query(shg_context* ctx, const char* kernel, double* total_ms, lo
   return SHG_OK;
 }
 
+}  // extern "C"
+
+struct shg_batch_plan {
+  shg::LimbJobDev* d_jobs = nullptr;
+  int num_jobs = 0;
+  int variant = SHG_NTT_F64;
+  int grid = 0;
+  size_t shmem = 0;
+  bool has_other = true;  // every job has a second operand (multiply allowed)
+};
+
+struct shg_graph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+extern "C" {
+
+int shg_batch_plan_create(shg_context* ctx, const shg_limb_job* host_jobs, int num_jobs,
+                          int variant, shg_batch_plan** out) {
+  if (!ctx || !out) return fail(SHG_E_INVALID_ARGUMENT, "null argument.");
+  if (num_jobs < 0 || (num_jobs > 0 && !host_jobs))
+    return fail(SHG_E_INVALID_ARGUMENT, "jobs must be a non-negative count with a job array.");
+  if (variant < SHG_NTT_AUTO || variant > SHG_NTT_F64)
+    return fail(SHG_E_INVALID_ARGUMENT, "unknown transform variant %d.", variant);
+  std::vector<shg::LimbJobDev> jobs(num_jobs);
+  bool f64_ok = true, has_other = true;
+  int max_log_n = shg::kBatchMinLogN;
+  {
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    for (int i = 0; i < num_jobs; ++i) {
+      const shg_limb_job& j = host_jobs[i];
+      if (j.prime < 0 || j.prime >= (int)ctx->primes.size())
+        return fail(SHG_E_INVALID_ARGUMENT, "job %d: invalid prime handle %d.", i, j.prime);
+      if (j.log_n < shg::kBatchMinLogN || j.log_n > shg::kBatchMaxLogN)
+        return fail(SHG_E_INVALID_ARGUMENT, "job %d: batched jobs need 2^%d <= n <= 2^%d.", i,
+                    shg::kBatchMinLogN, shg::kBatchMaxLogN);
+      if (ctx->primes[j.prime]->log_n != j.log_n)
+        return fail(SHG_E_INVALID_ARGUMENT, "job %d: prime was created for another ring degree.", i);
+      if (!aligned16(j.data) || (j.other && !aligned16(j.other)))
+        return fail(SHG_E_INVALID_ARGUMENT, "job %d: data must be 16-byte aligned device memory.", i);
+      if (ctx->primes[j.prime]->q >= (1ull << 50)) f64_ok = false;
+      has_other &= j.other != nullptr;
+      max_log_n = j.log_n > max_log_n ? j.log_n : max_log_n;
+      jobs[i] = shg::LimbJobDev{j.data, j.other, j.prime, j.log_n};
+    }
+  }
+  if (variant == SHG_NTT_F64 && !f64_ok)
+    return fail(SHG_E_INVALID_ARGUMENT, "FP64 transform requires moduli below 2^50.");
+  int v = variant;
+  if (v == SHG_NTT_AUTO) {
+    v = f64_ok ? SHG_NTT_F64 : SHG_NTT_INT_SHOUP;
+    auto it = ctx->tuned.find({max_log_n, -1});
+    if (f64_ok && it != ctx->tuned.end()) v = it->second;
+  }
+  // Largest rings first so the static stride balances the workgroups;
+  // then by prime for twiddle locality.
+  std::stable_sort(jobs.begin(), jobs.end(), [](const shg::LimbJobDev& a, const shg::LimbJobDev& b) {
+    return a.log_n != b.log_n ? a.log_n > b.log_n : a.prime < b.prime;
+  });
+  auto plan = std::make_unique<shg_batch_plan>();
+  plan->num_jobs = num_jobs;
+  plan->variant = v;
+  plan->has_other = has_other;
+  plan->shmem = shg::batch_shmem_bytes(max_log_n);
+  if (num_jobs > 0) {
+    const void* k = shg::batch_kernel_ptr(v);
+    if (plan->shmem > 65536)
+      shg::set_max_dynamic_shmem(k, plan->shmem);
+    int cus = 0, per_cu = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess ||
+        cus <= 0)
+      cus = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, shg::kBatchThreads, plan->shmem) !=
+            hipSuccess || per_cu <= 0)
+      per_cu = 1;
+    const long long cap = (long long)cus * per_cu;
+    plan->grid = (int)(cap < num_jobs ? cap : num_jobs);
+    hipError_t e = hipMalloc(&plan->d_jobs, sizeof(shg::LimbJobDev) * num_jobs);
+    if (e != hipSuccess) return fail(SHG_E_NO_MEMORY, "job table allocation failed.");
+    e = hipMemcpy(plan->d_jobs, jobs.data(), sizeof(shg::LimbJobDev) * num_jobs,
+                  hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      (void)hipFree(plan->d_jobs);
+      return hip_fail(e, "job table upload");
+    }
+  }
+  *out = plan.release();
+  return SHG_OK;
+}
+
+int shg_batch_plan_destroy(shg_context* ctx, shg_batch_plan* plan) {
+  if (!ctx || !plan) return fail(SHG_E_INVALID_ARGUMENT, "null argument.");
+  if (plan->d_jobs) {
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(plan->d_jobs);
+  }
+  delete plan;
+  return SHG_OK;
+}
+
+int shg_batch_run(shg_context* ctx, const shg_batch_plan* plan, int op) {
+  if (!ctx || !plan) return fail(SHG_E_INVALID_ARGUMENT, "null argument.");
+  if (op != SHG_BATCH_NTT_FWD && op != SHG_BATCH_NTT_INV && op != SHG_BATCH_MUL)
+    return fail(SHG_E_INVALID_ARGUMENT, "unknown batch op %d.", op);
+  if (op == SHG_BATCH_MUL && !plan->has_other)
+    return fail(SHG_E_INVALID_ARGUMENT, "multiply jobs need a second operand.");
+  ProfScope prof(ctx, "batch");
+  hipError_t e = shg::launch_batch_limbs(plan->d_jobs, plan->num_jobs, op, plan->variant,
+                                         ctx->d_primes, plan->grid, plan->shmem, ctx->stream);
+  return e == hipSuccess ? SHG_OK : hip_fail(e, "batch launch");
+}
+
 int shg_batch_limbs(shg_context* ctx, const shg_limb_job* host_jobs, int num_jobs, int op) {
-  return fail(SHG_E_UNSUPPORTED, "batched limb jobs are not available in this build.");
+  shg_batch_plan* plan = nullptr;
+  int rc = shg_batch_plan_create(ctx, host_jobs, num_jobs, SHG_NTT_AUTO, &plan);
+  if (rc) return rc;
+  rc = shg_batch_run(ctx, plan, op);
+  if (rc == SHG_OK) {
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) rc = hip_fail(e, "batch");
+  }
+  const std::string msg = g_last_error;
+  shg_batch_plan_destroy(ctx, plan);
+  if (rc) g_last_error = msg;
+  return rc;
+}
+
+int shg_graph_begin(shg_context* ctx) {
+  if (!ctx) return fail(SHG_E_INVALID_ARGUMENT, "context must not be null.");
+  hipError_t e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
+  return e == hipSuccess ? SHG_OK : hip_fail(e, "begin capture");
+}
+
+int shg_graph_end(shg_context* ctx, shg_graph** out) {
+  if (!ctx || !out) return fail(SHG_E_INVALID_ARGUMENT, "null argument.");
+  auto g = std::make_unique<shg_graph>();
+  hipError_t e = hipStreamEndCapture(ctx->stream, &g->graph);
+  if (e != hipSuccess) return hip_fail(e, "end capture");
+  e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    (void)hipGraphDestroy(g->graph);
+    return hip_fail(e, "graph instantiate");
+  }
+  *out = g.release();
+  return SHG_OK;
+}
+
+int shg_graph_launch(shg_context* ctx, const shg_graph* graph) {
+  if (!ctx || !graph) return fail(SHG_E_INVALID_ARGUMENT, "null argument.");
+  hipError_t e = hipGraphLaunch(graph->exec, ctx->stream);
+  return e == hipSuccess ? SHG_OK : hip_fail(e, "graph launch");
+}
+
+int shg_graph_destroy(shg_context* ctx, shg_graph* graph) {
+  if (!ctx || !graph) return fail(SHG_E_INVALID_ARGUMENT, "null argument.");
+  (void)hipStreamSynchronize(ctx->stream);
+  if (graph->exec) (void)hipGraphExecDestroy(graph->exec);
+  if (graph->graph) (void)hipGraphDestroy(graph->graph);
+  delete graph;
+  return SHG_OK;
 }
 
 int shg_autotune_ntt(shg_context* ctx, int log_n, int limbs, int* chosen) {

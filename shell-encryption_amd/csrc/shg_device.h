@@ -50,6 +50,11 @@ struct PrimeDev {
 // LDS wait counter and so serialise with the LDS exchanges.
 template <class T>
 using gptr = const T __attribute__((address_space(1)))*;
+// Writable global pointer: data pointers read from device tables (rather
+// than kernel arguments) are otherwise flat, and flat stores may alias LDS,
+// which serialises them with the LDS exchanges.
+template <class T>
+using gmut = T __attribute__((address_space(1)))*;
 
 struct LimbPrimes {
   int id[SHG_MAX_LIMBS];

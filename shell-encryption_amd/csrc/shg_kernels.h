@@ -109,15 +109,26 @@ hipError_t launch_lazy_fma(const uint64_t* as, size_t as_stride, const uint64_t*
                            const PrimeDev* primes, const LimbPrimes& lp, int num_limbs,
                            int logn, hipStream_t s);
 
-// Persistent launch over uneven limb jobs.
+// Sets hipFuncAttributeMaxDynamicSharedMemorySize once per kernel and device.
+void set_max_dynamic_shmem(const void* kernel, size_t bytes);
+
+// Persistent launch over uneven limb jobs (2^11 <= n <= 2^15).
 struct LimbJobDev {
   uint64_t* data;
   const uint64_t* other;
   int prime;
   int log_n;
 };
+constexpr int kBatchThreads = 1024;
+constexpr int kBatchMinLogN = 11;
+constexpr int kBatchMaxLogN = 15;
+constexpr int kBatchNttFwd = 0;
+constexpr int kBatchNttInv = 1;
+constexpr int kBatchMul = 2;
+size_t batch_shmem_bytes(int max_log_n);
+const void* batch_kernel_ptr(int variant);
 hipError_t launch_batch_limbs(const LimbJobDev* jobs, int num_jobs, int op, int variant,
-                              const PrimeDev* primes, int grid, hipStream_t s);
+                              const PrimeDev* primes, int grid, size_t shmem, hipStream_t s);
 
 }  // namespace shg
 

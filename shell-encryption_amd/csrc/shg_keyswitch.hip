@@ -227,8 +227,7 @@ static hipError_t launch_ks(const KeySwitchArgs& ka, const GadgetDesc& gd, const
   constexpr int T = (1 << LOGM) >> LOGE;
   const size_t shmem = (size_t)lds_words<LOGM>() * 8 * (SHG_KS_DB ? 2 : 1);
   auto k = keyswitch_kernel<LOGM, LOGE, HB, A>;
-  if (shmem > 65536)
-    (void)hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
+  if (shmem > 65536) set_max_dynamic_shmem((const void*)k, shmem);
   hipLaunchKernelGGL(k, dim3((unsigned)(count * num_limbs) << HB), dim3(T), shmem, s, ka, gd,
                      primes, lp, count, num_limbs);
   return hipGetLastError();

@@ -9,6 +9,10 @@
 // passes over HBM.
 #include <hip/hip_runtime.h>
 
+#include <map>
+#include <mutex>
+#include <type_traits>
+
 #include "shg_device.h"
 #include "shg_kernels.h"
 #include "shg_ntt_core.h"
@@ -193,6 +197,20 @@ __global__ void ntt_stream_stage(uint64_t* __restrict__ data, const PrimeDev* __
 // Launchers
 // ---------------------------------------------------------------------------
 
+// Raises a kernel's dynamic LDS limit once per (kernel, device): later
+// launches (for instance under graph capture) make no attribute calls.
+void set_max_dynamic_shmem(const void* k, size_t bytes) {
+  static std::mutex mu;
+  static std::map<std::pair<const void*, int>, size_t> done;
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  std::lock_guard<std::mutex> lock(mu);
+  size_t& cur = done[{k, dev}];
+  if (cur >= bytes) return;
+  if (hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) == hipSuccess)
+    cur = bytes;
+}
+
 // Prefetching the next item costs 2E VGPRs.  At E = 16 (1024 threads, 128
 // VGPRs) that spills and measured slower (fwd 0.80 vs 0.63 ms per 8192 limbs
 // of 2^14), so it is off by default; the integer policy never has room.
@@ -205,15 +223,31 @@ constexpr bool kPrefetch = SHG_NTT_PREFETCH && std::is_same<A, F64Arith>::value;
 // Workgroups resident at once on the current device for kernel k (at least
 // one per CU), capped by the item count and rounded down to a multiple of
 // `period` when that leaves at least one period.
+// The residency per (kernel, device) is cached, so launches after the first
+// (for instance while a graph is being captured) make no runtime queries.
 static int persistent_grid(const void* k, int threads, size_t shmem, int items, int period) {
-  int dev = 0, cus = 0, per_cu = 0;
-  if (hipGetDevice(&dev) != hipSuccess ||
-      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
-    return items;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, threads, shmem) != hipSuccess ||
-      per_cu <= 0)
-    per_cu = 1;
-  long long cap = (long long)cus * per_cu;
+  static std::mutex mu;
+  static std::map<std::pair<const void*, int>, long long> cache;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return items;
+  long long cap = 0;
+  {
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = cache.find({k, dev});
+    if (it != cache.end()) cap = it->second;
+  }
+  if (cap == 0) {
+    int cus = 0, per_cu = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        cus <= 0)
+      return items;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, threads, shmem) != hipSuccess ||
+        per_cu <= 0)
+      per_cu = 1;
+    cap = (long long)cus * per_cu;
+    std::lock_guard<std::mutex> lock(mu);
+    cache[{k, dev}] = cap;
+  }
   if (cap >= items) return items;
   if (period > 0 && cap >= period) cap -= cap % period;
   return (int)cap;
@@ -230,13 +264,13 @@ static hipError_t launch_fused_hb(bool inverse, uint64_t* data, const PrimeDev* 
   if (!inverse) {
     auto k = ntt_fwd_kernel<LOGM, LOGE, HB, A, kPrefetch<A>>;
     if (shmem > 65536)
-      (void)hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
+      set_max_dynamic_shmem((const void*)k, shmem);
     const int grid = persistent_grid((const void*)k, T, shmem, items, period);
     hipLaunchKernelGGL(k, dim3(grid), dim3(T), shmem, stream, data, primes, lp, num_limbs, items);
   } else {
     auto k = ntt_inv_kernel<LOGM, LOGE, HB, A, kPrefetch<A>>;
     if (shmem > 65536)
-      (void)hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
+      set_max_dynamic_shmem((const void*)k, shmem);
     const int grid = persistent_grid((const void*)k, T, shmem, items, period);
     hipLaunchKernelGGL(k, dim3(grid), dim3(T), shmem, stream, data, primes, lp, num_limbs, items,
                        scale);

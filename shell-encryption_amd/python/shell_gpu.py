@@ -17,6 +17,7 @@ INTP = ctypes.POINTER(ctypes.c_int)
 VP = ctypes.c_void_p
 
 OP_ADD, OP_SUB, OP_MUL, OP_NEG, OP_FMA = 0, 1, 2, 3, 4
+BATCH_NTT_FWD, BATCH_NTT_INV, BATCH_MUL = 0, 1, 2
 NTT_AUTO, NTT_INT, NTT_F64 = 0, 1, 2
 
 _lib = None
@@ -66,6 +67,13 @@ def lib():
         "shg_gadget_decompose": [VP, VP, INTP, VP, i, i, INTP],
         "shg_lazy_fma": [VP, VP, sz, VP, sz, i, VP, i, INTP],
         "shg_batch_limbs": [VP, ctypes.POINTER(LimbJob), i, i],
+        "shg_batch_plan_create": [VP, ctypes.POINTER(LimbJob), i, i, ctypes.POINTER(VP)],
+        "shg_batch_plan_destroy": [VP, VP],
+        "shg_batch_run": [VP, VP, i],
+        "shg_graph_begin": [VP],
+        "shg_graph_end": [VP, ctypes.POINTER(VP)],
+        "shg_graph_launch": [VP, VP],
+        "shg_graph_destroy": [VP, VP],
         "shg_autotune_ntt": [VP, i, i, INTP],
         "shg_profile_enable": [VP, i],
         "shg_profile_reset": [VP],
@@ -257,8 +265,20 @@ class Context:
                                   int(num_terms), out.p, len(primes), _ints(primes)))
 
     def batch_limbs(self, jobs, op):
+        """jobs: (data_ptr, other_ptr or None, prime, log_n) tuples."""
         arr = (LimbJob * len(jobs))(*[LimbJob(d, o, p, ln) for (d, o, p, ln) in jobs])
         _check(lib().shg_batch_limbs(self.h, arr, len(jobs), int(op)))
+
+    def batch_plan(self, jobs, variant=NTT_AUTO):
+        return BatchPlan(self, jobs, variant)
+
+    def graph_begin(self):
+        _check(lib().shg_graph_begin(self.h))
+
+    def graph_end(self):
+        g = VP()
+        _check(lib().shg_graph_end(self.h, ctypes.byref(g)))
+        return Graph(self, g)
 
     def profile(self, enable=True):
         _check(lib().shg_profile_enable(self.h, int(bool(enable))))
@@ -276,3 +296,50 @@ class Context:
         c = ctypes.c_int()
         _check(lib().shg_autotune_ntt(self.h, int(log_n), int(limbs), ctypes.byref(c)))
         return c.value
+
+
+class BatchPlan:
+    """A device-resident job table for shg_batch_run."""
+
+    def __init__(self, ctx, jobs, variant=NTT_AUTO):
+        self.ctx = ctx
+        arr = (LimbJob * max(len(jobs), 1))(*[LimbJob(d, o, p, ln) for (d, o, p, ln) in jobs])
+        self.h = VP()
+        _check(lib().shg_batch_plan_create(ctx.h, arr, len(jobs), int(variant),
+                                           ctypes.byref(self.h)))
+
+    def run(self, op):
+        _check(lib().shg_batch_run(self.ctx.h, self.h, int(op)))
+
+    def close(self):
+        if self.h:
+            lib().shg_batch_plan_destroy(self.ctx.h, self.h)
+            self.h = VP()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Graph:
+    """An instantiated HIP graph captured from the context's stream."""
+
+    def __init__(self, ctx, h):
+        self.ctx = ctx
+        self.h = h
+
+    def launch(self):
+        _check(lib().shg_graph_launch(self.ctx.h, self.h))
+
+    def close(self):
+        if self.h:
+            lib().shg_graph_destroy(self.ctx.h, self.h)
+            self.h = VP()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
