@@ -200,8 +200,24 @@ typedef struct shg_batch_plan shg_batch_plan;
 int shg_batch_plan_create(shg_context* ctx, const shg_limb_job* host_jobs,
                           int num_jobs, int variant, shg_batch_plan** out);
 int shg_batch_plan_destroy(shg_context* ctx, shg_batch_plan* plan);
+/* One polynomial of a batched modulus switch: RnsPolynomial::ModReduceLsb
+ * (rns/rns_polynomial.cc:262-344) in place, as shg_mod_reduce_lsb does for
+ * one shape; num_limbs <= 16. */
+typedef struct shg_poly_job {
+  uint64_t* data;              /* num_limbs limbs of n words, 16-byte aligned */
+  const int* primes;           /* host array of num_limbs prime handles */
+  const uint64_t* host_ql_inv; /* num_limbs - 1 Montgomery reps of q_l^-1 mod q_i */
+  int num_limbs;
+  int log_n;
+} shg_poly_job;
+#define SHG_BATCH_MOD_REDUCE 3
+int shg_batch_mod_reduce_plan_create(shg_context* ctx, const shg_poly_job* host_jobs,
+                                     int num_jobs, uint64_t t, int is_ntt, int variant,
+                                     shg_batch_plan** out);
 /* One kernel launch on the context's stream, nothing else: may be captured
- * in a graph (shg_graph_begin). */
+ * in a graph (shg_graph_begin).  A limb plan takes SHG_BATCH_NTT_FWD,
+ * SHG_BATCH_NTT_INV or SHG_BATCH_MUL, a modulus-switch plan
+ * SHG_BATCH_MOD_REDUCE. */
 int shg_batch_run(shg_context* ctx, const shg_batch_plan* plan, int op);
 /* Plan, run and wait in one call. */
 int shg_batch_limbs(shg_context* ctx, const shg_limb_job* host_jobs,

@@ -813,12 +813,14 @@ int shg_profile_query(shg_context* ctx, const char* kernel, double* total_ms, lo
 }  // extern "C"
 
 struct shg_batch_plan {
-  shg::LimbJobDev* d_jobs = nullptr;
+  void* d_jobs = nullptr;  // LimbJobDev[] or PolyJobDev[]
+  bool modreduce = false;
   int num_jobs = 0;
   int variant = SHG_NTT_F64;
   int grid = 0;
   size_t shmem = 0;
-  bool has_other = true;  // every job has a second operand (multiply allowed)
+  bool has_other = true;     // every job has a second operand (multiply allowed)
+  uint64_t* scratch = nullptr;  // modulus switch: 2^16 words per workgroup
 };
 
 struct shg_graph {
@@ -876,7 +878,7 @@ int shg_batch_plan_create(shg_context* ctx, const shg_limb_job* host_jobs, int n
   plan->has_other = has_other;
   plan->shmem = shg::batch_shmem_bytes(max_log_n);
   if (num_jobs > 0) {
-    const void* k = shg::batch_kernel_ptr(v);
+    const void* k = shg::batch_kernel_ptr(v, false);
     if (plan->shmem > 65536)
       shg::set_max_dynamic_shmem(k, plan->shmem);
     int cus = 0, per_cu = 0;
@@ -901,25 +903,138 @@ int shg_batch_plan_create(shg_context* ctx, const shg_limb_job* host_jobs, int n
   return SHG_OK;
 }
 
+int shg_batch_mod_reduce_plan_create(shg_context* ctx, const shg_poly_job* host_jobs,
+                                     int num_jobs, uint64_t t, int is_ntt, int variant,
+                                     shg_batch_plan** out) {
+  if (!ctx || !out) return fail(SHG_E_INVALID_ARGUMENT, "null argument.");
+  if (num_jobs < 0 || (num_jobs > 0 && !host_jobs))
+    return fail(SHG_E_INVALID_ARGUMENT, "jobs must be a non-negative count with a job array.");
+  if (t == 0) return fail(SHG_E_INVALID_ARGUMENT, "plaintext modulus must be positive.");
+  if (variant < SHG_NTT_AUTO || variant > SHG_NTT_F64)
+    return fail(SHG_E_INVALID_ARGUMENT, "unknown transform variant %d.", variant);
+  std::vector<shg::PolyJobDev> jobs(num_jobs);
+  bool f64_ok = true;
+  int max_log_n = shg::kBatchMinLogN;
+  {
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    for (int i = 0; i < num_jobs; ++i) {
+      const shg_poly_job& j = host_jobs[i];
+      const int L = j.num_limbs;
+      if (L <= 1)
+        return fail(SHG_E_INVALID_ARGUMENT,
+                    "job %d: ModReduceLsb cannot apply with only one prime modulus.", i);
+      if (L > shg::kBatchMaxPolyLimbs)
+        return fail(SHG_E_INVALID_ARGUMENT, "job %d: at most %d limbs per polynomial.", i,
+                    shg::kBatchMaxPolyLimbs);
+      if (!j.primes || !j.host_ql_inv)
+        return fail(SHG_E_INVALID_ARGUMENT, "job %d: `ql_inv` must be defined with respect to %d moduli.",
+                    i, L - 1);
+      if (j.log_n < shg::kBatchMinLogN || j.log_n > shg::kBatchMaxLogN)
+        return fail(SHG_E_INVALID_ARGUMENT, "job %d: batched jobs need 2^%d <= n <= 2^%d.", i,
+                    shg::kBatchMinLogN, shg::kBatchMaxLogN);
+      if (!aligned16(j.data))
+        return fail(SHG_E_INVALID_ARGUMENT, "job %d: data must be 16-byte aligned device memory.", i);
+      shg::PolyJobDev d{};
+      d.data = j.data;
+      d.log_n = j.log_n;
+      d.num_limbs = L;
+      d.is_ntt = is_ntt ? 1 : 0;
+      for (int l = 0; l < L; ++l) {
+        const int p = j.primes[l];
+        if (p < 0 || p >= (int)ctx->primes.size())
+          return fail(SHG_E_INVALID_ARGUMENT, "job %d: invalid prime handle %d.", i, p);
+        if (ctx->primes[p]->log_n != j.log_n)
+          return fail(SHG_E_INVALID_ARGUMENT, "job %d: prime was created for another ring degree.", i);
+        if (ctx->primes[p]->q >= (1ull << 50)) f64_ok = false;
+        d.prime[l] = p;
+      }
+      // The constants of shg_mod_reduce_lsb (rns/rns_polynomial.cc:262-300).
+      const Params* Pl = ctx->primes[d.prime[L - 1]]->mp.get();
+      const uint64_t r = Pl->modulus % t;
+      const uint64_t k = (Pl->modulus - r) / t;
+      d.k_mod_ql = ModInt::ImportInt(k, Pl).value().GetMontgomeryRepresentation();
+      for (int l = 0; l < L - 1; ++l) {
+        const Params* Pi = ctx->primes[d.prime[l]]->mp.get();
+        d.r_mod_qi[l] = ModInt::ImportInt(r, Pi).value().GetMontgomeryRepresentation();
+        d.t_mod_qi[l] = ModInt::ImportInt(t, Pi).value().GetMontgomeryRepresentation();
+        d.ql_inv[l] = j.host_ql_inv[l];
+      }
+      max_log_n = j.log_n > max_log_n ? j.log_n : max_log_n;
+      jobs[i] = d;
+    }
+  }
+  if (variant == SHG_NTT_F64 && !f64_ok)
+    return fail(SHG_E_INVALID_ARGUMENT, "FP64 transform requires moduli below 2^50.");
+  int v = variant == SHG_NTT_AUTO ? (f64_ok ? SHG_NTT_F64 : SHG_NTT_INT_SHOUP) : variant;
+  // Most expensive polynomials (limbs x ring size) first.
+  std::stable_sort(jobs.begin(), jobs.end(), [](const shg::PolyJobDev& a, const shg::PolyJobDev& b) {
+    return ((long long)a.num_limbs << a.log_n) > ((long long)b.num_limbs << b.log_n);
+  });
+  auto plan = std::make_unique<shg_batch_plan>();
+  plan->modreduce = true;
+  plan->num_jobs = num_jobs;
+  plan->variant = v;
+  plan->has_other = false;
+  plan->shmem = shg::batch_shmem_bytes(max_log_n);
+  if (num_jobs > 0) {
+    const void* k = shg::batch_kernel_ptr(v, true);
+    if (plan->shmem > 65536) shg::set_max_dynamic_shmem(k, plan->shmem);
+    int cus = 0, per_cu = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess ||
+        cus <= 0)
+      cus = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, shg::kBatchThreads, plan->shmem) !=
+            hipSuccess || per_cu <= 0)
+      per_cu = 1;
+    const long long cap = (long long)cus * per_cu;
+    plan->grid = (int)(cap < num_jobs ? cap : num_jobs);
+    hipError_t e = hipMalloc(&plan->d_jobs, sizeof(shg::PolyJobDev) * num_jobs);
+    if (e != hipSuccess) return fail(SHG_E_NO_MEMORY, "job table allocation failed.");
+    e = hipMalloc(&plan->scratch, ((size_t)plan->grid << 16) * 8);
+    if (e != hipSuccess) {
+      (void)hipFree(plan->d_jobs);
+      return fail(SHG_E_NO_MEMORY, "batch scratch allocation failed.");
+    }
+    e = hipMemcpy(plan->d_jobs, jobs.data(), sizeof(shg::PolyJobDev) * num_jobs,
+                  hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      (void)hipFree(plan->d_jobs);
+      (void)hipFree(plan->scratch);
+      return hip_fail(e, "job table upload");
+    }
+  }
+  *out = plan.release();
+  return SHG_OK;
+}
+
 int shg_batch_plan_destroy(shg_context* ctx, shg_batch_plan* plan) {
   if (!ctx || !plan) return fail(SHG_E_INVALID_ARGUMENT, "null argument.");
-  if (plan->d_jobs) {
-    (void)hipStreamSynchronize(ctx->stream);
-    (void)hipFree(plan->d_jobs);
-  }
+  if (plan->d_jobs || plan->scratch) (void)hipStreamSynchronize(ctx->stream);
+  if (plan->d_jobs) (void)hipFree(plan->d_jobs);
+  if (plan->scratch) (void)hipFree(plan->scratch);
   delete plan;
   return SHG_OK;
 }
 
 int shg_batch_run(shg_context* ctx, const shg_batch_plan* plan, int op) {
   if (!ctx || !plan) return fail(SHG_E_INVALID_ARGUMENT, "null argument.");
+  if (plan->modreduce) {
+    if (op != SHG_BATCH_MOD_REDUCE)
+      return fail(SHG_E_INVALID_ARGUMENT, "a modulus-switch plan runs SHG_BATCH_MOD_REDUCE only.");
+    ProfScope prof(ctx, "batch");
+    hipError_t e = shg::launch_batch_modreduce((const shg::PolyJobDev*)plan->d_jobs,
+                                               plan->num_jobs, plan->variant, ctx->d_primes,
+                                               plan->scratch, plan->grid, plan->shmem, ctx->stream);
+    return e == hipSuccess ? SHG_OK : hip_fail(e, "batch launch");
+  }
   if (op != SHG_BATCH_NTT_FWD && op != SHG_BATCH_NTT_INV && op != SHG_BATCH_MUL)
     return fail(SHG_E_INVALID_ARGUMENT, "unknown batch op %d.", op);
   if (op == SHG_BATCH_MUL && !plan->has_other)
     return fail(SHG_E_INVALID_ARGUMENT, "multiply jobs need a second operand.");
   ProfScope prof(ctx, "batch");
-  hipError_t e = shg::launch_batch_limbs(plan->d_jobs, plan->num_jobs, op, plan->variant,
-                                         ctx->d_primes, plan->grid, plan->shmem, ctx->stream);
+  hipError_t e = shg::launch_batch_limbs((const shg::LimbJobDev*)plan->d_jobs, plan->num_jobs, op,
+                                         plan->variant, ctx->d_primes, plan->grid, plan->shmem,
+                                         ctx->stream);
   return e == hipSuccess ? SHG_OK : hip_fail(e, "batch launch");
 }
 

@@ -4,7 +4,8 @@
 // address with its own prime: the forward or inverse negacyclic transform
 // (ForwardNumberTheoreticTransform / InverseNumberTheoreticTransform,
 // shell_encryption/dft_transformations.h:158-201) or the Montgomery product
-// with a second limb (MontgomeryInt::MulInPlace, montgomery.h:318-336).  One
+// with a second limb (MontgomeryInt::MulInPlace, montgomery.h:318-336); or,
+// per polynomial, ModReduceLsb.  One
 // launch of at most one workgroup per resident slot runs every job: the
 // workgroup (1024 threads) loops over jobs blockIdx.x, blockIdx.x +
 // gridDim.x, ... of a table sorted by decreasing size, so the static stride
@@ -24,34 +25,37 @@ namespace {
 constexpr int kT = kBatchThreads;
 
 // Forward transform of the 2^LOGM-word slice h of a limb with LOGM + HB
-// bits (the first HB stages already applied), in place at g.
-template <int LOGM, int HB, class A>
-__device__ __forceinline__ void fwd_slice(gmut<uint64_t> g, int h, const A& ar, typename A::V* lds,
-                                          int t) {
+// bits (the first HB stages already applied).  ld(p) gives the input word
+// at slice position p (already in the policy's representation), st(p, v)
+// takes the canonical output word.
+template <int LOGM, int HB, class A, class Ld, class St>
+__device__ __forceinline__ void fwd_slice_io(int h, const A& ar, typename A::V* lds, int t,
+                                             const Ld& ld, const St& st) {
   constexpr int LOGE = LOGM - 10;
   constexpr int E = 1 << LOGE;
   asm volatile("" : "+v"(t));  // keep this body's addressing local to it
   typename A::V x[E];
 #pragma unroll
-  for (int e = 0; e < E; ++e) x[e] = ar.load(g[e * kT + t]);
+  for (int e = 0; e < E; ++e) x[e] = ld(e * kT + t);
   FwdPasses<LOGM, LOGE, 0, A, true, HB>::run(x, lds, t, h, ar);
 #pragma unroll
-  for (int e = 0; e < E; ++e) g[e * kT + t] = ar.fwd_out(lds[lpad(e * kT + t)]);
+  for (int e = 0; e < E; ++e) st(e * kT + t, ar.fwd_out(lds[lpad(e * kT + t)]));
   // The next job's first exchange overwrites what was just read.
   __syncthreads();
 }
 
 // Inverse transform of slice h (local stages only), outputs times n^-1 when
-// scale is 1, canonical when 0.
-template <int LOGM, int HB, class A>
-__device__ __forceinline__ void inv_slice(gmut<uint64_t> g, int h, const A& ar, typename A::V* lds,
-                                          int scale, int t) {
+// scale is 1, canonical when 0; ld / st as for fwd_slice_io (ld returns the
+// raw residue word).
+template <int LOGM, int HB, class A, class Ld, class St>
+__device__ __forceinline__ void inv_slice_io(int h, const A& ar, typename A::V* lds, int scale,
+                                             int t, const Ld& ld, const St& st) {
   using V = typename A::V;
   constexpr int LOGE = LOGM - 10;
   constexpr int E = 1 << LOGE;
   asm volatile("" : "+v"(t));  // keep this body's addressing local to it
 #pragma unroll
-  for (int e = 0; e < E; ++e) lds[lpad(e * kT + t)] = ar.load(g[e * kT + t]);
+  for (int e = 0; e < E; ++e) lds[lpad(e * kT + t)] = ar.load(ld(e * kT + t));
   __syncthreads();
   V x[E];
   InvPasses<LOGM, LOGE, 0, A, false, LOGM + HB>::run(x, lds, t, h, ar);
@@ -62,36 +66,72 @@ __device__ __forceinline__ void inv_slice(gmut<uint64_t> g, int h, const A& ar, 
   for (int gg = 0; gg < G; ++gg) {
 #pragma unroll
     for (int e = 0; e < (1 << R); ++e)
-      g[(e << I0) | (gg * kT + t)] = ar.inv_out(x[gg * (1 << R) + e], scale);
+      st((e << I0) | (gg * kT + t), ar.inv_out(x[gg * (1 << R) + e], scale));
   }
   // The last pass read its exchange before its own barrier: the next job
   // may write LDS at once.
 }
 
-template <class A>
-__device__ __forceinline__ void fwd_job(gmut<uint64_t> g, int log_n, const A& ar, typename A::V* lds,
-                                        int t) {
+// Forward transform of one limb of 2^log_n words whose input word p is ld(p)
+// (policy representation) and whose output word p goes to st(p, v).  For
+// n = 2^15 stage 0 runs over the whole limb into tmp (2^15 words; may be
+// the limb itself when ld reads it), then the halves run as 2^14-word
+// slices; __syncthreads makes every thread's global writes visible to the
+// whole workgroup.
+template <class A, class Ld, class St>
+__device__ __forceinline__ void fwd_io(int log_n, const A& ar, typename A::V* lds, int t,
+                                       gmut<uint64_t> tmp, const Ld& ld, const St& st) {
   switch (log_n) {
-    case 11: fwd_slice<11, 0>(g, 0, ar, lds, t); break;
-    case 12: fwd_slice<12, 0>(g, 0, ar, lds, t); break;
-    case 13: fwd_slice<13, 0>(g, 0, ar, lds, t); break;
-    case 14: fwd_slice<14, 0>(g, 0, ar, lds, t); break;
+    case 11: fwd_slice_io<11, 0>(0, ar, lds, t, ld, st); break;
+    case 12: fwd_slice_io<12, 0>(0, ar, lds, t, ld, st); break;
+    case 13: fwd_slice_io<13, 0>(0, ar, lds, t, ld, st); break;
+    case 14: fwd_slice_io<14, 0>(0, ar, lds, t, ld, st); break;
     case 15: {
-      // Stage 0 over the whole limb (pairs i, i + 2^14), then the two
-      // halves as 2^14-word slices.  __syncthreads makes the global writes
-      // of every thread visible to the whole workgroup.
       constexpr int H = 1 << 14;
       const auto w = ar.fwd_tw(1);
       for (int i = t; i < H; i += kT) {
-        auto u = ar.load(g[i]);
-        auto v = ar.load(g[i + H]);
+        auto u = ld(i);
+        auto v = ld(i + H);
         ar.ct(u, v, w);
-        g[i] = ar.fwd_out(u);
-        g[i + H] = ar.fwd_out(v);
+        tmp[i] = ar.fwd_out(u);
+        tmp[i + H] = ar.fwd_out(v);
       }
       __syncthreads();
-      fwd_slice<14, 1>(g, 0, ar, lds, t);
-      fwd_slice<14, 1>(g + H, 1, ar, lds, t);
+      fwd_slice_io<14, 1>(0, ar, lds, t, [&](int p) { return ar.load(tmp[p]); }, st);
+      fwd_slice_io<14, 1>(1, ar, lds, t, [&](int p) { return ar.load(tmp[H + p]); },
+                          [&](int p, uint64_t v) { st(H + p, v); });
+      break;
+    }
+    default: break;
+  }
+}
+
+// Inverse transform (times n^-1) of one limb whose input word p is ld(p)
+// (raw residue) and whose output goes to st(p, v).  For n = 2^15 the halves
+// run as slices into tmp, then the last Gentleman-Sande stage (pairs i,
+// i + 2^14, twiddle index n - 2) applies the scaling.
+template <class A, class Ld, class St>
+__device__ __forceinline__ void inv_io(int log_n, const A& ar, typename A::V* lds, int t,
+                                       gmut<uint64_t> tmp, const Ld& ld, const St& st) {
+  switch (log_n) {
+    case 11: inv_slice_io<11, 0>(0, ar, lds, 1, t, ld, st); break;
+    case 12: inv_slice_io<12, 0>(0, ar, lds, 1, t, ld, st); break;
+    case 13: inv_slice_io<13, 0>(0, ar, lds, 1, t, ld, st); break;
+    case 14: inv_slice_io<14, 0>(0, ar, lds, 1, t, ld, st); break;
+    case 15: {
+      constexpr int H = 1 << 14;
+      inv_slice_io<14, 1>(0, ar, lds, 0, t, ld, [&](int p, uint64_t v) { tmp[p] = v; });
+      inv_slice_io<14, 1>(1, ar, lds, 0, t, [&](int p) { return ld(H + p); },
+                          [&](int p, uint64_t v) { tmp[H + p] = v; });
+      __syncthreads();
+      const auto w = ar.inv_tw(2 * H - 2);
+      for (int i = t; i < H; i += kT) {
+        auto u = ar.load(tmp[i]);
+        auto v = ar.load(tmp[i + H]);
+        ar.gs(u, v, w);
+        st(i, ar.inv_out(u, 1));
+        st(i + H, ar.inv_out(v, 1));
+      }
       break;
     }
     default: break;
@@ -99,33 +139,17 @@ __device__ __forceinline__ void fwd_job(gmut<uint64_t> g, int log_n, const A& ar
 }
 
 template <class A>
-__device__ __forceinline__ void inv_job(gmut<uint64_t> g, int log_n, const A& ar, typename A::V* lds,
-                                        int t) {
-  switch (log_n) {
-    case 11: inv_slice<11, 0>(g, 0, ar, lds, 1, t); break;
-    case 12: inv_slice<12, 0>(g, 0, ar, lds, 1, t); break;
-    case 13: inv_slice<13, 0>(g, 0, ar, lds, 1, t); break;
-    case 14: inv_slice<14, 0>(g, 0, ar, lds, 1, t); break;
-    case 15: {
-      // Stages 0..13 on each half, then the last Gentleman-Sande stage
-      // (pairs i, i + 2^14, twiddle index n - 2) with the n^-1 scaling.
-      constexpr int H = 1 << 14;
-      inv_slice<14, 1>(g, 0, ar, lds, 0, t);
-      inv_slice<14, 1>(g + H, 1, ar, lds, 0, t);
-      __syncthreads();
-      const auto w = ar.inv_tw(2 * H - 2);
-      for (int i = t; i < H; i += kT) {
-        auto u = ar.load(g[i]);
-        auto v = ar.load(g[i + H]);
-        ar.gs(u, v, w);
-        g[i] = ar.inv_out(u, 1);
-        g[i + H] = ar.inv_out(v, 1);
-      }
-      // The next job's LDS writes are independent of these global ones.
-      break;
-    }
-    default: break;
-  }
+__device__ __forceinline__ void fwd_job(gmut<uint64_t> g, int log_n, const A& ar,
+                                        typename A::V* lds, int t) {
+  fwd_io(log_n, ar, lds, t, g, [&](int p) { return ar.load(g[p]); },
+         [&](int p, uint64_t v) { g[p] = v; });
+}
+
+template <class A>
+__device__ __forceinline__ void inv_job(gmut<uint64_t> g, int log_n, const A& ar,
+                                        typename A::V* lds, int t) {
+  inv_io(log_n, ar, lds, t, g, [&](int p) { return (uint64_t)g[p]; },
+         [&](int p, uint64_t v) { g[p] = v; });
 }
 
 template <class A>
@@ -164,6 +188,60 @@ __global__ void __launch_bounds__(kT)
   }
 }
 
+// ModReduceLsb (rns/rns_polynomial.cc:262-344) of one polynomial per job,
+// NTT or coefficient form: delta = INTT(k * a_l) (coefficient form, k =
+// (q_l - (q_l mod t)) / t) goes to this workgroup's scratch, then every
+// other limb i becomes (r * a_i + t * NTT(lift_i(delta))) * q_l^-1 with the
+// balanced lift of ConvertModulusBalanced (modulus_conversion.h:30-83) and
+// r = q_l mod t.  Limbs 0..L-2 hold the result; the last limb is left as
+// it was.  Each workgroup owns 2^16 words of scratch.
+template <class A>
+__global__ void __launch_bounds__(kT)
+    batch_modreduce_kernel(const PolyJobDev* __restrict__ jobs, int num_jobs,
+                           const PrimeDev* __restrict__ primes, uint64_t* __restrict__ scratch) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  auto* lds = reinterpret_cast<typename A::V*>(smem);
+  const gmut<uint64_t> s0 = (gmut<uint64_t>)(scratch + ((size_t)blockIdx.x << 16));
+  const gmut<uint64_t> s1 = s0 + (1 << 15);
+  for (int j = blockIdx.x; j < num_jobs; j += gridDim.x) {
+    int t = threadIdx.x;
+    asm volatile("" : "+v"(t));
+    const PolyJobDev& job = jobs[j];
+    const int L = job.num_limbs;
+    const int log_n = job.log_n;
+    const int n = 1 << log_n;
+    const gmut<uint64_t> a = (gmut<uint64_t>)job.data;
+    const PrimeDev& Pl = primes[job.prime[L - 1]];
+    const gmut<uint64_t> al = a + (size_t)(L - 1) * n;
+    const uint64_t k = job.k_mod_ql;
+    if (job.is_ntt) {
+      const A arl(Pl);
+      inv_io(log_n, arl, lds, t, s0, [&](int p) { return mont_mul(al[p], k, Pl); },
+             [&](int p, uint64_t v) { s0[p] = v; });
+    } else {
+      for (int p = t; p < n; p += kT) s0[p] = mont_mul(al[p], k, Pl);
+    }
+    __syncthreads();
+    for (int i = 0; i < L - 1; ++i) {
+      const PrimeDev& Pi = primes[job.prime[i]];
+      const gmut<uint64_t> ai = a + (size_t)i * n;
+      const uint64_t r = job.r_mod_qi[i], tq = job.t_mod_qi[i], qinv = job.ql_inv[i];
+      auto combine = [&](int p, uint64_t d) {
+        ai[p] = mont_mul(mod_add(mont_mul(ai[p], r, Pi), mont_mul(d, tq, Pi), Pi.q), qinv, Pi);
+      };
+      if (job.is_ntt) {
+        const A ari(Pi);
+        fwd_io(log_n, ari, lds, t, s1, [&](int p) { return ari.load(convert_balanced(s0[p], Pl, Pi)); },
+               combine);
+      } else {
+        for (int p = t; p < n; p += kT) combine(p, convert_balanced(s0[p], Pl, Pi));
+      }
+    }
+    // delta (s0) is rewritten by the next job.
+    __syncthreads();
+  }
+}
+
 }  // namespace
 
 size_t batch_shmem_bytes(int max_log_n) {
@@ -171,9 +249,25 @@ size_t batch_shmem_bytes(int max_log_n) {
   return (size_t)((1 << logm) + ((1 << logm) >> 4)) * 8;
 }
 
-const void* batch_kernel_ptr(int variant) {
+const void* batch_kernel_ptr(int variant, bool modreduce) {
+  if (modreduce)
+    return variant == SHG_NTT_F64 ? (const void*)batch_modreduce_kernel<F64Arith>
+                                  : (const void*)batch_modreduce_kernel<IntArith>;
   return variant == SHG_NTT_F64 ? (const void*)batch_limbs_kernel<F64Arith>
                                 : (const void*)batch_limbs_kernel<IntArith>;
+}
+
+hipError_t launch_batch_modreduce(const PolyJobDev* jobs, int num_jobs, int variant,
+                                  const PrimeDev* primes, uint64_t* scratch, int grid,
+                                  size_t shmem, hipStream_t s) {
+  if (num_jobs <= 0) return hipSuccess;
+  if (variant == SHG_NTT_F64)
+    hipLaunchKernelGGL(batch_modreduce_kernel<F64Arith>, dim3(grid), dim3(kT), shmem, s, jobs,
+                       num_jobs, primes, scratch);
+  else
+    hipLaunchKernelGGL(batch_modreduce_kernel<IntArith>, dim3(grid), dim3(kT), shmem, s, jobs,
+                       num_jobs, primes, scratch);
+  return hipGetLastError();
 }
 
 hipError_t launch_batch_limbs(const LimbJobDev* jobs, int num_jobs, int op, int variant,

@@ -117,6 +117,20 @@ __device__ __forceinline__ uint64_t mod_sub(uint64_t a, uint64_t b, uint64_t q) 
   return csub(csub(a + (q - b), q), q);
 }
 
+// ConvertModulusBalanced (modulus_conversion.h:30-83) from q_l to each q_i.
+__device__ __forceinline__ uint64_t convert_balanced(uint64_t rep_q, const PrimeDev& Pq,
+                                                     const PrimeDev& Pp) {
+  if (Pq.q == Pp.q) return rep_q;
+  const uint64_t x = mont_export(rep_q, Pq);
+  uint64_t xp = mont_import(x, Pp);
+  if (x > (Pq.q >> 1)) {
+    const uint64_t diff = Pq.q > Pp.q ? Pq.q - Pp.q : Pp.q - Pq.q;
+    const uint64_t diff_p = mont_import(diff, Pp);
+    xp = (Pp.q > Pq.q) ? mod_add(xp, diff_p, Pp.q) : mod_sub(xp, diff_p, Pp.q);
+  }
+  return xp;
+}
+
 // ---------------------------------------------------------------------------
 // FP64 exact modular arithmetic (q < 2^50).  Integers are held exactly in
 // doubles; x*w = h + l exactly (h = fl(x*w), l = fma(x, w, -h)), the quotient

@@ -125,8 +125,26 @@ constexpr int kBatchMaxLogN = 15;
 constexpr int kBatchNttFwd = 0;
 constexpr int kBatchNttInv = 1;
 constexpr int kBatchMul = 2;
+constexpr int kBatchModReduce = 3;
+constexpr int kBatchMaxPolyLimbs = 16;
+// One polynomial of the batched modulus switch (ModReduceArgs per job).
+struct PolyJobDev {
+  uint64_t* data;
+  int log_n;
+  int num_limbs;
+  int is_ntt;
+  int prime[kBatchMaxPolyLimbs];
+  uint64_t k_mod_ql;
+  uint64_t r_mod_qi[kBatchMaxPolyLimbs];
+  uint64_t t_mod_qi[kBatchMaxPolyLimbs];
+  uint64_t ql_inv[kBatchMaxPolyLimbs];
+};
 size_t batch_shmem_bytes(int max_log_n);
-const void* batch_kernel_ptr(int variant);
+const void* batch_kernel_ptr(int variant, bool modreduce);
+// scratch: 2^16 words per workgroup of the grid.
+hipError_t launch_batch_modreduce(const PolyJobDev* jobs, int num_jobs, int variant,
+                                  const PrimeDev* primes, uint64_t* scratch, int grid,
+                                  size_t shmem, hipStream_t s);
 hipError_t launch_batch_limbs(const LimbJobDev* jobs, int num_jobs, int op, int variant,
                               const PrimeDev* primes, int grid, size_t shmem, hipStream_t s);
 

@@ -191,20 +191,6 @@ hipError_t launch_modreduce_extract(const uint64_t* polys, uint64_t* delta_l,
   return hipGetLastError();
 }
 
-// ConvertModulusBalanced (modulus_conversion.h:30-83) from q_l to each q_i.
-__device__ __forceinline__ uint64_t convert_balanced(uint64_t rep_q, const PrimeDev& Pq,
-                                                     const PrimeDev& Pp) {
-  if (Pq.q == Pp.q) return rep_q;
-  const uint64_t x = mont_export(rep_q, Pq);
-  uint64_t xp = mont_import(x, Pp);
-  if (x > (Pq.q >> 1)) {
-    const uint64_t diff = Pq.q > Pp.q ? Pq.q - Pp.q : Pp.q - Pq.q;
-    const uint64_t diff_p = mont_import(diff, Pp);
-    xp = (Pp.q > Pq.q) ? mod_add(xp, diff_p, Pp.q) : mod_sub(xp, diff_p, Pp.q);
-  }
-  return xp;
-}
-
 __global__ void modreduce_lift_kernel(const uint64_t* __restrict__ delta_l,
                                       uint64_t* __restrict__ delta,
                                       const PrimeDev* __restrict__ primes, int last_prime,

@@ -17,7 +17,7 @@ INTP = ctypes.POINTER(ctypes.c_int)
 VP = ctypes.c_void_p
 
 OP_ADD, OP_SUB, OP_MUL, OP_NEG, OP_FMA = 0, 1, 2, 3, 4
-BATCH_NTT_FWD, BATCH_NTT_INV, BATCH_MUL = 0, 1, 2
+BATCH_NTT_FWD, BATCH_NTT_INV, BATCH_MUL, BATCH_MOD_REDUCE = 0, 1, 2, 3
 NTT_AUTO, NTT_INT, NTT_F64 = 0, 1, 2
 
 _lib = None
@@ -32,6 +32,12 @@ class ShgError(RuntimeError):
 
 class LimbJob(ctypes.Structure):
     _fields_ = [("data", VP), ("other", VP), ("prime", ctypes.c_int), ("log_n", ctypes.c_int)]
+
+
+class PolyJob(ctypes.Structure):
+    _fields_ = [("data", VP), ("primes", ctypes.POINTER(ctypes.c_int)),
+                ("host_ql_inv", ctypes.POINTER(ctypes.c_uint64)), ("num_limbs", ctypes.c_int),
+                ("log_n", ctypes.c_int)]
 
 
 def lib():
@@ -69,6 +75,8 @@ def lib():
         "shg_batch_limbs": [VP, ctypes.POINTER(LimbJob), i, i],
         "shg_batch_plan_create": [VP, ctypes.POINTER(LimbJob), i, i, ctypes.POINTER(VP)],
         "shg_batch_plan_destroy": [VP, VP],
+        "shg_batch_mod_reduce_plan_create": [VP, ctypes.POINTER(PolyJob), i, ctypes.c_uint64, i, i,
+                                             ctypes.POINTER(VP)],
         "shg_batch_run": [VP, VP, i],
         "shg_graph_begin": [VP],
         "shg_graph_end": [VP, ctypes.POINTER(VP)],
@@ -272,6 +280,10 @@ class Context:
     def batch_plan(self, jobs, variant=NTT_AUTO):
         return BatchPlan(self, jobs, variant)
 
+    def mod_reduce_plan(self, jobs, t, is_ntt, variant=NTT_AUTO):
+        """jobs: (data_ptr, prime list, ql_inv list, log_n) per polynomial."""
+        return BatchPlan(self, None, variant, mod_reduce=(jobs, t, is_ntt))
+
     def graph_begin(self):
         _check(lib().shg_graph_begin(self.h))
 
@@ -301,12 +313,25 @@ class Context:
 class BatchPlan:
     """A device-resident job table for shg_batch_run."""
 
-    def __init__(self, ctx, jobs, variant=NTT_AUTO):
+    def __init__(self, ctx, jobs, variant=NTT_AUTO, mod_reduce=None):
         self.ctx = ctx
-        arr = (LimbJob * max(len(jobs), 1))(*[LimbJob(d, o, p, ln) for (d, o, p, ln) in jobs])
         self.h = VP()
-        _check(lib().shg_batch_plan_create(ctx.h, arr, len(jobs), int(variant),
-                                           ctypes.byref(self.h)))
+        if mod_reduce is None:
+            arr = (LimbJob * max(len(jobs), 1))(*[LimbJob(d, o, p, ln) for (d, o, p, ln) in jobs])
+            _check(lib().shg_batch_plan_create(ctx.h, arr, len(jobs), int(variant),
+                                               ctypes.byref(self.h)))
+            return
+        pjobs, t, is_ntt = mod_reduce
+        keep = []
+        arr = (PolyJob * max(len(pjobs), 1))()
+        for k, (d, primes, ql_inv, ln) in enumerate(pjobs):
+            pa = (ctypes.c_int * len(primes))(*primes)
+            qa = (ctypes.c_uint64 * max(len(ql_inv), 1))(*[int(v) for v in ql_inv])
+            keep += [pa, qa]
+            arr[k] = PolyJob(d, pa, qa, len(primes), ln)
+        _check(lib().shg_batch_mod_reduce_plan_create(ctx.h, arr, len(pjobs), int(t),
+                                                      int(bool(is_ntt)), int(variant),
+                                                      ctypes.byref(self.h)))
 
     def run(self, op):
         _check(lib().shg_batch_run(self.ctx.h, self.h, int(op)))

@@ -190,3 +190,33 @@ def test_graph_tensor_relin_modswitch(ctx):
         assert np.array_equal(o0.download()[:, : L - 1], e0), rep
         assert np.array_equal(o1.download()[:, : L - 1], e1), rep
     graph.close()
+
+
+@pytest.mark.parametrize("is_ntt", [True, False])
+def test_batch_mod_reduce_uneven(ctx, is_ntt):
+    """ModReduceLsb of polynomials of uneven ring degree (2^11..2^15) and
+    level (2..6 limbs) in one launch, against the oracle per polynomial."""
+    rng = np.random.default_rng(5 + is_ntt)
+    t = 65537
+    polys, jobs = [], []
+    for p in range(14):
+        log_n = int(rng.integers(11, 16))
+        L = int(rng.integers(2, 7))
+        qs = ntt_primes(log_n, L)
+        rns = oracle.Rns(qs, log_n)
+        x = np.stack([rng.integers(0, q, size=1 << log_n, dtype=np.uint64) for q in qs])
+        d = ctx.array(x)
+        ql_inv = rns.ql_inv()
+        polys.append((x, d, rns, ql_inv))
+        jobs.append((d.ptr, ctx.primes(qs, log_n), list(ql_inv), log_n))
+    plan = ctx.mod_reduce_plan(jobs, t, is_ntt)
+    plan.run(sg.BATCH_MOD_REDUCE)
+    ctx.sync()
+    for x, d, rns, ql_inv in polys:
+        L = x.shape[0]
+        got = d.download()[: L - 1]
+        assert np.array_equal(got, rns.mod_reduce_lsb(x, t, ql_inv, is_ntt))
+        assert np.array_equal(d.download()[L - 1], x[L - 1])  # last limb untouched
+    with pytest.raises(sg.ShgError):
+        plan.run(sg.BATCH_NTT_FWD)
+    plan.close()
