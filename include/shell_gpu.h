@@ -65,6 +65,13 @@ typedef enum shg_ntt_variant {
   SHG_NTT_INT_SHOUP = 1, /* 64-bit Shoup products from 32-bit mul_lo/mul_hi */
   SHG_NTT_F64 = 2        /* exact products from FP64 FMA (q < 2^50)        */
 } shg_ntt_variant;
+/* Launch-configuration flags that may be or-ed into a variant (the
+ * autotuner reports its choice in this form): the alternate register tiling
+ * of 2^12..2^14-word slices (8 words per thread below 2^14, 32 at 2^14,
+ * instead of 16), and loading the next limb while transforming the current
+ * one (FP64 only; ignored for the integer variant). */
+#define SHG_NTT_TILING_ALT 4
+#define SHG_NTT_PREFETCH 8
 
 /* ---- context, memory, streams ------------------------------------------ */
 int shg_create(int device, shg_context** out);
@@ -245,9 +252,16 @@ int shg_profile_reset(shg_context* ctx);
 int shg_profile_query(shg_context* ctx, const char* kernel, double* total_ms,
                       long long* launches);
 
-/* ---- autotuner ---------------------------------------------------------- */
-/* Times the NTT variants for (log_n, count*num_limbs) on the device and
- * records the fastest for SHG_NTT_AUTO.  Returns the chosen variant. */
+/* ---- autotuner ----------------------------------------------------------
+ * No counterpart in the reference.  Times every NTT launch configuration
+ * for `limbs` limbs of 2^log_n words (a forward and an inverse transform,
+ * three times each, after a warm-up) on a modulus below 2^50, and records
+ * the fastest for SHG_NTT_AUTO calls of that ring degree and limb count
+ * (rounded down to a power of two) and for batch plans of that ring degree.
+ * Configurations: arithmetic (FP64 or 64-bit integer Shoup), the register
+ * tiling of 2^12..2^14-word slices (16 words per thread, or 8 / 32), and
+ * next-item prefetch.  *chosen receives the configuration code: the variant
+ * in bits 0-1, 4 = alternate tiling, 8 = prefetch. */
 int shg_autotune_ntt(shg_context* ctx, int log_n, int limbs, int* chosen);
 
 #ifdef __cplusplus

@@ -152,20 +152,63 @@ int check_primes(shg_context* ctx, const int* primes, int num_limbs, int* log_n,
   return SHG_OK;
 }
 
+int rows_bucket(long long rows) {
+  int bucket = 0;
+  while ((1LL << (bucket + 1)) <= rows && bucket < 30) ++bucket;
+  return bucket;
+}
+
+// NTT launch configuration (shg::launch_ntt) for a call: the explicit
+// variant, or for SHG_NTT_AUTO the autotuner's choice for (log_n, rows) if
+// there is one, else FP64 when every modulus allows it.  -1: FP64 asked for
+// with a modulus of 2^50 or more.
 int resolve_variant(shg_context* ctx, int variant, const int* primes, int num_limbs, int log_n,
                     long long rows) {
   bool f64_ok = true;
   for (int i = 0; i < num_limbs; ++i)
     if (ctx->primes[primes[i]]->q >= (1ull << 50)) f64_ok = false;
-  if (variant == SHG_NTT_F64 && !f64_ok) return -1;
-  if (variant == SHG_NTT_INT_SHOUP) return SHG_NTT_INT_SHOUP;
-  if (variant == SHG_NTT_F64) return SHG_NTT_F64;
+  const int base = variant & 3;
+  if (base == SHG_NTT_F64 && !f64_ok) return -1;
+  if (base == SHG_NTT_INT_SHOUP) return variant & ~SHG_NTT_PREFETCH;
+  if (base == SHG_NTT_F64) return variant;
   if (!f64_ok) return SHG_NTT_INT_SHOUP;
-  int bucket = 0;
-  while ((1LL << (bucket + 1)) <= rows && bucket < 30) ++bucket;
-  auto it = ctx->tuned.find({log_n, bucket});
+  auto it = ctx->tuned.find({log_n, rows_bucket(rows)});
   if (it != ctx->tuned.end()) return it->second;
   return SHG_NTT_F64;
+}
+
+// Deterministic Miller-Rabin for 64-bit integers (these bases decide every
+// n < 3.3 * 10^24).
+bool is_prime_u64(uint64_t n) {
+  if (n < 2) return false;
+  for (uint64_t p : {2ull, 3ull, 5ull, 7ull, 11ull, 13ull, 17ull, 19ull, 23ull, 29ull, 31ull, 37ull})
+    if (n % p == 0) return n == p;
+  uint64_t d = n - 1;
+  int r = 0;
+  while ((d & 1) == 0) {
+    d >>= 1;
+    ++r;
+  }
+  auto mulmod = [n](uint64_t a, uint64_t b) { return (uint64_t)((unsigned __int128)a * b % n); };
+  for (uint64_t a : {2ull, 3ull, 5ull, 7ull, 11ull, 13ull, 17ull, 19ull, 23ull, 29ull, 31ull, 37ull}) {
+    uint64_t x = 1, base = a % n, e = d;
+    while (e) {
+      if (e & 1) x = mulmod(x, base);
+      base = mulmod(base, base);
+      e >>= 1;
+    }
+    if (x == 1 || x == n - 1) continue;
+    bool composite = true;
+    for (int i = 1; i < r; ++i) {
+      x = mulmod(x, x);
+      if (x == n - 1) {
+        composite = false;
+        break;
+      }
+    }
+    if (composite) return false;
+  }
+  return true;
 }
 
 bool aligned_ptr(const void* p) { return p != nullptr && ((uintptr_t)p % 8) == 0; }
@@ -419,6 +462,8 @@ static int ntt_common(shg_context* ctx, bool inverse, uint64_t* polys, int count
   if (count == 0) return SHG_OK;
   if (!aligned16(polys))
     return fail(SHG_E_INVALID_ARGUMENT, "polynomial data must be a non-null 16-byte aligned pointer.");
+  if (variant < 0 || (variant & 3) == 3 || (variant & ~15) != 0)
+    return fail(SHG_E_INVALID_ARGUMENT, "unknown transform variant %d.", variant);
   const long long rows = (long long)count * num_limbs;
   if ((rows << (log_n > shg::kMaxLogM ? log_n - shg::kMaxLogM : 0)) > 0x7fffffffLL)
     return fail(SHG_E_INVALID_ARGUMENT, "batch too large for one launch.");
@@ -650,7 +695,7 @@ static int keyswitch_common(shg_context* ctx, const uint64_t* comps, int comps_p
       ka.out_stride = poly_words;
       ka.accumulate = i > 0;
       ProfScope prof(ctx, "keyswitch");
-      e = shg::launch_keyswitch(ka, gd, v, ctx->d_primes, lp, count, num_limbs, log_n, s);
+      e = shg::launch_keyswitch(ka, gd, v & 3, ctx->d_primes, lp, count, num_limbs, log_n, s);
     } else {
       // Unfused form for ring degrees outside the fused kernel's range:
       // materialise the digits, transform them, and multiply-accumulate.
@@ -865,7 +910,7 @@ int shg_batch_plan_create(shg_context* ctx, const shg_limb_job* host_jobs, int n
   if (v == SHG_NTT_AUTO) {
     v = f64_ok ? SHG_NTT_F64 : SHG_NTT_INT_SHOUP;
     auto it = ctx->tuned.find({max_log_n, -1});
-    if (f64_ok && it != ctx->tuned.end()) v = it->second;
+    if (f64_ok && it != ctx->tuned.end()) v = it->second & 3;
   }
   // Largest rings first so the static stride balances the workgroups;
   // then by prime for twiddle locality.
@@ -1089,7 +1134,81 @@ int shg_graph_destroy(shg_context* ctx, shg_graph* graph) {
 }
 
 int shg_autotune_ntt(shg_context* ctx, int log_n, int limbs, int* chosen) {
-  return fail(SHG_E_UNSUPPORTED, "autotuning is not available in this build.");
+  if (!ctx || !chosen) return fail(SHG_E_INVALID_ARGUMENT, "null argument.");
+  if (log_n < 1 || log_n > 15) return fail(SHG_E_INVALID_ARGUMENT, "log_n must be in [1, 15].");
+  if (limbs <= 0) return fail(SHG_E_INVALID_ARGUMENT, "limbs must be positive.");
+  // A modulus of the shape the configurations serve: the largest prime
+  // below 2^50 with q = 1 mod 2n.
+  const uint64_t two_n = 2ull << log_n;
+  uint64_t q = ((1ull << 50) - 1) / two_n * two_n + 1;
+  while (q > two_n && !is_prime_u64(q)) q -= two_n;
+  int pid = -1;
+  int rc = shg_prime_create(ctx, q, log_n, &pid);
+  if (rc) return rc;
+  const size_t n = (size_t)1 << log_n;
+  const size_t words = (size_t)limbs * n;
+  std::vector<uint64_t> host(words);
+  uint64_t st = 0x9e3779b97f4a7c15ull;
+  for (auto& w : host) {
+    st += 0x9e3779b97f4a7c15ull;
+    uint64_t z = st;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    w = (z ^ (z >> 31)) % q;
+  }
+  void* buf = nullptr;
+  if (hipMalloc(&buf, words * 8) != hipSuccess)
+    return fail(SHG_E_NO_MEMORY, "autotune buffer of %zu bytes failed.", words * 8);
+  hipError_t e = hipMemcpy(buf, host.data(), words * 8, hipMemcpyHostToDevice);
+  LimbPrimes lp{};
+  lp.id[0] = pid;
+  const int logm = log_n < shg::kMaxLogM ? log_n : shg::kMaxLogM;
+  std::vector<int> cands = {SHG_NTT_F64, SHG_NTT_F64 | shg::kNttCfgPrefetch, SHG_NTT_INT_SHOUP};
+  if (logm >= 12) {
+    cands.push_back(SHG_NTT_F64 | shg::kNttCfgAlt);
+    cands.push_back(SHG_NTT_F64 | shg::kNttCfgAlt | shg::kNttCfgPrefetch);
+    cands.push_back(SHG_NTT_INT_SHOUP | shg::kNttCfgAlt);
+  }
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  if (e == hipSuccess) e = hipEventCreate(&ev0);
+  if (e == hipSuccess) e = hipEventCreate(&ev1);
+  int best = SHG_NTT_F64;
+  float best_ms = 1e30f;
+  hipStream_t s = ctx->stream;
+  for (int cfg : cands) {
+    if (e != hipSuccess) break;
+    auto run = [&]() {
+      hipError_t r = shg::launch_ntt(false, cfg, (uint64_t*)buf, ctx->d_primes, lp, limbs, 1,
+                                     log_n, shg::kMaxLogM, s);
+      if (r == hipSuccess)
+        r = shg::launch_ntt(true, cfg, (uint64_t*)buf, ctx->d_primes, lp, limbs, 1, log_n,
+                            shg::kMaxLogM, s);
+      return r;
+    };
+    e = run();  // warm-up (and first-use setup)
+    if (e == hipSuccess) e = hipEventRecord(ev0, s);
+    for (int r = 0; r < 3 && e == hipSuccess; ++r) e = run();
+    if (e == hipSuccess) e = hipEventRecord(ev1, s);
+    if (e == hipSuccess) e = hipEventSynchronize(ev1);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, ev0, ev1);
+    if (e == hipSuccess && ms < best_ms) {
+      best_ms = ms;
+      best = cfg;
+    }
+  }
+  if (ev0) (void)hipEventDestroy(ev0);
+  if (ev1) (void)hipEventDestroy(ev1);
+  (void)hipStreamSynchronize(s);
+  (void)hipFree(buf);
+  if (e != hipSuccess) return hip_fail(e, "autotune");
+  {
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    ctx->tuned[{log_n, rows_bucket(limbs)}] = best;
+    ctx->tuned[{log_n, -1}] = best;
+  }
+  *chosen = best;
+  return SHG_OK;
 }
 
 }  // extern "C"

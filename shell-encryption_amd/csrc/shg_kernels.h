@@ -15,7 +15,13 @@ constexpr int kMaxLogM = 14;
 
 // inv_mode (inverse only): 1 = times n^-1 (InverseNumberTheoreticTransform),
 // 2 = times n^-1 * 2^-64 (the inverse transform followed by ExportInt).
-hipError_t launch_ntt(bool inverse, int variant, uint64_t* data, const PrimeDev* primes,
+// NTT launch configuration: the arithmetic variant in bits 0-1
+// (SHG_NTT_INT_SHOUP or SHG_NTT_F64), the alternate register tiling of
+// 2^12..2^14-word slices in kNttCfgAlt, next-item prefetch (FP64 only) in
+// kNttCfgPrefetch.  A bare variant is a valid configuration.
+constexpr int kNttCfgAlt = 4;
+constexpr int kNttCfgPrefetch = 8;
+hipError_t launch_ntt(bool inverse, int cfg, uint64_t* data, const PrimeDev* primes,
                       const LimbPrimes& lp, int rows, int num_limbs, int logn, int max_logm,
                       hipStream_t s, int inv_mode = 1);
 hipError_t launch_scale_ninv(uint64_t* data, const PrimeDev* primes, const LimbPrimes& lp,

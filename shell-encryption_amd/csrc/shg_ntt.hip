@@ -211,14 +211,9 @@ void set_max_dynamic_shmem(const void* k, size_t bytes) {
     cur = bytes;
 }
 
-// Prefetching the next item costs 2E VGPRs.  At E = 16 (1024 threads, 128
-// VGPRs) that spills and measured slower (fwd 0.80 vs 0.63 ms per 8192 limbs
-// of 2^14), so it is off by default; the integer policy never has room.
-#ifndef SHG_NTT_PREFETCH
-#define SHG_NTT_PREFETCH 0
-#endif
-template <class A>
-constexpr bool kPrefetch = SHG_NTT_PREFETCH && std::is_same<A, F64Arith>::value;
+// Prefetching the next item (kNttCfgPrefetch) costs 2E VGPRs.  At E = 16
+// (1024 threads, 128 VGPRs) it spills and measured slower (fwd 0.80 vs 0.63
+// ms per 8192 limbs of 2^14); the autotuner may still pick it for a shape.
 
 // Workgroups resident at once on the current device for kernel k (at least
 // one per CU), capped by the item count and rounded down to a multiple of
@@ -253,7 +248,7 @@ static int persistent_grid(const void* k, int threads, size_t shmem, int items, 
   return (int)cap;
 }
 
-template <int LOGM, int LOGE, int HB, class A>
+template <int LOGM, int LOGE, int HB, class A, bool PF>
 static hipError_t launch_fused_hb(bool inverse, uint64_t* data, const PrimeDev* primes,
                                   const LimbPrimes& lp, int rows, int num_limbs, int scale,
                                   hipStream_t stream) {
@@ -262,13 +257,13 @@ static hipError_t launch_fused_hb(bool inverse, uint64_t* data, const PrimeDev* 
   const int items = rows << HB;
   const int period = num_limbs << HB;
   if (!inverse) {
-    auto k = ntt_fwd_kernel<LOGM, LOGE, HB, A, kPrefetch<A>>;
+    auto k = ntt_fwd_kernel<LOGM, LOGE, HB, A, PF>;
     if (shmem > 65536)
       set_max_dynamic_shmem((const void*)k, shmem);
     const int grid = persistent_grid((const void*)k, T, shmem, items, period);
     hipLaunchKernelGGL(k, dim3(grid), dim3(T), shmem, stream, data, primes, lp, num_limbs, items);
   } else {
-    auto k = ntt_inv_kernel<LOGM, LOGE, HB, A, kPrefetch<A>>;
+    auto k = ntt_inv_kernel<LOGM, LOGE, HB, A, PF>;
     if (shmem > 65536)
       set_max_dynamic_shmem((const void*)k, shmem);
     const int grid = persistent_grid((const void*)k, T, shmem, items, period);
@@ -278,17 +273,17 @@ static hipError_t launch_fused_hb(bool inverse, uint64_t* data, const PrimeDev* 
   return hipGetLastError();
 }
 
-template <int LOGM, int LOGE, class A>
+template <int LOGM, int LOGE, class A, bool PF>
 static hipError_t launch_fused(bool inverse, uint64_t* data, const PrimeDev* primes,
                                const LimbPrimes& lp, int rows, int num_limbs, int logn,
                                int scale, hipStream_t stream) {
   if (logn == LOGM)
-    return launch_fused_hb<LOGM, LOGE, 0, A>(inverse, data, primes, lp, rows, num_limbs, scale,
-                                             stream);
+    return launch_fused_hb<LOGM, LOGE, 0, A, PF>(inverse, data, primes, lp, rows, num_limbs,
+                                                 scale, stream);
   if constexpr (LOGM == kMaxLogM) {
     if (logn == LOGM + 1)
-      return launch_fused_hb<LOGM, LOGE, 1, A>(inverse, data, primes, lp, rows, num_limbs, scale,
-                                               stream);
+      return launch_fused_hb<LOGM, LOGE, 1, A, PF>(inverse, data, primes, lp, rows, num_limbs,
+                                                   scale, stream);
   }
   return hipErrorInvalidValue;
 }
@@ -309,31 +304,36 @@ static hipError_t launch_stream(bool inverse, uint64_t* data, const PrimeDev* pr
   return hipGetLastError();
 }
 
-template <class A>
-static hipError_t dispatch_fused(int logm, bool inverse, uint64_t* data, const PrimeDev* primes,
-                                 const LimbPrimes& lp, int rows, int num_limbs, int logn, int scale,
-                                 hipStream_t s) {
+// Slices of 2^12..2^14 words have two register tilings: the default (E =
+// 16 words per thread) and the alternate one selected by kNttCfgAlt (E = 8
+// below 2^14, E = 32 at 2^14).
+template <class A, bool PF>
+static hipError_t dispatch_fused(int logm, bool alt, bool inverse, uint64_t* data,
+                                 const PrimeDev* primes, const LimbPrimes& lp, int rows,
+                                 int num_limbs, int logn, int scale, hipStream_t s) {
+#define SHG_FUSED(M, E) launch_fused<M, E, A, PF>(inverse, data, primes, lp, rows, num_limbs, logn, scale, s)
   switch (logm) {
-    case 1: return launch_fused<1, 1, A>(inverse, data, primes, lp, rows, num_limbs, logn, scale, s);
-    case 2: return launch_fused<2, 2, A>(inverse, data, primes, lp, rows, num_limbs, logn, scale, s);
-    case 3: return launch_fused<3, 3, A>(inverse, data, primes, lp, rows, num_limbs, logn, scale, s);
-    case 4: return launch_fused<4, 4, A>(inverse, data, primes, lp, rows, num_limbs, logn, scale, s);
-    case 5: return launch_fused<5, 4, A>(inverse, data, primes, lp, rows, num_limbs, logn, scale, s);
-    case 6: return launch_fused<6, 4, A>(inverse, data, primes, lp, rows, num_limbs, logn, scale, s);
-    case 7: return launch_fused<7, 4, A>(inverse, data, primes, lp, rows, num_limbs, logn, scale, s);
-    case 8: return launch_fused<8, 4, A>(inverse, data, primes, lp, rows, num_limbs, logn, scale, s);
-    case 9: return launch_fused<9, 4, A>(inverse, data, primes, lp, rows, num_limbs, logn, scale, s);
-    case 10: return launch_fused<10, 4, A>(inverse, data, primes, lp, rows, num_limbs, logn, scale, s);
-    case 11: return launch_fused<11, 4, A>(inverse, data, primes, lp, rows, num_limbs, logn, scale, s);
-    case 12: return launch_fused<12, 4, A>(inverse, data, primes, lp, rows, num_limbs, logn, scale, s);
-    case 13: return launch_fused<13, 4, A>(inverse, data, primes, lp, rows, num_limbs, logn, scale, s);
-    case 14: return launch_fused<14, SHG_LOGE_14, A>(inverse, data, primes, lp, rows, num_limbs, logn, scale, s);
+    case 1: return SHG_FUSED(1, 1);
+    case 2: return SHG_FUSED(2, 2);
+    case 3: return SHG_FUSED(3, 3);
+    case 4: return SHG_FUSED(4, 4);
+    case 5: return SHG_FUSED(5, 4);
+    case 6: return SHG_FUSED(6, 4);
+    case 7: return SHG_FUSED(7, 4);
+    case 8: return SHG_FUSED(8, 4);
+    case 9: return SHG_FUSED(9, 4);
+    case 10: return SHG_FUSED(10, 4);
+    case 11: return SHG_FUSED(11, 4);
+    case 12: return alt ? SHG_FUSED(12, 3) : SHG_FUSED(12, 4);
+    case 13: return alt ? SHG_FUSED(13, 3) : SHG_FUSED(13, 4);
+    case 14: return alt ? SHG_FUSED(14, 5) : SHG_FUSED(14, SHG_LOGE_14);
     default: return hipErrorInvalidValue;
   }
+#undef SHG_FUSED
 }
 
-template <class A>
-static hipError_t run_ntt(bool inverse, uint64_t* data, const PrimeDev* primes,
+template <class A, bool PF>
+static hipError_t run_ntt(bool inverse, bool alt, uint64_t* data, const PrimeDev* primes,
                           const LimbPrimes& lp, int rows, int num_limbs, int logn, int max_logm,
                           int inv_mode, hipStream_t s) {
   const int logm = logn < max_logm ? logn : max_logm;
@@ -341,10 +341,11 @@ static hipError_t run_ntt(bool inverse, uint64_t* data, const PrimeDev* primes,
   if (!inverse) {
     for (int st = 0; st < logn - logm && e == hipSuccess; ++st)
       e = launch_stream<A>(false, data, primes, lp, rows, num_limbs, logn, st, s);
-    if (e == hipSuccess) e = dispatch_fused<A>(logm, false, data, primes, lp, rows, num_limbs, logn, 1, s);
+    if (e == hipSuccess)
+      e = dispatch_fused<A, PF>(logm, alt, false, data, primes, lp, rows, num_limbs, logn, 1, s);
   } else {
-    e = dispatch_fused<A>(logm, true, data, primes, lp, rows, num_limbs, logn,
-                          logn == logm ? inv_mode : 0, s);
+    e = dispatch_fused<A, PF>(logm, alt, true, data, primes, lp, rows, num_limbs, logn,
+                              logn == logm ? inv_mode : 0, s);
     for (int st = logm; st < logn && e == hipSuccess; ++st) {
       e = launch_stream<A>(true, data, primes, lp, rows, num_limbs, logn, st, s);
     }
@@ -354,12 +355,19 @@ static hipError_t run_ntt(bool inverse, uint64_t* data, const PrimeDev* primes,
   return e;
 }
 
-hipError_t launch_ntt(bool inverse, int variant, uint64_t* data, const PrimeDev* primes,
+hipError_t launch_ntt(bool inverse, int cfg, uint64_t* data, const PrimeDev* primes,
                       const LimbPrimes& lp, int rows, int num_limbs, int logn, int max_logm,
                       hipStream_t s, int inv_mode) {
-  if (variant == SHG_NTT_F64)
-    return run_ntt<F64Arith>(inverse, data, primes, lp, rows, num_limbs, logn, max_logm, inv_mode, s);
-  return run_ntt<IntArith>(inverse, data, primes, lp, rows, num_limbs, logn, max_logm, inv_mode, s);
+  const bool alt = (cfg & kNttCfgAlt) != 0;
+  if ((cfg & 3) == SHG_NTT_F64) {
+    if (cfg & kNttCfgPrefetch)
+      return run_ntt<F64Arith, true>(inverse, alt, data, primes, lp, rows, num_limbs, logn,
+                                     max_logm, inv_mode, s);
+    return run_ntt<F64Arith, false>(inverse, alt, data, primes, lp, rows, num_limbs, logn,
+                                    max_logm, inv_mode, s);
+  }
+  return run_ntt<IntArith, false>(inverse, alt, data, primes, lp, rows, num_limbs, logn, max_logm,
+                                  inv_mode, s);
 }
 
 }  // namespace shg

@@ -341,3 +341,43 @@ def test_ntt_reference_moduli(ctx, entry):
         assert np.array_equal(d.download(), exp)
         ctx.ntt_inv(d, 3, pids, variant)
         assert np.array_equal(d.download(), x)
+
+
+CONFIGS = [sg.NTT_F64 | sg.NTT_TILING_ALT, sg.NTT_F64 | sg.NTT_PREFETCH,
+           sg.NTT_F64 | sg.NTT_TILING_ALT | sg.NTT_PREFETCH, sg.NTT_INT | sg.NTT_TILING_ALT]
+
+
+@pytest.mark.parametrize("cfg", CONFIGS)
+@pytest.mark.parametrize("log_n,L,count", [(12, 2, 3), (13, 3, 2), (14, 4, 3), (15, 2, 2)])
+def test_ntt_launch_configs(ctx, cfg, log_n, L, count):
+    """Every register tiling and the prefetching loop, with more limbs than
+    resident workgroups at 2^14 so the persistent loop iterates."""
+    if log_n == 14:
+        count = 160  # 640 limbs > 256 CUs at one workgroup each
+    qs, rns = _rns(log_n, L)
+    pids = ctx.primes(qs, log_n)
+    x = rns.uniform(500 + log_n + cfg, count)
+    d = ctx.array(x)
+    ctx.ntt_fwd(d, count, pids, cfg)
+    got = d.download()
+    for c in sorted({0, 1, count - 1}):
+        assert np.array_equal(got[c], rns.ntt_fwd(x[c])), c
+    ctx.ntt_inv(d, count, pids, cfg)
+    assert np.array_equal(d.download(), x)
+
+
+def test_autotune_then_auto(ctx):
+    qs, rns = _rns(13, 4)
+    pids = ctx.primes(qs, 13)
+    chosen = ctx.autotune_ntt(13, 256)
+    assert chosen & 3 in (sg.NTT_INT, sg.NTT_F64) and chosen & ~15 == 0
+    x = rns.uniform(77, 64)  # 256 limbs: the tuned shape
+    d = ctx.array(x)
+    ctx.ntt_fwd(d, 64, pids)
+    got = d.download()
+    for c in (0, 63):
+        assert np.array_equal(got[c], rns.ntt_fwd(x[c]))
+    ctx.ntt_inv(d, 64, pids)
+    assert np.array_equal(d.download(), x)
+    with pytest.raises(sg.ShgError):
+        ctx.ntt_fwd(d, 64, pids, 3)
