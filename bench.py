@@ -96,6 +96,136 @@ def copy_probe():
     return 0
 
 
+UNEVEN_POLYS = 4096
+
+
+def uneven_batch(ctx, rng):
+    """The uneven-batching workload: 4096 polynomials with n uniform in
+    {2^11..2^15} and L uniform in 1..12 (the L largest primes below 2^50 with
+    q = 1 mod 2n), coefficients uniform mod each prime; forward NTT, pointwise
+    multiply and inverse NTT each as one persistent launch over all limbs,
+    the chain captured in a HIP graph, the modulus switch of every polynomial
+    with L >= 2 as one launch; beside the plain form of one call per
+    polynomial and operation."""
+    import shell_gpu as sg
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    print("[bench] uneven batch: generating", file=sys.stderr, flush=True)
+    log_ns = rng.integers(11, 16, size=UNEVEN_POLYS)
+    Ls = rng.integers(1, 13, size=UNEVEN_POLYS)
+    primes = {ln: primes_for(ln, 12) for ln in range(11, 16)}
+    pids = {ln: ctx.primes(primes[ln], ln) for ln in range(11, 16)}
+    sizes = [int(L) << int(ln) for ln, L in zip(log_ns, Ls)]
+    offs = np.concatenate([[0], np.cumsum(sizes)])
+    total = int(offs[-1])
+    host = np.empty(total, dtype=np.uint64)
+    other = np.empty(total, dtype=np.uint64)
+    for ln in range(11, 16):
+        n = 1 << ln
+        for i, q in enumerate(primes[ln]):
+            rows = [int(offs[p]) + i * n for p in range(UNEVEN_POLYS)
+                    if log_ns[p] == ln and Ls[p] > i]
+            if not rows:
+                continue
+            idx = (np.asarray(rows)[:, None] + np.arange(n)[None, :]).ravel()
+            host[idx] = rng.integers(0, q, size=idx.size, dtype=np.uint64)
+            other[idx] = rng.integers(0, q, size=idx.size, dtype=np.uint64)
+    d = ctx.array(host)
+    d_other = ctx.array(other)
+    jobs = []
+    for p in range(UNEVEN_POLYS):
+        ln, L, n = int(log_ns[p]), int(Ls[p]), 1 << int(log_ns[p])
+        for i in range(L):
+            off = 8 * (int(offs[p]) + i * n)
+            jobs.append((d.ptr + off, d_other.ptr + off, pids[ln][i], ln))
+    plan = ctx.batch_plan(jobs)
+    ops = [sg.BATCH_NTT_FWD, sg.BATCH_MUL, sg.BATCH_NTT_INV]
+    for op in ops:
+        plan.run(op)
+    ctx.sync()
+    res = {"polys": UNEVEN_POLYS, "limbs": len(jobs), "words": total}
+    for name, op in (("ntt_fwd", sg.BATCH_NTT_FWD), ("mul", sg.BATCH_MUL),
+                     ("ntt_inv", sg.BATCH_NTT_INV)):
+        ctx.profile(True)
+        ctx.profile_reset()
+        for _ in range(3):
+            plan.run(op)
+        ctx.sync()
+        ms, k = ctx.profile_query("batch")
+        ctx.profile(False)
+        res[name + "_ms"] = round(ms / k, 4)
+    butterflies = sum((L << ln) // 2 * ln for ln, L in zip(log_ns.tolist(), Ls.tolist()))
+    res["limb_ntts_per_s"] = round(len(jobs) / (res["ntt_fwd_ms"] * 1e-3), 1)
+    res["fwd_hbm_frac"] = round(total * 16 / (res["ntt_fwd_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+    res["fwd_butterflies_per_s"] = round(butterflies / (res["ntt_fwd_ms"] * 1e-3), 1)
+    # The chain as one HIP graph, replayed.
+    d.upload(host)
+    ctx.sync()
+    ctx.graph_begin()
+    for op in ops:
+        plan.run(op)
+    graph = ctx.graph_end()
+    graph.launch()
+    ctx.sync()
+    got = d.download()
+    t0 = time.perf_counter()
+    for _ in range(5):
+        graph.launch()
+    ctx.sync()
+    res["graph_chain_ms"] = round((time.perf_counter() - t0) * 1e3 / 5, 4)
+    graph.close()
+    # Bit-exactness of sampled polynomials (the first replay's output).
+    ok = True
+    for p in list(range(0, UNEVEN_POLYS, 512)) + [UNEVEN_POLYS - 1]:
+        ln, L = int(log_ns[p]), int(Ls[p])
+        rns = oracle.Rns(primes[ln][:L], ln)
+        n = 1 << ln
+        x = host[offs[p]:offs[p + 1]].reshape(L, n)
+        y = other[offs[p]:offs[p + 1]].reshape(L, n)
+        e = rns.ntt_inv(rns.pointwise("mul", rns.ntt_fwd(x), y))
+        ok &= bool(np.array_equal(got[offs[p]:offs[p + 1]].reshape(L, n), e))
+    res["bit_exact"] = ok
+    # Plain per-call form: one call per polynomial and operation.
+    d.upload(host)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for p in range(UNEVEN_POLYS):
+        ln, L = int(log_ns[p]), int(Ls[p])
+        v = d.view(int(offs[p]), (1, L, 1 << ln))
+        vo = d_other.view(int(offs[p]), (1, L, 1 << ln))
+        pl = pids[ln][:L]
+        ctx.ntt_fwd(v, 1, pl)
+        ctx.pointwise(sg.OP_MUL, v, vo, None, 1, pl)
+        ctx.ntt_inv(v, 1, pl)
+    ctx.sync()
+    res["plain_chain_ms"] = round((time.perf_counter() - t0) * 1e3, 4)
+    res["graph_vs_plain"] = round(res["plain_chain_ms"] / res["graph_chain_ms"], 2)
+    plan.close()
+    # Modulus switch of every polynomial with at least two limbs, one launch.
+    mjobs = []
+    for p in range(UNEVEN_POLYS):
+        ln, L = int(log_ns[p]), int(Ls[p])
+        if L < 2:
+            continue
+        rns_ql = oracle.Rns(primes[ln][:L], ln).ql_inv()
+        mjobs.append((d.ptr + 8 * int(offs[p]), pids[ln][:L], list(rns_ql), ln))
+    mplan = ctx.mod_reduce_plan(mjobs, 65537, True)
+    mplan.run(sg.BATCH_MOD_REDUCE)
+    ctx.sync()
+    ctx.profile(True)
+    ctx.profile_reset()
+    for _ in range(3):
+        mplan.run(sg.BATCH_MOD_REDUCE)
+    ctx.sync()
+    ms, k = ctx.profile_query("batch")
+    ctx.profile(False)
+    res["modswitch_polys"] = len(mjobs)
+    res["modswitch_ms"] = round(ms / k, 4)
+    mplan.close()
+    del d, d_other
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -275,6 +405,30 @@ def main():
             ctx.mod_reduce_lsb(ms_in, 65537, ql_inv, True, BATCH * 2, pids)
         ctx.sync()
         out["modswitch_ciphertexts_per_s"] = round(3 * BATCH / (time.perf_counter() - t0), 1)
+        del ms_in
+
+        # Autotuned transform configuration for the NTT workload's shape.
+        chosen = ctx.autotune_ntt(LOG_N, NTT_BATCH * L)
+        d_x = ctx.array(uniform(rng, qs, (NTT_BATCH,), n))
+        ctx.ntt_fwd(d_x, NTT_BATCH, pids)
+        ctx.sync()
+        ctx.profile(True)
+        ctx.profile_reset()
+        for _ in range(10):
+            ctx.ntt_fwd(d_x, NTT_BATCH, pids)
+        for _ in range(10):
+            ctx.ntt_inv(d_x, NTT_BATCH, pids)
+        ctx.sync()
+        f_ms, f_n = ctx.profile_query("ntt_fwd")
+        i_ms, i_n = ctx.profile_query("ntt_inv")
+        ctx.profile(False)
+        by = NTT_BATCH * 2 * poly_words * 8
+        out["ntt"]["autotuned"] = {"config": chosen, "fwd_avg_launch_ms": round(f_ms / f_n, 4),
+                                   "inv_avg_launch_ms": round(i_ms / i_n, 4),
+                                   "ntts_per_s": round(NTT_BATCH / (f_ms / f_n * 1e-3), 1),
+                                   "hbm_frac": round(by / (f_ms / f_n * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+        del d_x
+        out["uneven_batch"] = uneven_batch(ctx, rng)
 
     if not args.no_cpu:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)

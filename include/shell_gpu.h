@@ -184,6 +184,42 @@ int shg_lazy_fma(shg_context* ctx, const uint64_t* as, size_t as_stride,
                  const uint64_t* bs, size_t bs_stride, int num_terms,
                  uint64_t* out, int num_limbs, const int* primes);
 
+/* ---- encrypted queries against a device-resident database --------------
+ * Oblivious expansion (ObliviousExpander::ExpandCiphertext,
+ * oblivious_expand.h:173-227, there on the non-RNS stack) on RNS
+ * ciphertexts: host_cts[g] points to query g's buffer of 2^levels
+ * ciphertexts (each c0 then c1, NTT form, num_limbs limbs of n words);
+ * ciphertext 0 holds the query.  Level j (0-based) turns ciphertexts
+ * 0..2^j-1 into 0..2^(j+1)-1: sub = RnsGaloisKey::ApplyTo(Substitute(ct,
+ * n/2^j + 1)), ct[k + 2^j] = (ct[k] - sub) * X^(-2^j), ct[k] += sub, with
+ * the Galois key for power n/2^j + 1 (key_as / key_bs: k polynomials each,
+ * NTT form).  All queries of a call share the level. */
+int shg_oblivious_expand_level(shg_context* ctx, uint64_t* const* host_cts, int count,
+                               int level, const uint64_t* key_as, const uint64_t* key_bs,
+                               const int* host_log_bs, int num_limbs, const int* primes);
+/* All levels 0..levels-1; key_as / key_bs are host arrays of one device
+ * key pointer per level. */
+int shg_oblivious_expand(shg_context* ctx, uint64_t* const* host_cts, int count, int levels,
+                         const uint64_t* const* key_as, const uint64_t* const* key_bs,
+                         const int* host_log_bs, int num_limbs, const int* primes);
+/* Database records are RNS plaintext polynomials in NTT form (num_limbs
+ * limbs of n words each).  The inner product reads them exported
+ * (ExportInt of every word), which shg_pir_db_export writes. */
+int shg_pir_db_export(shg_context* ctx, const uint64_t* db, long long num_records,
+                      uint64_t* out, int num_limbs, const int* primes);
+/* Synthetic records for benchmarks: coefficients uniform in [0, t) from a
+ * seeded SplitMix64 stream, imported mod each limb and transformed to NTT
+ * form. */
+int shg_pir_db_synthesize(shg_context* ctx, uint64_t seed, uint64_t t, long long num_records,
+                          uint64_t* out, int num_limbs, const int* primes);
+/* For each query g: answer_g[c] = sum_{r < 2^levels} expanded_g[r] (.)
+ * record[r * num_columns + c] (AbsorbInPlace then AddInPlace,
+ * rns/rns_ciphertext.h:219-225), c < num_columns; host_out[g] receives
+ * num_columns ciphertexts.  Moduli below 2^50. */
+int shg_pir_inner_product(shg_context* ctx, const uint64_t* const* host_expanded, int count,
+                          int levels, const uint64_t* db_exported, int num_columns,
+                          uint64_t* const* host_out, int num_limbs, const int* primes);
+
 /* ---- batched uneven work (persistent launch) ---------------------------
  * No counterpart in the reference, which transforms one RnsPolynomial limb
  * at a time (rns/rns_polynomial.cc:171-207): these run the per-limb
@@ -254,8 +290,8 @@ int shg_profile_query(shg_context* ctx, const char* kernel, double* total_ms,
 
 /* ---- autotuner ----------------------------------------------------------
  * No counterpart in the reference.  Times every NTT launch configuration
- * for `limbs` limbs of 2^log_n words (a forward and an inverse transform,
- * three times each, after a warm-up) on a modulus below 2^50, and records
+ * for `limbs` limbs of 2^log_n words (forward + inverse pairs, best of
+ * three interleaved rounds after a warm-up) on a modulus below 2^50, and records
  * the fastest for SHG_NTT_AUTO calls of that ring degree and limb count
  * (rounded down to a power of two) and for batch plans of that ring degree.
  * Configurations: arithmetic (FP64 or 64-bit integer Shoup), the register

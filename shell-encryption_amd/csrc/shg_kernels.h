@@ -154,6 +154,25 @@ hipError_t launch_batch_modreduce(const PolyJobDev* jobs, int num_jobs, int vari
 hipError_t launch_batch_limbs(const LimbJobDev* jobs, int num_jobs, int op, int variant,
                               const PrimeDev* primes, int grid, size_t shmem, hipStream_t s);
 
+// Oblivious expansion and the inner product against a database (shg_pir.hip).
+hipError_t launch_expand_substitute(const uint64_t* const* cts, int count, int m, uint64_t* S,
+                                    int power, int logn, int num_limbs, hipStream_t s);
+hipError_t launch_expand_update(uint64_t* const* cts, int count, int m, const uint64_t* sub0,
+                                const uint64_t* sub1, const uint64_t* mono,
+                                const PrimeDev* primes, const LimbPrimes& lp, int logn,
+                                int num_limbs, hipStream_t s);
+int pir_k_split(int count, int C, int K, int logn, int num_limbs);
+hipError_t launch_pir_inner_product(const uint64_t* const* expd, int count, int K,
+                                    const uint64_t* db_x, int C, uint64_t* part, int KS,
+                                    uint64_t* const* out, const PrimeDev* primes,
+                                    const LimbPrimes& lp, int logn, int num_limbs, hipStream_t s);
+hipError_t launch_export_words(const uint64_t* in, uint64_t* out, long long polys,
+                               const PrimeDev* primes, const LimbPrimes& lp, int logn,
+                               int num_limbs, hipStream_t s);
+hipError_t launch_db_synth(uint64_t* out, long long records, uint64_t seed, uint64_t t,
+                           const PrimeDev* primes, const LimbPrimes& lp, int logn, int num_limbs,
+                           hipStream_t s);
+
 }  // namespace shg
 
 #endif  // SHG_KERNELS_H_

@@ -1,0 +1,261 @@
+// Oblivious expansion and the plaintext-by-ciphertext inner product of an
+// encrypted query against a device-resident database (RNS ciphertexts).
+//
+// Expansion follows ObliviousExpander::ExpandCiphertext
+// (shell_encryption/oblivious_expand.h:173-227) on the RNS stack: at level
+// j, with m = 2^j ciphertexts per query and substitution power
+// p_j = n / 2^j + 1,
+//   sub          = GaloisKeySwitch(Substitute(result[k], p_j))
+//   result[k+m]  = (result[k] - sub) * X^(-2^j)
+//   result[k]    = result[k] + sub
+// using RnsPolynomial::Substitute (rns/rns_polynomial.cc:209-260),
+// RnsGaloisKey::ApplyTo (rns/rns_galois_key.cc:238-287), SubInPlace,
+// AddInPlace and AbsorbInPlace (a MulInPlace by the NTT-form monomial,
+// rns/rns_ciphertext.h:219-225).  Queries are processed together: every
+// kernel runs over all (query, ciphertext) pairs of a level, the key switch
+// over one packed array.
+//
+// The inner product answer[c] = sum_r expanded[r] (.) record[r * C + c]
+// (AbsorbInPlace then AddInPlace per term) is computed with exact FP64
+// products of the expanded residues and the records' exported residues
+// (x * 2^-64 mod q): sum_r e_r * (d_r 2^-64) is the Montgomery product sum
+// the reference forms, reduced once per eight terms.
+#include <hip/hip_runtime.h>
+
+#include "shg_device.h"
+#include "shg_kernels.h"
+
+namespace shg {
+
+namespace {
+
+unsigned grid_of(long long total, int block) { return (unsigned)((total + block - 1) / block); }
+
+__device__ __forceinline__ unsigned brev_bits(unsigned x, int bits) {
+  return __brev(x) >> (32 - bits);
+}
+
+// S[(g*m + k)*2 + comp] = Substitute(result_g[k][comp], power) for every
+// query g, active ciphertext k < m and component; packed.
+__global__ void expand_substitute_kernel(const uint64_t* const* __restrict__ cts, int m,
+                                         uint64_t* __restrict__ S, int power, int logn,
+                                         int num_limbs, long long total) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  const long long poly = (long long)num_limbs << logn;
+  const long long ct = i / (2 * poly);  // g * m + k
+  const long long rem = i - ct * 2 * poly;
+  const int g = (int)(ct / m);
+  const int k = (int)(ct - (long long)g * m);
+  const unsigned n = 1u << logn;
+  const long long limb_base = rem & ~(long long)(n - 1);
+  const unsigned pos = (unsigned)(rem & (n - 1));
+  const unsigned j = brev_bits(pos, logn);
+  const unsigned idx = (unsigned)((((unsigned long long)(power - 1) / 2) +
+                                   (unsigned long long)j * (unsigned)power) & (n - 1));
+  const uint64_t* src = cts[g] + (long long)k * 2 * poly + limb_base;
+  S[i] = src[brev_bits(idx, logn)];
+}
+
+// result_g[k + m] = (result_g[k] - sub) * mono, result_g[k] += sub, both
+// components, for every query g and k < m.  sub0 / sub1 are the packed key
+// switch outputs (index g*m + k).
+__global__ void expand_update_kernel(uint64_t* const* __restrict__ cts, int m,
+                                     const uint64_t* __restrict__ sub0,
+                                     const uint64_t* __restrict__ sub1,
+                                     const uint64_t* __restrict__ mono,
+                                     const PrimeDev* __restrict__ primes, LimbPrimes lp,
+                                     int logn, int num_limbs, long long total) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  const long long poly = (long long)num_limbs << logn;
+  const long long ct = i / poly;  // g * m + k
+  const long long off = i - ct * poly;
+  const int g = (int)(ct / m);
+  const int k = (int)(ct - (long long)g * m);
+  const int limb = (int)(off >> logn);
+  const PrimeDev& P = primes[lp.id[limb]];
+  uint64_t* r = cts[g] + (long long)k * 2 * poly + off;
+  uint64_t* w = cts[g] + (long long)(k + m) * 2 * poly + off;
+  const uint64_t s0 = sub0[i], s1 = sub1[i];
+  const uint64_t r0 = r[0], r1 = r[poly];
+  const uint64_t mo = mono[off];
+  w[0] = mont_mul(mod_sub(r0, s0, P.q), mo, P);
+  w[poly] = mont_mul(mod_sub(r1, s1, P.q), mo, P);
+  r[0] = mod_add(r0, s0, P.q);
+  r[poly] = mod_add(r1, s1, P.q);
+}
+
+constexpr int kCB = 4;  // columns per thread
+
+// Partial inner products: thread (g, column block, K-chunk, limb, pos).
+// part[((g * C + c) * KS + ks) * 2 + comp][limb][pos] holds the canonical
+// sum over the chunk's terms.
+__global__ void pir_partial_kernel(const uint64_t* const* __restrict__ expd, int K,
+                                   const uint64_t* __restrict__ db_x, int C, int KS,
+                                   uint64_t* __restrict__ part, const PrimeDev* __restrict__ primes,
+                                   LimbPrimes lp, int logn, int num_limbs, long long total) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  const long long poly = (long long)num_limbs << logn;
+  const long long off = i % poly;
+  long long rest = i / poly;
+  const int ks = (int)(rest % KS);
+  rest /= KS;
+  const int cb_count = (C + kCB - 1) / kCB;
+  const int cb = (int)(rest % cb_count);
+  const int g = (int)(rest / cb_count);
+  const int limb = (int)(off >> logn);
+  const PrimeDev& P = primes[lp.id[limb]];
+  const double q = P.qd, qinv = P.qinv;
+  const int chunk = (K + KS - 1) / KS;
+  const int r0 = ks * chunk;
+  const int r1 = r0 + chunk < K ? r0 + chunk : K;
+  const int c0 = cb * kCB;
+  double a0[kCB], a1[kCB];
+#pragma unroll
+  for (int cc = 0; cc < kCB; ++cc) a0[cc] = a1[cc] = 0.0;
+  const uint64_t* ex = expd[g];
+  int since = 0;
+  for (int r = r0; r < r1; ++r) {
+    const double e0 = u52_to_f64(ex[(long long)r * 2 * poly + off]);
+    const double e1 = u52_to_f64(ex[(long long)r * 2 * poly + poly + off]);
+#pragma unroll
+    for (int cc = 0; cc < kCB; ++cc) {
+      if (c0 + cc < C) {
+        const double w = u52_to_f64(db_x[((long long)r * C + c0 + cc) * poly + off]);
+        const double wq = w * qinv;
+        a0[cc] += f64_mulmod(e0, w, wq, q);
+        a1[cc] += f64_mulmod(e1, w, wq, q);
+      }
+    }
+    // Terms are below 0.875 q (canonical operands, q < 2^50): eight of them
+    // on a reduced accumulator stay below 7.5 q < 2^53.
+    if (++since == 8) {
+      since = 0;
+#pragma unroll
+      for (int cc = 0; cc < kCB; ++cc) {
+        a0[cc] = f64_reduce(a0[cc], q, qinv);
+        a1[cc] = f64_reduce(a1[cc], q, qinv);
+      }
+    }
+  }
+#pragma unroll
+  for (int cc = 0; cc < kCB; ++cc) {
+    if (c0 + cc >= C) continue;
+    const long long base = (((long long)g * C + c0 + cc) * KS + ks) * 2 * poly + off;
+    part[base] = f64_to_u52(f64_canon(f64_reduce(a0[cc], q, qinv), q));
+    part[base + poly] = f64_to_u52(f64_canon(f64_reduce(a1[cc], q, qinv), q));
+  }
+}
+
+// answer_g[c][comp] = sum over KS partials, canonical.
+__global__ void pir_sum_kernel(const uint64_t* __restrict__ part, int C, int KS,
+                               uint64_t* const* __restrict__ out,
+                               const PrimeDev* __restrict__ primes, LimbPrimes lp, int logn,
+                               int num_limbs, long long total) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  const long long poly = (long long)num_limbs << logn;
+  const long long off = i % (2 * poly);  // comp * poly + limb * n + pos
+  const long long gc = i / (2 * poly);   // g * C + c
+  const int g = (int)(gc / C);
+  const int c = (int)(gc - (long long)g * C);
+  const int limb = (int)((off % poly) >> logn);
+  const PrimeDev& P = primes[lp.id[limb]];
+  uint64_t s = 0;
+  for (int ks = 0; ks < KS; ++ks) s = mod_add(s, part[(gc * KS + ks) * 2 * poly + off], P.q);
+  out[g][(long long)c * 2 * poly + off] = s;
+}
+
+__global__ void export_words_kernel(const uint64_t* __restrict__ in, uint64_t* __restrict__ out,
+                                    const PrimeDev* __restrict__ primes, LimbPrimes lp, int logn,
+                                    int num_limbs, long long total) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  const long long poly = (long long)num_limbs << logn;
+  const int limb = (int)((i % poly) >> logn);
+  out[i] = mont_export(in[i], primes[lp.id[limb]]);
+}
+
+// Records with coefficients uniform in [0, t) from SplitMix64 of (seed,
+// record, coefficient), imported mod each limb (coefficient form).
+__global__ void db_synth_kernel(uint64_t* __restrict__ out, uint64_t seed, uint64_t t,
+                                const PrimeDev* __restrict__ primes, LimbPrimes lp, int logn,
+                                int num_limbs, long long total) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  const long long poly = (long long)num_limbs << logn;
+  const long long rec = i / poly;
+  const long long off = i - rec * poly;
+  const int limb = (int)(off >> logn);
+  const long long coeff = off & ((1LL << logn) - 1);
+  uint64_t z = seed + 0x9e3779b97f4a7c15ull * (uint64_t)(rec * (1LL << logn) + coeff + 1);
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  z ^= z >> 31;
+  out[i] = mont_import(z % t, primes[lp.id[limb]]);
+}
+
+}  // namespace
+
+hipError_t launch_expand_substitute(const uint64_t* const* cts, int count, int m, uint64_t* S,
+                                    int power, int logn, int num_limbs, hipStream_t s) {
+  const long long total = ((long long)count * m * 2 * num_limbs) << logn;
+  hipLaunchKernelGGL(expand_substitute_kernel, dim3(grid_of(total, 256)), dim3(256), 0, s, cts, m,
+                     S, power, logn, num_limbs, total);
+  return hipGetLastError();
+}
+
+hipError_t launch_expand_update(uint64_t* const* cts, int count, int m, const uint64_t* sub0,
+                                const uint64_t* sub1, const uint64_t* mono,
+                                const PrimeDev* primes, const LimbPrimes& lp, int logn,
+                                int num_limbs, hipStream_t s) {
+  const long long total = ((long long)count * m * num_limbs) << logn;
+  hipLaunchKernelGGL(expand_update_kernel, dim3(grid_of(total, 256)), dim3(256), 0, s, cts, m,
+                     sub0, sub1, mono, primes, lp, logn, num_limbs, total);
+  return hipGetLastError();
+}
+
+int pir_k_split(int count, int C, int K, int logn, int num_limbs) {
+  const long long threads = (long long)count * ((C + kCB - 1) / kCB) * ((long long)num_limbs << logn);
+  int ks = 1;
+  while (threads * ks < (1LL << 19) && ks * 2 <= 64 && (K + ks * 2 - 1) / (ks * 2) >= 16) ks *= 2;
+  return ks;
+}
+
+hipError_t launch_pir_inner_product(const uint64_t* const* expd, int count, int K,
+                                    const uint64_t* db_x, int C, uint64_t* part, int KS,
+                                    uint64_t* const* out, const PrimeDev* primes,
+                                    const LimbPrimes& lp, int logn, int num_limbs, hipStream_t s) {
+  const long long poly = (long long)num_limbs << logn;
+  long long total = (long long)count * ((C + kCB - 1) / kCB) * KS * poly;
+  hipLaunchKernelGGL(pir_partial_kernel, dim3(grid_of(total, 256)), dim3(256), 0, s, expd, K, db_x,
+                     C, KS, part, primes, lp, logn, num_limbs, total);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  total = (long long)count * C * 2 * poly;
+  hipLaunchKernelGGL(pir_sum_kernel, dim3(grid_of(total, 256)), dim3(256), 0, s, part, C, KS, out,
+                     primes, lp, logn, num_limbs, total);
+  return hipGetLastError();
+}
+
+hipError_t launch_export_words(const uint64_t* in, uint64_t* out, long long polys,
+                               const PrimeDev* primes, const LimbPrimes& lp, int logn,
+                               int num_limbs, hipStream_t s) {
+  const long long total = (polys * num_limbs) << logn;
+  hipLaunchKernelGGL(export_words_kernel, dim3(grid_of(total, 256)), dim3(256), 0, s, in, out,
+                     primes, lp, logn, num_limbs, total);
+  return hipGetLastError();
+}
+
+hipError_t launch_db_synth(uint64_t* out, long long records, uint64_t seed, uint64_t t,
+                           const PrimeDev* primes, const LimbPrimes& lp, int logn, int num_limbs,
+                           hipStream_t s) {
+  const long long total = (records * num_limbs) << logn;
+  hipLaunchKernelGGL(db_synth_kernel, dim3(grid_of(total, 256)), dim3(256), 0, s, out, seed, t,
+                     primes, lp, logn, num_limbs, total);
+  return hipGetLastError();
+}
+
+}  // namespace shg

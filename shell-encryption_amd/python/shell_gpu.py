@@ -79,6 +79,13 @@ def lib():
         "shg_batch_mod_reduce_plan_create": [VP, ctypes.POINTER(PolyJob), i, ctypes.c_uint64, i, i,
                                              ctypes.POINTER(VP)],
         "shg_batch_run": [VP, VP, i],
+        "shg_oblivious_expand_level": [VP, ctypes.POINTER(VP), i, i, VP, VP, INTP, i, INTP],
+        "shg_oblivious_expand": [VP, ctypes.POINTER(VP), i, i, ctypes.POINTER(VP),
+                                 ctypes.POINTER(VP), INTP, i, INTP],
+        "shg_pir_db_export": [VP, VP, ctypes.c_longlong, VP, i, INTP],
+        "shg_pir_db_synthesize": [VP, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_longlong, VP, i,
+                                  INTP],
+        "shg_pir_inner_product": [VP, ctypes.POINTER(VP), i, i, VP, i, ctypes.POINTER(VP), i, INTP],
         "shg_graph_begin": [VP],
         "shg_graph_end": [VP, ctypes.POINTER(VP)],
         "shg_graph_launch": [VP, VP],
@@ -284,6 +291,37 @@ class Context:
     def mod_reduce_plan(self, jobs, t, is_ntt, variant=NTT_AUTO):
         """jobs: (data_ptr, prime list, ql_inv list, log_n) per polynomial."""
         return BatchPlan(self, None, variant, mod_reduce=(jobs, t, is_ntt))
+
+    @staticmethod
+    def _ptrs(bufs):
+        vals = [b.ptr if isinstance(b, DeviceArray) else int(b) for b in bufs]
+        return (VP * max(len(vals), 1))(*vals)
+
+    def expand_level(self, bufs, level, key_as, key_bs, log_bs, primes):
+        """Oblivious expansion level `level` for every query buffer in bufs."""
+        _check(lib().shg_oblivious_expand_level(self.h, self._ptrs(bufs), len(bufs), int(level),
+                                                key_as.p, key_bs.p, _ints(log_bs), len(primes),
+                                                _ints(primes)))
+
+    def expand(self, bufs, levels, keys, log_bs, primes):
+        """keys: one (key_as, key_bs) DeviceArray pair per level."""
+        ka = self._ptrs([k[0] for k in keys])
+        kb = self._ptrs([k[1] for k in keys])
+        _check(lib().shg_oblivious_expand(self.h, self._ptrs(bufs), len(bufs), int(levels), ka, kb,
+                                          _ints(log_bs), len(primes), _ints(primes)))
+
+    def pir_db_export(self, db, num_records, out, primes):
+        _check(lib().shg_pir_db_export(self.h, db.p, int(num_records), out.p, len(primes),
+                                       _ints(primes)))
+
+    def pir_db_synthesize(self, seed, t, num_records, out, primes):
+        _check(lib().shg_pir_db_synthesize(self.h, int(seed), int(t), int(num_records), out.p,
+                                           len(primes), _ints(primes)))
+
+    def pir_inner_product(self, expanded, levels, db_x, num_columns, outs, primes):
+        _check(lib().shg_pir_inner_product(self.h, self._ptrs(expanded), len(expanded), int(levels),
+                                           db_x.p, int(num_columns), self._ptrs(outs), len(primes),
+                                           _ints(primes)))
 
     def graph_begin(self):
         _check(lib().shg_graph_begin(self.h))
