@@ -220,6 +220,52 @@ int shg_pir_inner_product(shg_context* ctx, const uint64_t* const* host_expanded
                           int levels, const uint64_t* db_exported, int num_columns,
                           uint64_t* const* host_out, int num_limbs, const int* primes);
 
+/* ---- serving: continuous batching of encrypted queries ------------------
+ * No counterpart in the reference.  A server owns a worker thread that
+ * admits queued requests while a slot is free and their HBM footprint
+ * ((2^levels + max columns) ciphertexts each) fits the budget, then
+ * advances every admitted request by one stage per step: all requests at
+ * expansion level j share one shg_oblivious_expand_level call, all fully
+ * expanded requests of one database one shg_pir_inner_product call.
+ * Requests arriving meanwhile join at the next step.  Keys (one Galois key
+ * per level, power n/2^j + 1) and databases (exported records, see
+ * shg_pir_db_export; row r column c is record r * num_columns + c) stay
+ * resident and are not owned by the server.  A request's answer
+ * (num_columns ciphertexts) depends only on its query. */
+typedef struct shg_pir_server shg_pir_server;
+typedef struct shg_pir_server_config {
+  int log_n;               /* ring degree of queries and records */
+  int levels;              /* expansion levels: 2^levels database rows */
+  int max_active;          /* slots: most requests on the device at once */
+  size_t hbm_budget_bytes; /* per-request buffer budget; 0 = slots only */
+} shg_pir_server_config;
+int shg_pir_server_create(shg_context* ctx, const int* primes, int num_limbs,
+                          const int* host_log_bs, const uint64_t* const* key_as,
+                          const uint64_t* const* key_bs, const shg_pir_server_config* cfg,
+                          shg_pir_server** out);
+int shg_pir_server_add_db(shg_pir_server* s, const uint64_t* db_exported, int num_columns,
+                          int* db_id);
+/* Starts the worker; databases cannot be added afterwards. */
+int shg_pir_server_start(shg_pir_server* s);
+/* Queues a query (host copy of one ciphertext, c0 then c1, NTT form). */
+int shg_pir_server_submit(shg_pir_server* s, int db_id, const uint64_t* host_query,
+                          long long* request);
+/* Blocks until the request is answered; host_answer (may be NULL) receives
+ * num_columns ciphertexts, latency_ms the time from submission. */
+int shg_pir_server_wait(shg_pir_server* s, long long request, uint64_t* host_answer,
+                        double* latency_ms);
+int shg_pir_server_stats(shg_pir_server* s, long long* steps, long long* stage_runs,
+                         long long* batched_items, int* capacity);
+/* Submits num_requests queries (cycling over num_distinct host queries) at
+ * Poisson arrivals of rate_per_s from a thread of its own, waits for all and
+ * returns each latency and the wall time. */
+int shg_pir_server_load_test(shg_pir_server* s, int db_id, const uint64_t* host_queries,
+                             int num_distinct, int num_requests, double rate_per_s,
+                             uint64_t seed, double* latencies_ms, double* wall_s);
+/* Stops the worker (queued and running requests fail) and frees the slots.
+ * No other thread may be waiting on the server. */
+int shg_pir_server_destroy(shg_pir_server* s);
+
 /* ---- batched uneven work (persistent launch) ---------------------------
  * No counterpart in the reference, which transforms one RnsPolynomial limb
  * at a time (rns/rns_polynomial.cc:171-207): these run the per-limb

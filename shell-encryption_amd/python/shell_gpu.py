@@ -35,6 +35,11 @@ class LimbJob(ctypes.Structure):
     _fields_ = [("data", VP), ("other", VP), ("prime", ctypes.c_int), ("log_n", ctypes.c_int)]
 
 
+class PirServerConfig(ctypes.Structure):
+    _fields_ = [("log_n", ctypes.c_int), ("levels", ctypes.c_int), ("max_active", ctypes.c_int),
+                ("hbm_budget_bytes", ctypes.c_size_t)]
+
+
 class PolyJob(ctypes.Structure):
     _fields_ = [("data", VP), ("primes", ctypes.POINTER(ctypes.c_int)),
                 ("host_ql_inv", ctypes.POINTER(ctypes.c_uint64)), ("num_limbs", ctypes.c_int),
@@ -86,6 +91,19 @@ def lib():
         "shg_pir_db_synthesize": [VP, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_longlong, VP, i,
                                   INTP],
         "shg_pir_inner_product": [VP, ctypes.POINTER(VP), i, i, VP, i, ctypes.POINTER(VP), i, INTP],
+        "shg_pir_server_create": [VP, INTP, i, INTP, ctypes.POINTER(VP), ctypes.POINTER(VP),
+                                  ctypes.POINTER(PirServerConfig), ctypes.POINTER(VP)],
+        "shg_pir_server_add_db": [VP, VP, i, INTP],
+        "shg_pir_server_start": [VP],
+        "shg_pir_server_submit": [VP, i, U64P, ctypes.POINTER(ctypes.c_longlong)],
+        "shg_pir_server_wait": [VP, ctypes.c_longlong, U64P, ctypes.POINTER(ctypes.c_double)],
+        "shg_pir_server_stats": [VP, ctypes.POINTER(ctypes.c_longlong),
+                                 ctypes.POINTER(ctypes.c_longlong),
+                                 ctypes.POINTER(ctypes.c_longlong), INTP],
+        "shg_pir_server_load_test": [VP, i, U64P, i, i, ctypes.c_double, ctypes.c_uint64,
+                                     ctypes.POINTER(ctypes.c_double),
+                                     ctypes.POINTER(ctypes.c_double)],
+        "shg_pir_server_destroy": [VP],
         "shg_graph_begin": [VP],
         "shg_graph_end": [VP, ctypes.POINTER(VP)],
         "shg_graph_launch": [VP, VP],
@@ -400,6 +418,76 @@ class Graph:
     def close(self):
         if self.h:
             lib().shg_graph_destroy(self.ctx.h, self.h)
+            self.h = VP()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class PirServer:
+    """Continuous-batching server of encrypted queries (shg_pir_server_*)."""
+
+    def __init__(self, ctx, primes, log_n, log_bs, keys, max_active, hbm_budget=0):
+        self.ctx = ctx
+        self.levels = len(keys)
+        self.keys = keys  # keep the key arrays alive
+        ka = Context._ptrs([k[0] for k in keys])
+        kb = Context._ptrs([k[1] for k in keys])
+        cfg = PirServerConfig(int(log_n), self.levels, int(max_active), int(hbm_budget))
+        self.h = VP()
+        _check(lib().shg_pir_server_create(ctx.h, _ints(primes), len(primes), _ints(log_bs), ka, kb,
+                                           ctypes.byref(cfg), ctypes.byref(self.h)))
+        self.ct_words = 2 * len(primes) << int(log_n)
+        self.columns = {}
+        self.dbs = []
+
+    def add_db(self, db_exported, num_columns):
+        d = ctypes.c_int()
+        _check(lib().shg_pir_server_add_db(self.h, db_exported.p, int(num_columns), ctypes.byref(d)))
+        self.columns[d.value] = int(num_columns)
+        self.dbs.append(db_exported)
+        return d.value
+
+    def start(self):
+        _check(lib().shg_pir_server_start(self.h))
+
+    def submit(self, db, query):
+        q = np.ascontiguousarray(query, dtype=np.uint64).ravel()
+        assert q.size == self.ct_words
+        r = ctypes.c_longlong()
+        _check(lib().shg_pir_server_submit(self.h, int(db), q.ctypes.data_as(U64P), ctypes.byref(r)))
+        return r.value, int(db)
+
+    def wait(self, ticket):
+        rid, db = ticket
+        out = np.empty(self.columns[db] * self.ct_words, dtype=np.uint64)
+        ms = ctypes.c_double()
+        _check(lib().shg_pir_server_wait(self.h, rid, out.ctypes.data_as(U64P), ctypes.byref(ms)))
+        return out, ms.value
+
+    def stats(self):
+        a, b, c = ctypes.c_longlong(), ctypes.c_longlong(), ctypes.c_longlong()
+        cap = ctypes.c_int()
+        _check(lib().shg_pir_server_stats(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c),
+                                          ctypes.byref(cap)))
+        return {"steps": a.value, "stage_runs": b.value, "batched_items": c.value,
+                "capacity": cap.value}
+
+    def load_test(self, db, queries, num_requests, rate, seed=1):
+        q = np.ascontiguousarray(queries, dtype=np.uint64)
+        lat = (ctypes.c_double * num_requests)()
+        wall = ctypes.c_double()
+        _check(lib().shg_pir_server_load_test(self.h, int(db), q.ctypes.data_as(U64P),
+                                              q.size // self.ct_words, int(num_requests),
+                                              float(rate), int(seed), lat, ctypes.byref(wall)))
+        return np.array(lat[:]), wall.value
+
+    def close(self):
+        if self.h:
+            lib().shg_pir_server_destroy(self.h)
             self.h = VP()
 
     def __del__(self):

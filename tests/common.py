@@ -198,3 +198,45 @@ def substitute_coeffs(vals, power, q):
         else:
             out[e - n] = (out[e - n] - int(v)) % q
     return out
+
+
+# Oblivious expansion and the database inner product composed from oracle
+# steps exactly as ObliviousExpander::ExpandCiphertext orders them
+# (oblivious_expand.h:173-227).
+
+def monomial_ntt(rns, j):
+    """-X^(n - 2^j) = X^(-2^j) in NTT form (Montgomery representations)."""
+    x = np.zeros((len(rns.qs), rns.n), dtype=np.uint64)
+    for i, p in enumerate(rns.primes):
+        x[i, rns.n - (1 << j)] = p.mont.imp(rns.qs[i] - 1)
+    return rns.ntt_fwd(x)
+
+
+def expand_oracle(rns, ct, levels, keys, log_bs):
+    n = rns.n
+    res = [None] * (1 << levels)
+    res[0] = [ct[0], ct[1]]
+    for j in range(levels):
+        p = n // (1 << j) + 1
+        mono = monomial_ntt(rns, j)
+        kas, kbs = keys[j]
+        for k in range(1 << j):
+            s0, s1 = rns.substitute(res[k][0], p), rns.substitute(res[k][1], p)
+            g0, g1 = rns.galois_apply(s0, s1, kas, kbs, log_bs)
+            d = [rns.pointwise("sub", res[k][0], g0), rns.pointwise("sub", res[k][1], g1)]
+            res[k + (1 << j)] = [rns.pointwise("mul", d[0], mono), rns.pointwise("mul", d[1], mono)]
+            res[k] = [rns.pointwise("add", res[k][0], g0), rns.pointwise("add", res[k][1], g1)]
+    return np.stack([np.stack(r) for r in res])
+
+
+def inner_oracle(rns, expd, db, C):
+    K = expd.shape[0]
+    out = []
+    for c in range(C):
+        acc = [np.zeros_like(expd[0, 0]), np.zeros_like(expd[0, 0])]
+        for r in range(K):
+            for comp in range(2):
+                acc[comp] = rns.pointwise("add", acc[comp],
+                                          rns.pointwise("mul", expd[r, comp], db[r * C + c]))
+        out.append(np.stack(acc))
+    return np.stack(out)
