@@ -226,6 +226,59 @@ def uneven_batch(ctx, rng):
     return res
 
 
+def serving(ctx, rng):
+    """The serving workload: encrypted queries (one ciphertext at n=2^12,
+    L=2 primes below 2^50) against three device-resident synthetic databases
+    of 2^14, 2^16 and 2^18 records (plaintext polynomials, coefficients
+    uniform mod t=65537), each query expanded by 12 levels of oblivious
+    expansion (Galois substitutions + key switching, base 2^10) into 4096
+    row selectors and then multiplied into the database (4096 rows x 4, 16
+    or 64 columns).  Poisson arrivals at a low and a saturating rate; the
+    server admits by HBM footprint and batches requests of one stage."""
+    import shell_gpu as sg
+    log_n, Lq, levels, t, log_b = 12, 2, 12, 65537, 10
+    qs = primes_for(log_n, Lq)
+    pids = ctx.primes(qs, log_n)
+    n = 1 << log_n
+    k = sum((q.bit_length() + log_b - 1) // log_b for q in qs)
+    print("[bench] serving: databases and keys", file=sys.stderr, flush=True)
+    keys = [(ctx.array(uniform(rng, qs, (k,), n)), ctx.array(uniform(rng, qs, (k,), n)))
+            for _ in range(levels)]
+    rows = 1 << levels
+    db_sizes = [1 << 14, 1 << 16, 1 << 18]
+    dbs = []
+    for i, records in enumerate(db_sizes):
+        d = ctx.empty((records, Lq, n))
+        ctx.pir_db_synthesize(1000 + i, t, records, d, pids)
+        ctx.pir_db_export(d, records, d, pids)  # in place: the server reads exported records
+        dbs.append((d, records // rows))
+    ctx.sync()
+    queries = uniform(rng, qs, (8, 2), n)
+    budget = 16 << 30
+    srv = sg.PirServer(ctx, pids, log_n, [log_b] * Lq, keys, max_active=16, hbm_budget=budget)
+    ids = [srv.add_db(d, c) for d, c in dbs]
+    srv.start()
+    srv.wait(srv.submit(ids[0], queries[0]))  # warm-up: first-use allocations
+    res = {"workload": "n=2^12, L=2, 12 expansion levels (4096 rows), t=65537, base 2^10",
+           "capacity": srv.stats()["capacity"],
+           "per_query_hbm_bytes": ((rows + 64) * 2 * Lq * n * 8), "dbs": {}}
+    for (d, cols), records, db in zip(dbs, db_sizes, ids):
+        entry = {"records": records, "columns": cols, "db_bytes": records * Lq * n * 8}
+        for name, rate, count in (("low_rate", 20.0, 12), ("high_rate", 4000.0, 64)):
+            print("[bench] serving: %d records, %s" % (records, name), file=sys.stderr, flush=True)
+            lat, wall = srv.load_test(db, queries, count, rate, seed=records + int(rate))
+            entry[name] = {"offered_qps": rate, "requests": count,
+                           "queries_per_s": round(count / wall, 2),
+                           "p50_ms": round(percentile(lat, 50), 3),
+                           "p99_ms": round(percentile(lat, 99), 3)}
+        res["dbs"][str(records)] = entry
+    st = srv.stats()
+    res["mean_batch"] = round(st["batched_items"] / max(st["stage_runs"], 1), 2)
+    srv.close()
+    del dbs
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -429,6 +482,7 @@ def main():
                                    "hbm_frac": round(by / (f_ms / f_n * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
         del d_x
         out["uneven_batch"] = uneven_batch(ctx, rng)
+        out["serving"] = serving(ctx, rng)
 
     if not args.no_cpu:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)

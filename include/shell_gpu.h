@@ -235,7 +235,7 @@ int shg_pir_inner_product(shg_context* ctx, const uint64_t* const* host_expanded
 typedef struct shg_pir_server shg_pir_server;
 typedef struct shg_pir_server_config {
   int log_n;               /* ring degree of queries and records */
-  int levels;              /* expansion levels: 2^levels database rows */
+  int levels;              /* expansion levels (1..log_n): 2^levels database rows */
   int max_active;          /* slots: most requests on the device at once */
   size_t hbm_budget_bytes; /* per-request buffer budget; 0 = slots only */
 } shg_pir_server_config;

@@ -268,6 +268,10 @@ int build_gadget(shg_context* ctx, const int* primes, int num_limbs, const int* 
 
 }  // namespace
 
+namespace shg {
+int set_error(int code, const char* msg) { return fail(code, "%s", msg); }
+}  // namespace shg
+
 extern "C" {
 
 const char* shg_last_error(void) { return g_last_error.c_str(); }

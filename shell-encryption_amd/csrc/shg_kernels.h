@@ -115,6 +115,10 @@ hipError_t launch_lazy_fma(const uint64_t* as, size_t as_stride, const uint64_t*
                            const PrimeDev* primes, const LimbPrimes& lp, int num_limbs,
                            int logn, hipStream_t s);
 
+// Records msg as the calling thread's last error (shg_last_error) and
+// returns code.
+int set_error(int code, const char* msg);
+
 // Sets hipFuncAttributeMaxDynamicSharedMemorySize once per kernel and device.
 void set_max_dynamic_shmem(const void* kernel, size_t bytes);
 

@@ -30,6 +30,7 @@
 #include <vector>
 
 #include "shell_gpu.h"
+#include "shg_kernels.h"
 
 namespace {
 
@@ -218,12 +219,15 @@ int shg_pir_server_create(shg_context* ctx, const int* primes, int num_limbs,
                           const int* host_log_bs, const uint64_t* const* key_as,
                           const uint64_t* const* key_bs, const shg_pir_server_config* cfg,
                           shg_pir_server** out) {
+  using shg::set_error;
   if (!ctx || !primes || !host_log_bs || !cfg || !out || num_limbs <= 0)
-    return SHG_E_INVALID_ARGUMENT;
-  if (cfg->levels < 1 || cfg->levels > 14 || cfg->max_active < 1 || cfg->log_n < 4 ||
-      cfg->log_n > 14 || cfg->levels >= cfg->log_n)
-    return SHG_E_INVALID_ARGUMENT;
-  if (!key_as || !key_bs) return SHG_E_INVALID_ARGUMENT;
+    return set_error(SHG_E_INVALID_ARGUMENT, "null argument or no moduli.");
+  if (cfg->log_n < 4 || cfg->log_n > 14)
+    return set_error(SHG_E_INVALID_ARGUMENT, "log_n must be in [4, 14].");
+  if (cfg->levels < 1 || cfg->levels > cfg->log_n)
+    return set_error(SHG_E_INVALID_ARGUMENT, "levels must be in [1, log_n].");
+  if (cfg->max_active < 1) return set_error(SHG_E_INVALID_ARGUMENT, "max_active must be positive.");
+  if (!key_as || !key_bs) return set_error(SHG_E_INVALID_ARGUMENT, "one Galois key per level is required.");
   auto s = std::make_unique<shg_pir_server>();
   s->ctx = ctx;
   s->primes.assign(primes, primes + num_limbs);
@@ -241,9 +245,11 @@ int shg_pir_server_create(shg_context* ctx, const int* primes, int num_limbs,
 
 int shg_pir_server_add_db(shg_pir_server* s, const uint64_t* db_exported, int num_columns,
                           int* db_id) {
-  if (!s || !db_exported || num_columns <= 0 || !db_id) return SHG_E_INVALID_ARGUMENT;
+  if (!s || !db_exported || num_columns <= 0 || !db_id)
+    return shg::set_error(SHG_E_INVALID_ARGUMENT, "invalid database.");
   std::lock_guard<std::mutex> lock(s->mu);
-  if (s->started) return SHG_E_FAILED_PRECONDITION;
+  if (s->started)
+    return shg::set_error(SHG_E_FAILED_PRECONDITION, "databases must be added before start.");
   s->dbs.push_back({db_exported, num_columns});
   s->max_columns = num_columns > s->max_columns ? num_columns : s->max_columns;
   *db_id = (int)s->dbs.size() - 1;
@@ -254,12 +260,13 @@ int shg_pir_server_start(shg_pir_server* s) {
   if (!s) return SHG_E_INVALID_ARGUMENT;
   std::lock_guard<std::mutex> lock(s->mu);
   if (s->started) return SHG_OK;
-  if (s->dbs.empty()) return SHG_E_FAILED_PRECONDITION;
+  if (s->dbs.empty()) return shg::set_error(SHG_E_FAILED_PRECONDITION, "no database.");
   s->footprint = (((size_t)1 << s->levels) + (size_t)s->max_columns) * s->ct_words * 8;
   const size_t by_budget =
       s->cfg.hbm_budget_bytes ? s->cfg.hbm_budget_bytes / s->footprint : (size_t)s->cfg.max_active;
   s->capacity = (int)(by_budget < (size_t)s->cfg.max_active ? by_budget : s->cfg.max_active);
-  if (s->capacity < 1) return SHG_E_NO_MEMORY;
+  if (s->capacity < 1)
+    return shg::set_error(SHG_E_NO_MEMORY, "the HBM budget does not fit one request.");
   s->started = true;
   s->worker = std::thread([s] { s->run(); });
   return SHG_OK;
@@ -272,8 +279,10 @@ int shg_pir_server_submit(shg_pir_server* s, int db_id, const uint64_t* host_que
   req->db = db_id;
   req->query.assign(host_query, host_query + s->ct_words);
   std::lock_guard<std::mutex> lock(s->mu);
-  if (!s->started || s->stop) return SHG_E_FAILED_PRECONDITION;
-  if (db_id < 0 || db_id >= (int)s->dbs.size()) return SHG_E_INVALID_ARGUMENT;
+  if (!s->started || s->stop)
+    return shg::set_error(SHG_E_FAILED_PRECONDITION, "the server is not running.");
+  if (db_id < 0 || db_id >= (int)s->dbs.size())
+    return shg::set_error(SHG_E_INVALID_ARGUMENT, "unknown database.");
   req->id = s->next_id++;
   req->submitted = Clock::now();
   s->requests[req->id] = req;
@@ -290,7 +299,7 @@ int shg_pir_server_wait(shg_pir_server* s, long long request, uint64_t* host_ans
   {
     std::unique_lock<std::mutex> lock(s->mu);
     auto it = s->requests.find(request);
-    if (it == s->requests.end()) return SHG_E_INVALID_ARGUMENT;
+    if (it == s->requests.end()) return shg::set_error(SHG_E_INVALID_ARGUMENT, "unknown request.");
     req = it->second;
     s->cv_done.wait(lock, [&] { return req->done; });
     s->requests.erase(it);
