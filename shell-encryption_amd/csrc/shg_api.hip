@@ -71,7 +71,7 @@ struct shg_context {
     void* p = nullptr;
     size_t size = 0;
   };
-  DevBuf exp_s, exp_sub, exp_ptrs, exp_ptrs2, pir_part;
+  DevBuf exp_s, exp_sub, exp_ptrs, exp_ptrs2, pir_part, ks_planes;
   std::map<std::pair<std::vector<int>, int>, void*> monomials;
   std::mutex mu;
   // Optional per-kernel timing: HIP events recorded around launches.
@@ -254,6 +254,7 @@ int build_gadget(shg_context* ctx, const int* primes, int num_limbs, const int* 
       gd->log_b[k] = (unsigned char)b;
       ++k;
     }
+    gd->limb_first[i] = (unsigned char)(k - dims);
     gd->limb_digits[i] = (unsigned char)dims;
     gd->limb_log_b[i] = (unsigned char)b;
     max_b = b > max_b ? b : max_b;
@@ -311,7 +312,8 @@ int shg_destroy(shg_context* ctx) {
     if (p->d_tables) hipFree(p->d_tables);
   if (ctx->d_primes) hipFree(ctx->d_primes);
   if (ctx->scratch) hipFree(ctx->scratch);
-  for (auto* b : {&ctx->exp_s, &ctx->exp_sub, &ctx->exp_ptrs, &ctx->exp_ptrs2, &ctx->pir_part})
+  for (auto* b : {&ctx->exp_s, &ctx->exp_sub, &ctx->exp_ptrs, &ctx->exp_ptrs2, &ctx->pir_part,
+                  &ctx->ks_planes})
     if (b->p) hipFree(b->p);
   for (auto& m : ctx->monomials) hipFree(m.second);
   if (ctx->own_stream) hipStreamDestroy(ctx->own_stream);
@@ -693,7 +695,42 @@ static int keyswitch_common(shg_context* ctx, const uint64_t* comps, int comps_p
                                 (long long)count * num_limbs);
   hipError_t e = hipSuccess;
   const bool fused = log_n <= shg::kMaxLogM && log_n >= 4;
+  // Digit planes: the inverse transform writes the gadget digits as 16-bit
+  // planes read straight from the components (FP64 key switch, bases up to
+  // 2^16, NTT-form source).
+  const bool planes_ok = fused && (v & 3) == SHG_NTT_F64 && gd.small_digits && gd.max_log_b <= 16;
+  uint16_t* planes = nullptr;
+  if (planes_ok) {
+    rc = ensure_buf(ctx, ctx->ks_planes, (size_t)count * gd.k * n * 2);
+    if (rc) return rc;
+    planes = (uint16_t*)ctx->ks_planes.p;
+  }
   for (int i = 0; i < num_src && e == hipSuccess; ++i) {
+    if (planes_ok && src_is_ntt[i]) {
+      {
+        ProfScope prof(ctx, "keyswitch_intt");
+        e = shg::launch_ntt_inv_digits(v, comps + (size_t)src_comp[i] * poly_words,
+                                       (size_t)comps_per_ct * poly_words, planes, gd,
+                                       ctx->d_primes, lp, count, num_limbs, log_n, s);
+      }
+      if (e != hipSuccess) break;
+      shg::KeySwitchArgs ka{};
+      ka.planes = planes;
+      ka.c0 = comps;
+      ka.c0_stride = (size_t)comps_per_ct * poly_words;
+      ka.c1 = init_c1 ? comps + poly_words : nullptr;
+      ka.c1_stride = (size_t)comps_per_ct * poly_words;
+      ka.key_a = key_as;
+      ka.key_b = key_bs;
+      ka.key_base = key_base[i];
+      ka.out0 = out_c0;
+      ka.out1 = out_c1;
+      ka.out_stride = poly_words;
+      ka.accumulate = i > 0;
+      ProfScope prof(ctx, "keyswitch");
+      e = shg::launch_keyswitch(ka, gd, v & 3, ctx->d_primes, lp, count, num_limbs, log_n, s);
+      continue;
+    }
     e = hipMemcpy2DAsync(src, poly_words * 8, comps + (size_t)src_comp[i] * poly_words,
                          (size_t)comps_per_ct * poly_words * 8, poly_words * 8, count,
                          hipMemcpyDeviceToDevice, s);

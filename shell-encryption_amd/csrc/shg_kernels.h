@@ -79,6 +79,7 @@ struct GadgetDesc {
   // consecutive in j).
   unsigned char limb_digits[SHG_MAX_LIMBS];
   unsigned char limb_log_b[SHG_MAX_LIMBS];
+  unsigned char limb_first[SHG_MAX_LIMBS];  // global index of limb i's first digit
   int max_log_b;
   // 2^max_log_b * 64 <= every modulus: digits enter the transform far below
   // q, which lets the FP64 key switch defer its first reduction.
@@ -94,6 +95,7 @@ hipError_t launch_gadget_decompose(const uint64_t* a, uint64_t* digits, const Ga
 struct KeySwitchArgs {
   const uint64_t* src;       // exported (ExportInt) coefficient-form input, count x L x n
   size_t src_stride;         // words between ciphertexts in src
+  const uint16_t* planes;    // or: digit planes (launch_ntt_inv_digits), count x k x n
   const uint64_t* c0;        // NTT-form c0 (added), or null
   size_t c0_stride;
   const uint64_t* c1;        // NTT-form c1 (added), or null
@@ -118,6 +120,20 @@ hipError_t launch_lazy_fma(const uint64_t* as, size_t as_stride, const uint64_t*
 // Records msg as the calling thread's last error (shg_last_error) and
 // returns code.
 int set_error(int code, const char* msg);
+
+// Gadget digits of the inverse transform of a component, written as 16-bit
+// planes for the key switch: limb l of ciphertext c is read at
+// in + c * in_stride + l * n (NTT form), transformed to coefficient form
+// with ExportInt folded into the n^-1 scaling, and digit dd of limb l goes
+// to plane (c * k + limb_first[l] + dd), which stores position p at index
+// (p % TC) * VC + p / TC with VC = 16 (n <= 2^13) or 32 (n = 2^14) and
+// TC = n / VC: a key-switch thread reads its VC digits contiguously.
+// Requires every limb's base log <= 16 and 4 <= log n <= 14.
+hipError_t launch_ntt_inv_digits(int variant, const uint64_t* in, size_t in_stride,
+                                 uint16_t* planes, const GadgetDesc& gd, const PrimeDev* primes,
+                                 const LimbPrimes& lp, int count, int num_limbs, int logn,
+                                 hipStream_t s);
+constexpr int ks_planes_vc(int logn) { return logn >= 14 ? 32 : 16; }
 
 // Sets hipFuncAttributeMaxDynamicSharedMemorySize once per kernel and device.
 void set_max_dynamic_shmem(const void* kernel, size_t bytes);

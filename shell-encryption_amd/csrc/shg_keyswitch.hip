@@ -68,7 +68,7 @@ struct F64KsArith : F64Arith {
 // so that each XCD runs one contiguous range (one output limb at the bench
 // sizes): its CUs then share that limb's key rows and both halves of a
 // ciphertext's source in their own L2.
-template <int LOGM, int LOGE, int HB, class A>
+template <int LOGM, int LOGE, int HB, class A, bool PL>
 __global__ void __launch_bounds__((1 << LOGM) >> LOGE)
     keyswitch_kernel(KeySwitchArgs ka, GadgetDesc gd, const PrimeDev* __restrict__ primes,
                      LimbPrimes lp, int count, int num_limbs) {
@@ -128,81 +128,141 @@ __global__ void __launch_bounds__((1 << LOGM) >> LOGE)
     }
   }
 
-  const uint64_t* src_ct = ka.src + ct * ka.src_stride;
-  const bool digits32 = gd.max_log_b <= 32;
-  int j = 0;
-  // Digits of source limb i are consecutive: j runs over (i, dd).
-  for (int i = 0; i < num_limbs; ++i) {
-    const uint64_t* src = src_ct + ((size_t)i << LOGN);
-    const int lb = gd.limb_log_b[i];
-    const int nd = gd.limb_digits[i];
-    const uint64_t mask = (1ull << lb) - 1;
-    for (int dd = 0; dd < nd; ++dd, ++j) {
-      const int sh = dd * lb;
-      V x[E];
-      // Digit extraction (one uniform branch per digit on the base width),
-      // then for HB = 1 the first Cooley-Tukey stage on this half.
-      auto load_digits = [&](auto conv) {
-        if constexpr (HB == 0) {
+  // Forward transform of digit j's x (stage 0 already applied for HB = 1)
+  // and its multiply-accumulate with key row j.
+  auto transform_and_mac = [&](int j, V (&x)[E]) {
+    Passes::run(x, lds0, lds1, t, h, ar);
+    if constexpr (SHG_KS_DB && (Passes::kExchanges & 1)) {
+      V* tmp = lds0;
+      lds0 = lds1;
+      lds1 = tmp;
+    }
+    const uint64_t* kb = ka.key_b + (size_t)(ka.key_base + j) * poly_words + limb_off;
+    const uint64_t* kap = ka.key_a + (size_t)(ka.key_base + j) * poly_words + limb_off;
 #pragma unroll
-          for (int e = 0; e < E; ++e) x[e] = conv(src[e * T + t] >> sh);
+    for (int gg = 0; gg < G; ++gg) {
+      const size_t base = (size_t)(gg * T + t) << R;
+#pragma unroll
+      for (int e = 0; e < (1 << R); e += 2) {
+        const u64x2 kbv = *reinterpret_cast<const u64x2*>(kb + base + e);
+        const u64x2 kav = *reinterpret_cast<const u64x2*>(kap + base + e);
+        const int r = gg * (1 << R) + e;
+        typename A::Prep d0, d1;
+        if constexpr (kPrep) {
+          d0 = ar.prep(x[r]);
+          d1 = ar.prep(x[r + 1]);
         } else {
-          const auto w = ar.fwd_tw(1);
-#pragma unroll
-          for (int e = 0; e < E; ++e) {
-            V u = conv(src[e * T + t] >> sh);
-            V v = conv(src[M + e * T + t] >> sh);
-            ar.ct(u, v, w);
-            x[e] = h ? v : u;
-            if (ar.fwd_reduce_at(0)) ar.fwd_reduce(x[e]);
-          }
+          d0 = x[r];
+          d1 = x[r + 1];
         }
+        ar.mac(acc0[r], d0, kbv.x, P);
+        ar.mac(acc0[r + 1], d1, kbv.y, P);
+        ar.mac(acc1[r], d0, kav.x, P);
+        ar.mac(acc1[r + 1], d1, kav.y, P);
+      }
+    }
+    if (((j + 1) % kAccEvery) == 0) {
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        ar.acc_reduce(acc0[e]);
+        ar.acc_reduce(acc1[e]);
+      }
+    }
+    // The buffer the next digit writes first was last read before the
+    // barrier of this digit's final exchange, so no barrier is needed.
+  };
+
+  if constexpr (PL) {
+    // Digits from 16-bit planes: thread t's VC digits of a plane are
+    // contiguous (VC / 8 16-byte loads); the next digit's are loaded while
+    // this one is transformed.
+    constexpr int VC = ks_planes_vc(LOGN);
+    constexpr int NW = VC / 8;
+    static_assert(VC == E << HB, "plane layout must match the slice tiling");
+    const uint4* pl = reinterpret_cast<const uint4*>(ka.planes + ((size_t)ct * gd.k << LOGN)) +
+                      (size_t)t * NW;
+    constexpr size_t kPlane = ((size_t)1 << LOGN) / 8;  // uint4 per plane
+    uint4 cur[NW];
+#pragma unroll
+    for (int w = 0; w < NW; ++w) cur[w] = pl[w];
+    for (int j = 0; j < gd.k; ++j) {
+      uint4 nxt[NW];
+      if (j + 1 < gd.k) {
+#pragma unroll
+        for (int w = 0; w < NW; ++w) nxt[w] = pl[(size_t)(j + 1) * kPlane + w];
+      }
+      uint32_t wd[VC / 2];
+#pragma unroll
+      for (int w = 0; w < NW; ++w) {
+        wd[4 * w] = cur[w].x;
+        wd[4 * w + 1] = cur[w].y;
+        wd[4 * w + 2] = cur[w].z;
+        wd[4 * w + 3] = cur[w].w;
+      }
+      auto digit = [&](int i) {
+        const uint32_t v = (i & 1) ? (wd[i / 2] >> 16) : (wd[i / 2] & 0xffffu);
+        return ar.digit_in32(v, P);
       };
-      if (digits32) {
-        const uint32_t m32 = (uint32_t)mask;
-        load_digits([&](uint64_t s64) { return ar.digit_in32((uint32_t)s64 & m32, P); });
+      V x[E];
+      if constexpr (HB == 0) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) x[e] = digit(e);
       } else {
-        load_digits([&](uint64_t s64) { return ar.digit_in(s64 & mask, P); });
-      }
-      Passes::run(x, lds0, lds1, t, h, ar);
-      if constexpr (SHG_KS_DB && (Passes::kExchanges & 1)) {
-        V* tmp = lds0;
-        lds0 = lds1;
-        lds1 = tmp;
-      }
-      const uint64_t* kb = ka.key_b + (size_t)(ka.key_base + j) * poly_words + limb_off;
-      const uint64_t* kap = ka.key_a + (size_t)(ka.key_base + j) * poly_words + limb_off;
-#pragma unroll
-      for (int gg = 0; gg < G; ++gg) {
-        const size_t base = (size_t)(gg * T + t) << R;
-#pragma unroll
-        for (int e = 0; e < (1 << R); e += 2) {
-          const u64x2 kbv = *reinterpret_cast<const u64x2*>(kb + base + e);
-          const u64x2 kav = *reinterpret_cast<const u64x2*>(kap + base + e);
-          const int r = gg * (1 << R) + e;
-          typename A::Prep d0, d1;
-          if constexpr (kPrep) {
-            d0 = ar.prep(x[r]);
-            d1 = ar.prep(x[r + 1]);
-          } else {
-            d0 = x[r];
-            d1 = x[r + 1];
-          }
-          ar.mac(acc0[r], d0, kbv.x, P);
-          ar.mac(acc0[r + 1], d1, kbv.y, P);
-          ar.mac(acc1[r], d0, kav.x, P);
-          ar.mac(acc1[r + 1], d1, kav.y, P);
-        }
-      }
-      if (((j + 1) % kAccEvery) == 0) {
+        const auto w = ar.fwd_tw(1);
 #pragma unroll
         for (int e = 0; e < E; ++e) {
-          ar.acc_reduce(acc0[e]);
-          ar.acc_reduce(acc1[e]);
+          V u = digit(e);
+          V v = digit(E + e);
+          ar.ct(u, v, w);
+          x[e] = h ? v : u;
+          if (ar.fwd_reduce_at(0)) ar.fwd_reduce(x[e]);
         }
       }
-      // The buffer the next digit writes first was last read before the
-      // barrier of this digit's final exchange, so no barrier is needed.
+      transform_and_mac(j, x);
+      if (j + 1 < gd.k) {
+#pragma unroll
+        for (int w = 0; w < NW; ++w) cur[w] = nxt[w];
+      }
+    }
+  } else {
+    const uint64_t* src_ct = ka.src + ct * ka.src_stride;
+    const bool digits32 = gd.max_log_b <= 32;
+    int j = 0;
+    // Digits of source limb i are consecutive: j runs over (i, dd).
+    for (int i = 0; i < num_limbs; ++i) {
+      const uint64_t* src = src_ct + ((size_t)i << LOGN);
+      const int lb = gd.limb_log_b[i];
+      const int nd = gd.limb_digits[i];
+      const uint64_t mask = (1ull << lb) - 1;
+      for (int dd = 0; dd < nd; ++dd, ++j) {
+        const int sh = dd * lb;
+        V x[E];
+        // Digit extraction (one uniform branch per digit on the base
+        // width), then for HB = 1 the first Cooley-Tukey stage on this half.
+        auto load_digits = [&](auto conv) {
+          if constexpr (HB == 0) {
+#pragma unroll
+            for (int e = 0; e < E; ++e) x[e] = conv(src[e * T + t] >> sh);
+          } else {
+            const auto w = ar.fwd_tw(1);
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+              V u = conv(src[e * T + t] >> sh);
+              V v = conv(src[M + e * T + t] >> sh);
+              ar.ct(u, v, w);
+              x[e] = h ? v : u;
+              if (ar.fwd_reduce_at(0)) ar.fwd_reduce(x[e]);
+            }
+          }
+        };
+        if (digits32) {
+          const uint32_t m32 = (uint32_t)mask;
+          load_digits([&](uint64_t s64) { return ar.digit_in32((uint32_t)s64 & m32, P); });
+        } else {
+          load_digits([&](uint64_t s64) { return ar.digit_in(s64 & mask, P); });
+        }
+        transform_and_mac(j, x);
+      }
     }
   }
 #pragma unroll
@@ -221,47 +281,52 @@ __global__ void __launch_bounds__((1 << LOGM) >> LOGE)
   }
 }
 
-template <int LOGM, int LOGE, class A, int HB = 0>
+template <int LOGM, int LOGE, class A, int HB, bool PL>
 static hipError_t launch_ks(const KeySwitchArgs& ka, const GadgetDesc& gd, const PrimeDev* primes,
                             const LimbPrimes& lp, int count, int num_limbs, hipStream_t s) {
   constexpr int T = (1 << LOGM) >> LOGE;
   const size_t shmem = (size_t)lds_words<LOGM>() * 8 * (SHG_KS_DB ? 2 : 1);
-  auto k = keyswitch_kernel<LOGM, LOGE, HB, A>;
+  auto k = keyswitch_kernel<LOGM, LOGE, HB, A, PL>;
   if (shmem > 65536) set_max_dynamic_shmem((const void*)k, shmem);
   hipLaunchKernelGGL(k, dim3((unsigned)(count * num_limbs) << HB), dim3(T), shmem, s, ka, gd,
                      primes, lp, count, num_limbs);
   return hipGetLastError();
 }
 
-template <class A>
+template <class A, bool PL>
 static hipError_t dispatch_ks(int logn, const KeySwitchArgs& ka, const GadgetDesc& gd,
                               const PrimeDev* primes, const LimbPrimes& lp, int count,
                               int num_limbs, hipStream_t s) {
+#define SHG_KS(M, E, HB) launch_ks<M, E, A, HB, PL>(ka, gd, primes, lp, count, num_limbs, s)
   switch (logn) {
-    case 4: return launch_ks<4, 4, A>(ka, gd, primes, lp, count, num_limbs, s);
-    case 5: return launch_ks<5, 4, A>(ka, gd, primes, lp, count, num_limbs, s);
-    case 6: return launch_ks<6, 4, A>(ka, gd, primes, lp, count, num_limbs, s);
-    case 7: return launch_ks<7, 4, A>(ka, gd, primes, lp, count, num_limbs, s);
-    case 8: return launch_ks<8, 4, A>(ka, gd, primes, lp, count, num_limbs, s);
-    case 9: return launch_ks<9, 4, A>(ka, gd, primes, lp, count, num_limbs, s);
-    case 10: return launch_ks<10, 4, A>(ka, gd, primes, lp, count, num_limbs, s);
-    case 11: return launch_ks<11, 4, A>(ka, gd, primes, lp, count, num_limbs, s);
-    case 12: return launch_ks<12, 4, A>(ka, gd, primes, lp, count, num_limbs, s);
-    case 13: return launch_ks<13, 4, A>(ka, gd, primes, lp, count, num_limbs, s);
-    case 14: return launch_ks<13, SHG_KS_LOGE, A, 1>(ka, gd, primes, lp, count, num_limbs, s);
+    case 4: return SHG_KS(4, 4, 0);
+    case 5: return SHG_KS(5, 4, 0);
+    case 6: return SHG_KS(6, 4, 0);
+    case 7: return SHG_KS(7, 4, 0);
+    case 8: return SHG_KS(8, 4, 0);
+    case 9: return SHG_KS(9, 4, 0);
+    case 10: return SHG_KS(10, 4, 0);
+    case 11: return SHG_KS(11, 4, 0);
+    case 12: return SHG_KS(12, 4, 0);
+    case 13: return SHG_KS(13, 4, 0);
+    case 14: return SHG_KS(13, SHG_KS_LOGE, 1);
     default: return hipErrorInvalidValue;
   }
+#undef SHG_KS
 }
 
 hipError_t launch_keyswitch(const KeySwitchArgs& ka, const GadgetDesc& gd, int variant,
                             const PrimeDev* primes, const LimbPrimes& lp, int count, int num_limbs,
                             int logn, hipStream_t s) {
   if (variant == SHG_NTT_F64) {
-    if (gd.small_digits)
-      return dispatch_ks<F64KsArith>(logn, ka, gd, primes, lp, count, num_limbs, s);
-    return dispatch_ks<F64Arith>(logn, ka, gd, primes, lp, count, num_limbs, s);
+    if (gd.small_digits) {
+      if (ka.planes)
+        return dispatch_ks<F64KsArith, true>(logn, ka, gd, primes, lp, count, num_limbs, s);
+      return dispatch_ks<F64KsArith, false>(logn, ka, gd, primes, lp, count, num_limbs, s);
+    }
+    return dispatch_ks<F64Arith, false>(logn, ka, gd, primes, lp, count, num_limbs, s);
   }
-  return dispatch_ks<IntArith>(logn, ka, gd, primes, lp, count, num_limbs, s);
+  return dispatch_ks<IntArith, false>(logn, ka, gd, primes, lp, count, num_limbs, s);
 }
 
 }  // namespace shg

@@ -157,6 +157,119 @@ __global__ void __launch_bounds__((1 << LOGM) >> LOGE)
   }
 }
 
+// Inverse transform of one limb per workgroup followed by gadget digit
+// extraction into 16-bit planes (launch_ntt_inv_digits).  After the last
+// pass the exported values go through LDS in natural order so that thread
+// t can gather the E words of consumer indices [t*E, t*E + E) and store each
+// plane's E digits with 16-byte writes.
+template <int LOGM, int LOGE, class A>
+__global__ void __launch_bounds__((1 << LOGM) >> LOGE)
+    ntt_inv_digits_kernel(const uint64_t* __restrict__ in, size_t in_stride,
+                          uint16_t* __restrict__ planes, GadgetDesc gd,
+                          const PrimeDev* __restrict__ primes, LimbPrimes lp, int num_limbs) {
+  using V = typename A::V;
+  constexpr int E = 1 << LOGE;
+  constexpr int T = (1 << LOGM) >> LOGE;
+  constexpr int n = 1 << LOGM;
+  constexpr int VC = ks_planes_vc(LOGM);
+  constexpr int TC = n / VC;
+  static_assert(E == 16, "digit planes are written 16 words per thread");
+  extern __shared__ __align__(16) unsigned char smem[];
+  V* lds = reinterpret_cast<V*>(smem);
+  uint64_t* lds_u = reinterpret_cast<uint64_t*>(smem);
+  const int row = blockIdx.x;
+  const int ct = row / num_limbs;
+  const int l = row - ct * num_limbs;
+  const int t = threadIdx.x;
+  const A ar(primes[lp.id[l]]);
+  const uint64_t* g = in + (size_t)ct * in_stride + ((size_t)l << LOGM);
+#pragma unroll
+  for (int e = 0; e < E; ++e) lds[lpad(e * T + t)] = ar.load(g[e * T + t]);
+  __syncthreads();
+  V x[E];
+  InvPasses<LOGM, LOGE, 0, A, false, LOGM>::run(x, lds, t, 0, ar);
+  // The last pass read its exchange before its barrier: LDS is free.
+  constexpr int I0 = InvLast<LOGM, LOGE, 0, A>::value;
+  constexpr int R = LOGM - I0;
+  constexpr int G = E >> R;
+#pragma unroll
+  for (int gg = 0; gg < G; ++gg) {
+#pragma unroll
+    for (int e = 0; e < (1 << R); ++e)
+      lds_u[lpad((e << I0) | (gg * T + t))] = ar.inv_out(x[gg * (1 << R) + e], 2);
+  }
+  __syncthreads();
+  uint64_t v[E];
+#pragma unroll
+  for (int i = 0; i < E; ++i) {
+    const int idx = t * E + i;
+    v[i] = lds_u[lpad((idx % VC) * TC + idx / VC)];
+  }
+  const int lb = gd.limb_log_b[l];
+  const int nd = gd.limb_digits[l];
+  const uint32_t mask = (1u << lb) - 1;
+  uint16_t* base = planes + ((size_t)ct * gd.k + gd.limb_first[l]) * n + (size_t)t * E;
+  for (int dd = 0; dd < nd; ++dd) {
+    const int sh = dd * lb;
+    uint32_t w[E / 2];
+#pragma unroll
+    for (int i = 0; i < E; i += 2)
+      w[i / 2] = ((uint32_t)(v[i] >> sh) & mask) | (((uint32_t)(v[i + 1] >> sh) & mask) << 16);
+    uint4* dst = reinterpret_cast<uint4*>(base + (size_t)dd * n);
+    dst[0] = make_uint4(w[0], w[1], w[2], w[3]);
+    dst[1] = make_uint4(w[4], w[5], w[6], w[7]);
+  }
+}
+
+template <int LOGM, class A>
+static hipError_t launch_inv_digits(const uint64_t* in, size_t in_stride, uint16_t* planes,
+                                    const GadgetDesc& gd, const PrimeDev* primes,
+                                    const LimbPrimes& lp, int count, int num_limbs,
+                                    hipStream_t s) {
+  constexpr int LOGE = 4;
+  constexpr int T = (1 << LOGM) >> LOGE;
+  const size_t shmem = (size_t)lds_words<LOGM>() * 8;
+  auto k = ntt_inv_digits_kernel<LOGM, LOGE, A>;
+  if (shmem > 65536) set_max_dynamic_shmem((const void*)k, shmem);
+  hipLaunchKernelGGL(k, dim3((unsigned)(count * num_limbs)), dim3(T), shmem, s, in, in_stride,
+                     planes, gd, primes, lp, num_limbs);
+  return hipGetLastError();
+}
+
+template <class A>
+static hipError_t dispatch_inv_digits(int logn, const uint64_t* in, size_t in_stride,
+                                      uint16_t* planes, const GadgetDesc& gd,
+                                      const PrimeDev* primes, const LimbPrimes& lp, int count,
+                                      int num_limbs, hipStream_t s) {
+#define SHG_DIG(M) launch_inv_digits<M, A>(in, in_stride, planes, gd, primes, lp, count, num_limbs, s)
+  switch (logn) {
+    case 4: return SHG_DIG(4);
+    case 5: return SHG_DIG(5);
+    case 6: return SHG_DIG(6);
+    case 7: return SHG_DIG(7);
+    case 8: return SHG_DIG(8);
+    case 9: return SHG_DIG(9);
+    case 10: return SHG_DIG(10);
+    case 11: return SHG_DIG(11);
+    case 12: return SHG_DIG(12);
+    case 13: return SHG_DIG(13);
+    case 14: return SHG_DIG(14);
+    default: return hipErrorInvalidValue;
+  }
+#undef SHG_DIG
+}
+
+hipError_t launch_ntt_inv_digits(int variant, const uint64_t* in, size_t in_stride,
+                                 uint16_t* planes, const GadgetDesc& gd, const PrimeDev* primes,
+                                 const LimbPrimes& lp, int count, int num_limbs, int logn,
+                                 hipStream_t s) {
+  if ((variant & 3) == SHG_NTT_F64)
+    return dispatch_inv_digits<F64Arith>(logn, in, in_stride, planes, gd, primes, lp, count,
+                                         num_limbs, s);
+  return dispatch_inv_digits<IntArith>(logn, in, in_stride, planes, gd, primes, lp, count,
+                                       num_limbs, s);
+}
+
 // Streaming radix-2 stage over HBM for limbs larger than the LDS slice:
 // forward stage s (pairs at distance n/2^(s+1)), or inverse stage i.
 // Integer values are kept in [0, 2q); F64 values are kept balanced.
