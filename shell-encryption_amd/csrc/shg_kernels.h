@@ -126,14 +126,19 @@ int set_error(int code, const char* msg);
 // in + c * in_stride + l * n (NTT form), transformed to coefficient form
 // with ExportInt folded into the n^-1 scaling, and digit dd of limb l goes
 // to plane (c * k + limb_first[l] + dd), which stores position p at index
-// (p % TC) * VC + p / TC with VC = 16 (n <= 2^13) or 32 (n = 2^14) and
+// (p % TC) * VC + p / TC with VC = 16 (n <= 2^13) or 2^(SHG_KS_LOGE+1)
+// (n = 2^14) and
 // TC = n / VC: a key-switch thread reads its VC digits contiguously.
 // Requires every limb's base log <= 16 and 4 <= log n <= 14.
 hipError_t launch_ntt_inv_digits(int variant, const uint64_t* in, size_t in_stride,
                                  uint16_t* planes, const GadgetDesc& gd, const PrimeDev* primes,
                                  const LimbPrimes& lp, int count, int num_limbs, int logn,
                                  hipStream_t s);
-constexpr int ks_planes_vc(int logn) { return logn >= 14 ? 32 : 16; }
+// Words per thread of the key switch at n = 2^14 (two 2^13-word slices).
+#ifndef SHG_KS_LOGE
+#define SHG_KS_LOGE 4
+#endif
+constexpr int ks_planes_vc(int logn) { return logn >= 14 ? (2 << SHG_KS_LOGE) : 16; }
 
 // Sets hipFuncAttributeMaxDynamicSharedMemorySize once per kernel and device.
 void set_max_dynamic_shmem(const void* kernel, size_t bytes);

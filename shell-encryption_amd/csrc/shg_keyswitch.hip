@@ -22,10 +22,6 @@
 #include "shg_kernels.h"
 #include "shg_ntt_core.h"
 
-// Per-thread word count (log2) of the 2^13-word key-switch slices.
-#ifndef SHG_KS_LOGE
-#define SHG_KS_LOGE 4
-#endif
 #ifndef SHG_KS_XCD_REMAP
 #define SHG_KS_XCD_REMAP 1
 #endif
