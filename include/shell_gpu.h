@@ -72,6 +72,10 @@ typedef enum shg_ntt_variant {
  * one (FP64 only; ignored for the integer variant). */
 #define SHG_NTT_TILING_ALT 4
 #define SHG_NTT_PREFETCH 8
+/* Forward transforms of 2^14-coefficient limbs split each limb over two
+ * workgroups (both compute stage 0; each finishes one half), two of which
+ * share a CU. */
+#define SHG_NTT_SPLIT_LIMB 16
 
 /* ---- context, memory, streams ------------------------------------------ */
 int shg_create(int device, shg_context** out);

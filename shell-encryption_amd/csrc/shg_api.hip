@@ -493,7 +493,7 @@ static int ntt_common(shg_context* ctx, bool inverse, uint64_t* polys, int count
   if (count == 0) return SHG_OK;
   if (!aligned16(polys))
     return fail(SHG_E_INVALID_ARGUMENT, "polynomial data must be a non-null 16-byte aligned pointer.");
-  if (variant < 0 || (variant & 3) == 3 || (variant & ~15) != 0)
+  if (variant < 0 || (variant & 3) == 3 || (variant & ~31) != 0)
     return fail(SHG_E_INVALID_ARGUMENT, "unknown transform variant %d.", variant);
   const long long rows = (long long)count * num_limbs;
   if ((rows << (log_n > shg::kMaxLogM ? log_n - shg::kMaxLogM : 0)) > 0x7fffffffLL)
@@ -1416,6 +1416,7 @@ int shg_autotune_ntt(shg_context* ctx, int log_n, int limbs, int* chosen) {
     cands.push_back(SHG_NTT_F64 | shg::kNttCfgAlt | shg::kNttCfgPrefetch);
     cands.push_back(SHG_NTT_INT_SHOUP | shg::kNttCfgAlt);
   }
+  if (log_n == 14) cands.push_back(SHG_NTT_F64 | shg::kNttCfgSplit);
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   if (e == hipSuccess) e = hipEventCreate(&ev0);
   if (e == hipSuccess) e = hipEventCreate(&ev1);

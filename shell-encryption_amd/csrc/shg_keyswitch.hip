@@ -43,6 +43,8 @@ namespace shg {
 // unreduced and the accumulator is reduced every three digits.
 struct F64KsArith : F64Arith {
   using F64Arith::F64Arith;
+  // Without prep (small n) the accumulation relies on the last reduction.
+  static constexpr bool kSkipLastReduce = false;
   __device__ static constexpr bool fwd_reduce_at(int s) { return s == 5 || s == 10; }
   __host__ __device__ static constexpr int last_reduce(int logn) {
     return logn - 1 >= 10 ? 10 : (logn - 1 >= 5 ? 5 : -1);

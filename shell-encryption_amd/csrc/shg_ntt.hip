@@ -23,6 +23,9 @@
 #ifndef SHG_LOGE_14
 #define SHG_LOGE_14 4
 #endif
+#ifndef SHG_NTT_DIRECT_STORE
+#define SHG_NTT_DIRECT_STORE 1
+#endif
 
 namespace shg {
 
@@ -46,8 +49,33 @@ __device__ __forceinline__ uint64_t* slice_ptr(uint64_t* data, int item) {
   return data + ((size_t)(item >> HB) << logn) + ((size_t)(item & ((1 << HB) - 1)) << LOGM);
 }
 
+// Canonical outputs of a forward run with LAST_TO_LDS = false: thread t
+// holds, for each group gamma = gg*T + t of the last pass, the 2^R words
+// (gamma << R) .. (gamma << R) + 2^R - 1, stored as 16-byte pairs.
+template <int LOGM, int LOGE, class A>
+__device__ __forceinline__ void store_fwd_last(uint64_t* g, const typename A::V (&x)[1 << LOGE],
+                                               int t, const A& ar) {
+  constexpr int T = (1 << LOGM) >> LOGE;
+  constexpr int R = FwdLast<LOGM, LOGE>::R_LAST;
+  constexpr int G = FwdLast<LOGM, LOGE>::G_LAST;
+  static_assert(R >= 1, "pairs of words");
+#pragma unroll
+  for (int gg = 0; gg < G; ++gg) {
+    u64x2* o = reinterpret_cast<u64x2*>(g + ((size_t)(gg * T + t) << R));
+#pragma unroll
+    for (int e = 0; e < (1 << R); e += 2) {
+      u64x2 v;
+      v.x = ar.fwd_out(x[gg * (1 << R) + e]);
+      v.y = ar.fwd_out(x[gg * (1 << R) + e + 1]);
+      o[e / 2] = v;
+    }
+  }
+}
+
 // HB = log n - LOGM: the number of leading stages already applied by
-// streaming passes (limbs larger than one LDS slice).
+// streaming passes (limbs larger than one LDS slice).  When the last pass
+// covers at most two stages its words are stored straight from VGPRs (two
+// 16-byte pieces per group of four words); otherwise they go through LDS.
 template <int LOGM, int LOGE, int HB, class A, bool PF>
 __global__ void __launch_bounds__((1 << LOGM) >> LOGE)
     ntt_fwd_kernel(uint64_t* __restrict__ data, const PrimeDev* __restrict__ primes,
@@ -55,6 +83,8 @@ __global__ void __launch_bounds__((1 << LOGM) >> LOGE)
   using V = typename A::V;
   constexpr int E = 1 << LOGE;
   constexpr int T = (1 << LOGM) >> LOGE;
+  constexpr bool kDirectStore =
+      SHG_NTT_DIRECT_STORE && FwdLast<LOGM, LOGE>::R_LAST >= 1 && FwdLast<LOGM, LOGE>::R_LAST <= 2;
   extern __shared__ __align__(16) unsigned char smem[];
   V* lds = reinterpret_cast<V*>(smem);
   const int t = threadIdx.x;
@@ -87,12 +117,60 @@ __global__ void __launch_bounds__((1 << LOGM) >> LOGE)
         }
       }
     };
-    // The last pass goes through LDS so that lanes store consecutive words.
-    FwdPasses<LOGM, LOGE, 0, A, true, HB>::run(x, lds, lds, t, h, ar, prefetch);
+    if constexpr (kDirectStore) {
+      // The last pass leaves 2^R consecutive words per group in VGPRs: they
+      // are stored from there in 16-byte pieces (its reads were followed by
+      // a barrier, so the next item may write LDS at once).
+      FwdPasses<LOGM, LOGE, 0, A, false, HB>::run(x, lds, lds, t, h, ar, prefetch);
+      store_fwd_last<LOGM, LOGE>(g, x, t, ar);
+    } else {
+      // The last pass goes through LDS so that lanes store consecutive words.
+      FwdPasses<LOGM, LOGE, 0, A, true, HB>::run(x, lds, lds, t, h, ar, prefetch);
 #pragma unroll
-    for (int e = 0; e < E; ++e) g[e * T + t] = ar.fwd_out(lds[lpad(e * T + t)]);
-    // The next item's first exchange overwrites what was just read.
-    __syncthreads();
+      for (int e = 0; e < E; ++e) g[e * T + t] = ar.fwd_out(lds[lpad(e * T + t)]);
+      // The next item's first exchange overwrites what was just read.
+      __syncthreads();
+    }
+  }
+}
+
+// Forward transform of 2^14-word limbs with each limb split over two
+// workgroups: workgroup half h reads the whole limb, computes its half of
+// stage 0 (u + w v for h = 0, u - w v for h = 1, pairs (i, i + 2^13)), then
+// runs stages 1..13 on that 2^13-word half in 68 KiB of LDS, so two
+// workgroups fit on a CU and one's HBM phases overlap the other's
+// arithmetic.  Workgroups b and b + 8 (the same XCD under round-robin
+// dispatch) take the two halves of one limb, so the second read of the limb
+// is served from that XCD's L2.  Persistent: the grid is a multiple of 16 and
+// each pair loops over limbs row = slot, slot + grid/2, ...
+template <class A>
+__global__ void __launch_bounds__(512, 4)
+    ntt_fwd_split_kernel(uint64_t* __restrict__ data, const PrimeDev* __restrict__ primes,
+                         LimbPrimes lp, int num_limbs, int rows) {
+  using V = typename A::V;
+  constexpr int LOGM = 13;
+  constexpr int LOGE = 4;
+  constexpr int E = 1 << LOGE;
+  constexpr int T = (1 << LOGM) >> LOGE;
+  constexpr int H = 1 << LOGM;
+  extern __shared__ __align__(16) unsigned char smem[];
+  V* lds = reinterpret_cast<V*>(smem);
+  const int t = threadIdx.x;
+  const int k = blockIdx.x >> 3;
+  const int h = k & 1;
+  const int step = gridDim.x >> 1;
+  for (int row = ((k >> 1) << 3) + (blockIdx.x & 7); row < rows; row += step) {
+    const A ar(primes[lp.id[row % num_limbs]]);
+    uint64_t* g = data + ((size_t)row << (LOGM + 1));
+    const auto w = ar.fwd_tw(1);
+    V x[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e)
+      x[e] = ar.ct_half(ar.load(g[e * T + t]), ar.load(g[H + e * T + t]), w, h != 0);
+    // The last pass is one stage: each thread stores its pairs of words
+    // straight from VGPRs, and the pass's own barrier frees LDS.
+    FwdPasses<LOGM, LOGE, 0, A, false, 1>::run(x, lds, lds, t, h, ar);
+    store_fwd_last<LOGM, LOGE>(g + h * H, x, t, ar);
   }
 }
 
@@ -386,6 +464,20 @@ static hipError_t launch_fused_hb(bool inverse, uint64_t* data, const PrimeDev* 
   return hipGetLastError();
 }
 
+template <class A>
+static hipError_t launch_fwd_split(uint64_t* data, const PrimeDev* primes, const LimbPrimes& lp,
+                                   int rows, int num_limbs, hipStream_t stream) {
+  constexpr int T = 512;
+  const size_t shmem = (size_t)lds_words<13>() * 8;
+  auto k = ntt_fwd_split_kernel<A>;
+  set_max_dynamic_shmem((const void*)k, shmem);
+  // Two workgroups per limb; the grid must be a multiple of 16.
+  int grid = persistent_grid((const void*)k, T, shmem, 2 * rows, 0);
+  grid = grid < 16 ? 16 : grid - grid % 16;
+  hipLaunchKernelGGL(k, dim3(grid), dim3(T), shmem, stream, data, primes, lp, num_limbs, rows);
+  return hipGetLastError();
+}
+
 template <int LOGM, int LOGE, class A, bool PF>
 static hipError_t launch_fused(bool inverse, uint64_t* data, const PrimeDev* primes,
                                const LimbPrimes& lp, int rows, int num_limbs, int logn,
@@ -472,6 +564,11 @@ hipError_t launch_ntt(bool inverse, int cfg, uint64_t* data, const PrimeDev* pri
                       const LimbPrimes& lp, int rows, int num_limbs, int logn, int max_logm,
                       hipStream_t s, int inv_mode) {
   const bool alt = (cfg & kNttCfgAlt) != 0;
+  if (!inverse && logn == 14 && max_logm >= 13 && (cfg & kNttCfgSplit)) {
+    if ((cfg & 3) == SHG_NTT_F64)
+      return launch_fwd_split<F64Arith>(data, primes, lp, rows, num_limbs, s);
+    return launch_fwd_split<IntArith>(data, primes, lp, rows, num_limbs, s);
+  }
   if ((cfg & 3) == SHG_NTT_F64) {
     if (cfg & kNttCfgPrefetch)
       return run_ntt<F64Arith, true>(inverse, alt, data, primes, lp, rows, num_limbs, logn,

@@ -63,6 +63,9 @@ constexpr int lds_words() {
 
 struct IntArith {
   using V = uint64_t;
+  // Whether the reduction scheduled after the last forward stage may be
+  // left to the consumer (fwd_out or the key switch's prep reduce anyway).
+  static constexpr bool kSkipLastReduce = false;
   uint64_t q, q2, bn;
   gptr<uint64_t> fw;
   gptr<uint64_t> iw;
@@ -93,6 +96,11 @@ struct IntArith {
     const uint64_t t = shoup_lazy(v, w.c, w.cb, q);
     v = u + q2 - t;
     u = u + t;
+  }
+  // One output of a Cooley-Tukey butterfly: u - w v when `low`, else u + w v.
+  __device__ __forceinline__ V ct_half(V u, V v, const Tw& w, bool low) const {
+    const uint64_t t = shoup_lazy(v, w.c, w.cb, q);
+    return low ? u + q2 - t : u + t;
   }
   // Gentleman-Sande on [0, 2q) inputs, outputs in [0, 2q).
   __device__ __forceinline__ void gs(V& u, V& v, const Tw& w) const {
@@ -137,6 +145,7 @@ struct IntArith {
 
 struct F64Arith {
   using V = double;
+  static constexpr bool kSkipLastReduce = true;
   double q, qinv, ninv, ninvq, ninvx, ninvxq;
   gptr<double> fw;
   gptr<double> iw;
@@ -160,6 +169,10 @@ struct F64Arith {
     const double t = f64_mulmod(v, w.w, w.wq, q);
     v = u - t;
     u = u + t;
+  }
+  __device__ __forceinline__ V ct_half(V u, V v, const Tw& w, bool low) const {
+    const double t = f64_mulmod(v, w.w, w.wq, q);
+    return u + (low ? -t : t);
   }
   __device__ __forceinline__ void gs(V& u, V& v, const Tw& w) const {
     const double s = u + v;
@@ -211,6 +224,9 @@ struct F64Arith {
   // reduction (B = q/2 + 2^-49 q) five stages give B < 7.7 q; both stay below
   // 8 q < 2^53, so every sum and product term is an exact integer.  A GS
   // stage at most doubles B: three stages from B = q, four from B = q/2.
+  // A reduction due after the last forward stage is skipped
+  // (kSkipLastReduce): the words then go to fwd_out or the key switch's
+  // prep, which reduce any |x| < 2^53 themselves.
   __device__ static constexpr bool fwd_reduce_at(int s) {
     return s == 3 || (s > 3 && (s - 3) % 5 == 0);
   }
@@ -236,9 +252,9 @@ struct PassTw {
 // Forward pass: stages S0..S0+R-1 (global stage s toggles index bit
 // LOGN-1-s).  The slice is sub-block `h` of a limb with LOGN bits; local
 // indices have LOGM bits, so the twiddle block carries h in its top bits.
-// Twiddles are loaded by fwd_pass_tw before the pass's LDS exchange so that
-// their latency overlaps it; in the first pass they are uniform across the
-// workgroup and come through the scalar cache.
+// Twiddles are loaded by fwd_pass_tw before the LDS exchange that precedes
+// the pass, so that their latency overlaps it; in the first pass they are
+// uniform across the workgroup and come through the scalar cache.
 template <int LOGM, int LOGE, int S0, int R, int HB, class A>
 __device__ __forceinline__ void fwd_pass_tw(typename A::TwRaw* tw, int t, int h, const A& ar) {
   constexpr int T = (1 << LOGM) >> LOGE;
@@ -284,7 +300,7 @@ __device__ __forceinline__ void fwd_pass(typename A::V (&x)[1 << LOGE],
         }
       }
     }
-    if (ar.fwd_reduce_at(s)) {
+    if (ar.fwd_reduce_at(s) && !(A::kSkipLastReduce && s == LOGM + HB - 1)) {
 #pragma unroll
       for (int i = 0; i < E; ++i) ar.fwd_reduce(x[i]);
     }
@@ -332,39 +348,58 @@ struct NoHook {
   __device__ __forceinline__ void operator()(P) const {}
 };
 
-// With DB (double buffering) consecutive exchanges alternate between lds0
-// and lds1, so a pass may write its exchange as soon as it has computed: the
-// barrier that separates the previous exchange's reads from new writes to
-// the same buffer is the one after the intervening exchange's writes.  The
-// caller must continue the alternation across calls (swap the buffers after
-// a call with an odd number of exchanges).  hook(integral_constant<S0>) runs
-// in every pass right after its twiddle loads are issued and before its LDS
-// reads, for callers that issue loads of their own there (vector loads
-// complete in issue order, so a load issued after the twiddles does not
-// delay this pass's twiddles).
+// Every exchange keeps word i at LDS slot lpad(i), and a pass reads and
+// writes back the same words, so a thread's writes never land on a slot
+// that another thread still has to read in that pass: one barrier (after
+// the writes) separates consecutive passes.  Only a last pass that leaves
+// its words in VGPRs waits after its reads, so that the caller may write
+// LDS again.  The twiddles of pass p + 1 are issued before pass p's
+// exchange, so their latency overlaps its writes and barrier.
+// With DB consecutive exchanges alternate between lds0 and lds1 and no last
+// pass waits after its reads (the caller must continue the alternation
+// across calls: swap the buffers after a call with an odd number of
+// exchanges).  hook(integral_constant<S0>) runs right after the twiddle
+// loads of the pass starting at stage S0 are issued, for callers that issue
+// loads of their own there (vector loads complete in issue order, so a load
+// issued after the twiddles does not delay them).
 template <int LOGM, int LOGE, int S0, class A, bool LAST_TO_LDS = true, int HB = 0,
           bool DB = false>
 struct FwdPasses {
   static constexpr int R = (LOGM - S0) < LOGE ? (LOGM - S0) : LOGE;
   static constexpr int P = S0 / LOGE;  // pass index (all passes but the last have R = LOGE)
+  static constexpr bool kLast = S0 + R >= LOGM;
+  using Tw = typename A::TwRaw[PassTw<LOGE, R>::N];
   template <class Hook = NoHook>
   __device__ __forceinline__ static void run(typename A::V (&x)[1 << LOGE], typename A::V* lds0,
                                              typename A::V* lds1, int t, int h, const A& ar,
                                              const Hook& hook = Hook()) {
-    typename A::TwRaw tw[PassTw<LOGE, R>::N];
+    Tw tw;
     fwd_pass_tw<LOGM, LOGE, S0, R, HB>(tw, t, h, ar);
     hook(std::integral_constant<int, S0>());
+    run_tw(x, tw, lds0, lds1, t, h, ar, hook);
+  }
+  // The same with this pass's twiddles already loaded.
+  template <class Hook>
+  __device__ __forceinline__ static void run_tw(typename A::V (&x)[1 << LOGE], const Tw& tw,
+                                                typename A::V* lds0, typename A::V* lds1, int t,
+                                                int h, const A& ar, const Hook& hook) {
     if constexpr (S0 > 0) {
       fwd_from_lds<LOGM, LOGE, S0, R>(x, (DB && ((P - 1) & 1)) ? lds1 : lds0, t);
-      if constexpr (!DB) __syncthreads();
+      if constexpr (!DB && kLast && !LAST_TO_LDS) __syncthreads();
     }
     fwd_pass<LOGM, LOGE, S0, R, HB>(x, tw, ar);
-    if constexpr (S0 + R < LOGM || LAST_TO_LDS) {
+    if constexpr (!kLast) {
+      using Next = FwdPasses<LOGM, LOGE, S0 + R, A, LAST_TO_LDS, HB, DB>;
+      typename Next::Tw tw2;
+      fwd_pass_tw<LOGM, LOGE, S0 + R, Next::R, HB>(tw2, t, h, ar);
+      hook(std::integral_constant<int, S0 + R>());
+      fwd_to_lds<LOGM, LOGE, S0, R>(x, (DB && (P & 1)) ? lds1 : lds0, t);
+      __syncthreads();
+      Next::run_tw(x, tw2, lds0, lds1, t, h, ar, hook);
+    } else if constexpr (LAST_TO_LDS) {
       fwd_to_lds<LOGM, LOGE, S0, R>(x, (DB && (P & 1)) ? lds1 : lds0, t);
       __syncthreads();
     }
-    if constexpr (S0 + R < LOGM)
-      FwdPasses<LOGM, LOGE, S0 + R, A, LAST_TO_LDS, HB, DB>::run(x, lds0, lds1, t, h, ar, hook);
   }
   __device__ __forceinline__ static void run(typename A::V (&x)[1 << LOGE], typename A::V* lds,
                                              int t, int h, const A& ar) {
@@ -478,27 +513,40 @@ __device__ __forceinline__ void inv_from_lds(V (&x)[1 << LOGE], const V* lds, in
 // With FIRST_IN_REGS the caller has already loaded the first pass's words
 // into VGPRs: thread t holds, for each group gamma = gg*T + t, the 2^R
 // consecutive words (gamma << R) + e (the first pass toggles the low bits).
-// hook(integral_constant<I0>) runs as in FwdPasses: after the pass's
-// twiddle loads, before its LDS reads.
+// As in FwdPasses, words stay at slot lpad(i), so only the last pass waits
+// after its reads (the caller may then write LDS at once), and the next
+// pass's twiddles are issued before an exchange.  hook(integral_constant<I0>)
+// runs right after the twiddle loads of the pass starting at I0.
 template <int LOGM, int LOGE, int I0, class A, bool FIRST_IN_REGS = false, int LOGN = LOGM>
 struct InvPasses {
   static constexpr int R = (LOGM - I0) < LOGE ? (LOGM - I0) : LOGE;
+  using Tw = typename A::TwRaw[PassTw<LOGE, R>::N];
   template <class Hook = NoHook>
   __device__ __forceinline__ static void run(typename A::V (&x)[1 << LOGE], typename A::V* lds,
                                              int t, int h, const A& ar,
                                              const Hook& hook = Hook()) {
-    typename A::TwRaw tw[PassTw<LOGE, R>::N];
+    Tw tw;
     inv_pass_tw<LOGM, LOGE, I0, R, LOGN>(tw, t, h, ar);
     hook(std::integral_constant<int, I0>());
+    run_tw(x, tw, lds, t, h, ar, hook);
+  }
+  template <class Hook>
+  __device__ __forceinline__ static void run_tw(typename A::V (&x)[1 << LOGE], const Tw& tw,
+                                                typename A::V* lds, int t, int h, const A& ar,
+                                                const Hook& hook) {
     if constexpr (!(FIRST_IN_REGS && I0 == 0)) {
       inv_from_lds<LOGM, LOGE, I0, R>(x, lds, t);
-      __syncthreads();
+      if constexpr (I0 + R >= LOGM) __syncthreads();
     }
     inv_pass<LOGM, LOGE, I0, R>(x, tw, ar);
     if constexpr (I0 + R < LOGM) {
+      using Next = InvPasses<LOGM, LOGE, I0 + R, A, false, LOGN>;
+      typename Next::Tw tw2;
+      inv_pass_tw<LOGM, LOGE, I0 + R, Next::R, LOGN>(tw2, t, h, ar);
+      hook(std::integral_constant<int, I0 + R>());
       inv_to_lds<LOGM, LOGE, I0, R>(x, lds, t);
       __syncthreads();
-      InvPasses<LOGM, LOGE, I0 + R, A, false, LOGN>::run(x, lds, t, h, ar, hook);
+      Next::run_tw(x, tw2, lds, t, h, ar, hook);
     }
   }
 };
