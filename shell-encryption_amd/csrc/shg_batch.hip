@@ -38,9 +38,18 @@ __device__ __forceinline__ void fwd_slice_io(int h, const A& ar, typename A::V* 
 #pragma unroll
   for (int e = 0; e < E; ++e) x[e] = ld(e * kT + t);
   FwdPasses<LOGM, LOGE, 0, A, true, HB>::run(x, lds, t, h, ar);
+  // The next job's first exchange overwrites what is read here: each group
+  // of (up to) four reads is pinned before the barrier.
+  constexpr int C = E < 4 ? E : 4;
 #pragma unroll
-  for (int e = 0; e < E; ++e) st(e * kT + t, ar.fwd_out(lds[lpad(e * kT + t)]));
-  // The next job's first exchange overwrites what was just read.
+  for (int e0 = 0; e0 < E; e0 += C) {
+    typename A::V y[C];
+#pragma unroll
+    for (int e = 0; e < C; ++e) y[e] = lds[lpad((e0 + e) * kT + t)];
+    pin_words(y);
+#pragma unroll
+    for (int e = 0; e < C; ++e) st((e0 + e) * kT + t, ar.fwd_out(y[e]));
+  }
   __syncthreads();
 }
 

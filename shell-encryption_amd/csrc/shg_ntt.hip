@@ -26,6 +26,12 @@
 #ifndef SHG_NTT_DIRECT_STORE
 #define SHG_NTT_DIRECT_STORE 1
 #endif
+// Where the next item's words are loaded with kNttCfgPrefetch: 0 = after
+// the first pass's twiddles, 1 = after the last pass's, 2 = right before
+// the current item's stores.
+#ifndef SHG_NTT_PF_AT
+#define SHG_NTT_PF_AT 0
+#endif
 
 namespace shg {
 
@@ -85,6 +91,8 @@ __global__ void __launch_bounds__((1 << LOGM) >> LOGE)
   constexpr int T = (1 << LOGM) >> LOGE;
   constexpr bool kDirectStore =
       SHG_NTT_DIRECT_STORE && FwdLast<LOGM, LOGE>::R_LAST >= 1 && FwdLast<LOGM, LOGE>::R_LAST <= 2;
+  constexpr int kPfStage =
+      SHG_NTT_PF_AT == 0 ? 0 : (SHG_NTT_PF_AT == 1 ? LOGM - FwdLast<LOGM, LOGE>::R_LAST : -1);
   extern __shared__ __align__(16) unsigned char smem[];
   V* lds = reinterpret_cast<V*>(smem);
   const int t = threadIdx.x;
@@ -109,7 +117,16 @@ __global__ void __launch_bounds__((1 << LOGM) >> LOGE)
     for (int e = 0; e < E; ++e) x[e] = ar.load(pf[e]);
     const int next = item + gridDim.x;
     auto prefetch = [&](auto s0) {
-      if constexpr (PF && decltype(s0)::value == 0) {
+      if constexpr (PF && decltype(s0)::value == kPfStage) {
+        if (next < items) {
+          const uint64_t* gn = slice_ptr<LOGM, HB>(data, next);
+#pragma unroll
+          for (int e = 0; e < E; ++e) pf[e] = gn[e * T + t];
+        }
+      }
+    };
+    auto prefetch_before_store = [&]() {
+      if constexpr (PF && SHG_NTT_PF_AT == 2) {
         if (next < items) {
           const uint64_t* gn = slice_ptr<LOGM, HB>(data, next);
 #pragma unroll
@@ -122,13 +139,24 @@ __global__ void __launch_bounds__((1 << LOGM) >> LOGE)
       // are stored from there in 16-byte pieces (its reads were followed by
       // a barrier, so the next item may write LDS at once).
       FwdPasses<LOGM, LOGE, 0, A, false, HB>::run(x, lds, lds, t, h, ar, prefetch);
+      prefetch_before_store();
       store_fwd_last<LOGM, LOGE>(g, x, t, ar);
     } else {
       // The last pass goes through LDS so that lanes store consecutive words.
       FwdPasses<LOGM, LOGE, 0, A, true, HB>::run(x, lds, lds, t, h, ar, prefetch);
+      prefetch_before_store();
+      // The next item's first exchange overwrites what is read here: each
+      // group of (up to) four reads is pinned before the barrier.
+      constexpr int C = E < 4 ? E : 4;
 #pragma unroll
-      for (int e = 0; e < E; ++e) g[e * T + t] = ar.fwd_out(lds[lpad(e * T + t)]);
-      // The next item's first exchange overwrites what was just read.
+      for (int e0 = 0; e0 < E; e0 += C) {
+        V y[C];
+#pragma unroll
+        for (int e = 0; e < C; ++e) y[e] = lds[lpad((e0 + e) * T + t)];
+        pin_words(y);
+#pragma unroll
+        for (int e = 0; e < C; ++e) g[(e0 + e) * T + t] = ar.fwd_out(y[e]);
+      }
       __syncthreads();
     }
   }
@@ -183,6 +211,8 @@ __global__ void __launch_bounds__((1 << LOGM) >> LOGE)
   constexpr int E = 1 << LOGE;
   constexpr int T = (1 << LOGM) >> LOGE;
   constexpr int logn = LOGM + HB;
+  constexpr int kPfStage =
+      SHG_NTT_PF_AT == 0 ? 0 : (SHG_NTT_PF_AT == 1 ? InvLast<LOGM, LOGE, 0, A>::value : -1);
   extern __shared__ __align__(16) unsigned char smem[];
   V* lds = reinterpret_cast<V*>(smem);
   const int t = threadIdx.x;
@@ -208,7 +238,7 @@ __global__ void __launch_bounds__((1 << LOGM) >> LOGE)
     for (int e = 0; e < E; ++e) lds[lpad(e * T + t)] = ar.load(pf[e]);
     __syncthreads();
     auto prefetch = [&](auto i0) {
-      if constexpr (PF && decltype(i0)::value == 0) {
+      if constexpr (PF && decltype(i0)::value == kPfStage) {
         if (next < items) {
           const uint64_t* gn = slice_ptr<LOGM, HB>(data, next);
 #pragma unroll
@@ -218,18 +248,33 @@ __global__ void __launch_bounds__((1 << LOGM) >> LOGE)
     };
     V x[E];
     InvPasses<LOGM, LOGE, 0, A, false, logn>::run(x, lds, t, h, ar, prefetch);
+    if constexpr (PF && SHG_NTT_PF_AT == 2) {
+      if (next < items) {
+        const uint64_t* gn = slice_ptr<LOGM, HB>(data, next);
+#pragma unroll
+        for (int e = 0; e < E; ++e) pf[e] = gn[e * T + t];
+      }
+    }
     // The last pass toggles the top local bits: word (e << I0) | gamma.
+    // The scaling mode is uniform: one store loop per mode keeps branches
+    // out of the per-word path.
     constexpr int I0 = InvLast<LOGM, LOGE, 0, A>::value;
     constexpr int R = LOGM - I0;
     constexpr int G = E >> R;
+    auto store_all = [&](auto mode) {
 #pragma unroll
-    for (int gg = 0; gg < G; ++gg) {
+      for (int gg = 0; gg < G; ++gg) {
 #pragma unroll
-      for (int e = 0; e < (1 << R); ++e) {
-        const V v = x[gg * (1 << R) + e];
-        g[(e << I0) | (gg * T + t)] = ar.inv_out(v, scale);
+        for (int e = 0; e < (1 << R); ++e)
+          g[(e << I0) | (gg * T + t)] = ar.inv_out(x[gg * (1 << R) + e], decltype(mode)::value);
       }
-    }
+    };
+    if (scale == 1)
+      store_all(std::integral_constant<int, 1>());
+    else if (scale == 2)
+      store_all(std::integral_constant<int, 2>());
+    else
+      store_all(std::integral_constant<int, 0>());
     // The last pass read its exchange before its own barrier, so the next
     // item may write LDS without another one.
   }

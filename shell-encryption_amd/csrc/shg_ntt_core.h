@@ -31,6 +31,17 @@
 namespace shg {
 
 
+// Makes the words in x materialize in VGPRs at this point.  Placed after
+// LDS reads whose slots other threads overwrite once a following barrier is
+// passed: the volatile asm cannot move past that barrier and consumes the
+// read values, so the reads cannot be sunk below it (the backend otherwise
+// may sink loads whose only uses lie beyond the barrier).
+template <class V, int E>
+__device__ __forceinline__ void pin_words(V (&x)[E]) {
+#pragma unroll
+  for (int i = 0; i < E; ++i) asm volatile("" : "+v"(x[i]));
+}
+
 // One padding word per 16 keeps strided u64 LDS accesses spread over banks.
 __device__ __forceinline__ int lpad(int i) { return i + (i >> 4); }
 __host__ __device__ constexpr int cpad(int i) { return i + (i >> 4); }
@@ -385,7 +396,10 @@ struct FwdPasses {
                                                 int h, const A& ar, const Hook& hook) {
     if constexpr (S0 > 0) {
       fwd_from_lds<LOGM, LOGE, S0, R>(x, (DB && ((P - 1) & 1)) ? lds1 : lds0, t);
-      if constexpr (!DB && kLast && !LAST_TO_LDS) __syncthreads();
+      if constexpr (kLast && !LAST_TO_LDS) {
+        pin_words(x);
+        if constexpr (!DB) __syncthreads();
+      }
     }
     fwd_pass<LOGM, LOGE, S0, R, HB>(x, tw, ar);
     if constexpr (!kLast) {
@@ -536,7 +550,10 @@ struct InvPasses {
                                                 const Hook& hook) {
     if constexpr (!(FIRST_IN_REGS && I0 == 0)) {
       inv_from_lds<LOGM, LOGE, I0, R>(x, lds, t);
-      if constexpr (I0 + R >= LOGM) __syncthreads();
+      if constexpr (I0 + R >= LOGM) {
+        pin_words(x);
+        __syncthreads();
+      }
     }
     inv_pass<LOGM, LOGE, I0, R>(x, tw, ar);
     if constexpr (I0 + R < LOGM) {
