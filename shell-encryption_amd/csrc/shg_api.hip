@@ -89,8 +89,10 @@ namespace {
 int ensure_scratch(shg_context* ctx, size_t bytes) {
   if (bytes <= ctx->scratch_size) return SHG_OK;
   if (ctx->scratch) {
-    hipStreamSynchronize(ctx->stream);
-    hipFree(ctx->scratch);
+    // Work queued on the stream may still use the old buffer.
+    hipError_t se = hipStreamSynchronize(ctx->stream);
+    if (se == hipSuccess) se = hipFree(ctx->scratch);
+    if (se != hipSuccess) return hip_fail(se, "releasing the previous scratch buffer");
     ctx->scratch = nullptr;
     ctx->scratch_size = 0;
   }
@@ -103,8 +105,10 @@ int ensure_scratch(shg_context* ctx, size_t bytes) {
 int ensure_buf(shg_context* ctx, shg_context::DevBuf& b, size_t bytes) {
   if (bytes <= b.size) return SHG_OK;
   if (b.p) {
-    hipStreamSynchronize(ctx->stream);
-    hipFree(b.p);
+    // Work queued on the stream may still use the old buffer.
+    hipError_t se = hipStreamSynchronize(ctx->stream);
+    if (se == hipSuccess) se = hipFree(b.p);
+    if (se != hipSuccess) return hip_fail(se, "releasing a previous work buffer");
     b.p = nullptr;
     b.size = 0;
   }
@@ -296,7 +300,7 @@ int shg_create(int device, shg_context** out) {
   ctx->d_capacity = 256;
   e = hipMalloc(&ctx->d_primes, sizeof(PrimeDev) * ctx->d_capacity);
   if (e != hipSuccess) {
-    hipStreamDestroy(ctx->own_stream);
+    (void)hipStreamDestroy(ctx->own_stream);  // already failing; report the allocation
     delete ctx;
     return hip_fail(e, "hipMalloc(prime table)");
   }
@@ -306,19 +310,24 @@ int shg_create(int device, shg_context** out) {
 
 int shg_destroy(shg_context* ctx) {
   if (!ctx) return SHG_OK;
-  hipSetDevice(ctx->device);
-  hipStreamSynchronize(ctx->stream);
+  // Teardown releases everything it can; the first failure is reported.
+  hipError_t first = hipSuccess;
+  auto note = [&](hipError_t e) {
+    if (first == hipSuccess) first = e;
+  };
+  note(hipSetDevice(ctx->device));
+  note(hipStreamSynchronize(ctx->stream));
   for (auto& p : ctx->primes)
-    if (p->d_tables) hipFree(p->d_tables);
-  if (ctx->d_primes) hipFree(ctx->d_primes);
-  if (ctx->scratch) hipFree(ctx->scratch);
+    if (p->d_tables) note(hipFree(p->d_tables));
+  if (ctx->d_primes) note(hipFree(ctx->d_primes));
+  if (ctx->scratch) note(hipFree(ctx->scratch));
   for (auto* b : {&ctx->exp_s, &ctx->exp_sub, &ctx->exp_ptrs, &ctx->exp_ptrs2, &ctx->pir_part,
                   &ctx->ks_planes})
-    if (b->p) hipFree(b->p);
-  for (auto& m : ctx->monomials) hipFree(m.second);
-  if (ctx->own_stream) hipStreamDestroy(ctx->own_stream);
+    if (b->p) note(hipFree(b->p));
+  for (auto& m : ctx->monomials) note(hipFree(m.second));
+  if (ctx->own_stream) note(hipStreamDestroy(ctx->own_stream));
   delete ctx;
-  return SHG_OK;
+  return first == hipSuccess ? SHG_OK : hip_fail(first, "context teardown");
 }
 
 int shg_set_stream(shg_context* ctx, void* stream) {
@@ -344,8 +353,9 @@ int shg_alloc(shg_context* ctx, size_t bytes, void** dptr) {
 int shg_free(shg_context* ctx, void* dptr) {
   if (!ctx) return fail(SHG_E_INVALID_ARGUMENT, "context must not be null.");
   if (!dptr) return SHG_OK;
-  hipStreamSynchronize(ctx->stream);
-  hipError_t e = hipFree(dptr);
+  // Work queued on the stream may still use the buffer.
+  hipError_t e = hipStreamSynchronize(ctx->stream);
+  if (e == hipSuccess) e = hipFree(dptr);
   return e == hipSuccess ? SHG_OK : hip_fail(e, "hipFree");
 }
 
@@ -404,10 +414,10 @@ int shg_prime_create(shg_context* ctx, uint64_t q, int log_n, int* prime) {
   hipError_t e = hipMalloc(&rec->d_tables, bytes);
   if (e != hipSuccess) return fail(SHG_E_NO_MEMORY, "twiddle table allocation failed");
   char* base = (char*)rec->d_tables;
-  hipMemcpy(base, fw.data(), 2 * n * 8, hipMemcpyHostToDevice);
-  hipMemcpy(base + 2 * n * 8, iw.data(), 2 * n * 8, hipMemcpyHostToDevice);
-  hipMemcpy(base + 4 * n * 8, fwd.data(), n * 8, hipMemcpyHostToDevice);
-  e = hipMemcpy(base + 5 * n * 8, iwd.data(), n * 8, hipMemcpyHostToDevice);
+  e = hipMemcpy(base, fw.data(), 2 * n * 8, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(base + 2 * n * 8, iw.data(), 2 * n * 8, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(base + 4 * n * 8, fwd.data(), n * 8, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(base + 5 * n * 8, iwd.data(), n * 8, hipMemcpyHostToDevice);
   if (e != hipSuccess) return hip_fail(e, "twiddle upload");
   PrimeDev& d = rec->dev;
   const Params& P = *rec->mp;
@@ -440,9 +450,15 @@ int shg_prime_create(shg_context* ctx, uint64_t q, int log_n, int* prime) {
     PrimeDev* bigger = nullptr;
     e = hipMalloc(&bigger, sizeof(PrimeDev) * ctx->d_capacity * 2);
     if (e != hipSuccess) return fail(SHG_E_NO_MEMORY, "prime table growth failed");
-    hipStreamSynchronize(ctx->stream);
-    hipMemcpy(bigger, ctx->d_primes, sizeof(PrimeDev) * ctx->d_capacity, hipMemcpyDeviceToDevice);
-    hipFree(ctx->d_primes);
+    e = hipStreamSynchronize(ctx->stream);
+    if (e == hipSuccess)
+      e = hipMemcpy(bigger, ctx->d_primes, sizeof(PrimeDev) * ctx->d_capacity,
+                    hipMemcpyDeviceToDevice);
+    if (e == hipSuccess) e = hipFree(ctx->d_primes);
+    if (e != hipSuccess) {
+      (void)hipFree(bigger);  // already failing; report the copy
+      return hip_fail(e, "prime table growth");
+    }
     ctx->d_primes = bigger;
     ctx->d_capacity *= 2;
   }
@@ -771,9 +787,14 @@ static int keyswitch_common(shg_context* ctx, const uint64_t* comps, int comps_p
         void* bigger = nullptr;
         if (hipMalloc(&bigger, need) != hipSuccess)
           return fail(SHG_E_NO_MEMORY, "scratch allocation of %zu bytes failed", need);
-        hipMemcpyAsync(bigger, src, (size_t)count * poly_words * 8, hipMemcpyDeviceToDevice, s);
-        hipStreamSynchronize(s);
-        hipFree(ctx->scratch);
+        hipError_t ge =
+            hipMemcpyAsync(bigger, src, (size_t)count * poly_words * 8, hipMemcpyDeviceToDevice, s);
+        if (ge == hipSuccess) ge = hipStreamSynchronize(s);
+        if (ge == hipSuccess) ge = hipFree(ctx->scratch);
+        if (ge != hipSuccess) {
+          (void)hipFree(bigger);  // already failing; report the copy
+          return hip_fail(ge, "scratch growth");
+        }
         ctx->scratch = bigger;
         ctx->scratch_size = need;
         src = (uint64_t*)bigger;
@@ -902,7 +923,7 @@ static int expand_monomial(shg_context* ctx, const int* primes, int num_limbs, i
                         ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   if (e != hipSuccess) {
-    hipFree(d);
+    (void)hipFree(d);  // already failing; report the first error
     return hip_fail(e, "monomial");
   }
   ctx->monomials[{key, level}] = d;
@@ -1050,7 +1071,7 @@ int shg_pir_inner_product(shg_context* ctx, const uint64_t* const* host_expanded
                                 hipMemcpyHostToDevice, ctx->stream);
   ProfScope prof(ctx, "pir_inner_product");
   if (e == hipSuccess)
-    e = shg::launch_pir_inner_product(d_ptrs, count, K, db_exported, num_columns,
+    e = shg::launch_pir_inner_product(d_ptrs, host_expanded[0], count, K, db_exported, num_columns,
                                       (uint64_t*)ctx->pir_part.p, ks, (uint64_t* const*)(d_ptrs + count),
                                       ctx->d_primes, lp, log_n, num_limbs, ctx->stream);
   return e == hipSuccess ? SHG_OK : hip_fail(e, "inner product");

@@ -189,8 +189,11 @@ hipError_t launch_expand_update(uint64_t* const* cts, int count, int m, const ui
                                 const PrimeDev* primes, const LimbPrimes& lp, int logn,
                                 int num_limbs, hipStream_t s);
 int pir_k_split(int count, int C, int K, int logn, int num_limbs);
-hipError_t launch_pir_inner_product(const uint64_t* const* expd, int count, int K,
-                                    const uint64_t* db_x, int C, uint64_t* part, int KS,
+// expd: device array of the count expanded-query pointers; expd0: the first
+// of them as known on the host (the single-query kernel takes it directly).
+hipError_t launch_pir_inner_product(const uint64_t* const* expd, const uint64_t* expd0,
+                                    int count, int K, const uint64_t* db_x, int C,
+                                    uint64_t* part, int KS,
                                     uint64_t* const* out, const PrimeDev* primes,
                                     const LimbPrimes& lp, int logn, int num_limbs, hipStream_t s);
 hipError_t launch_export_words(const uint64_t* in, uint64_t* out, long long polys,
