@@ -628,8 +628,34 @@ int shg_mod_reduce_lsb(shg_context* ctx, uint64_t* polys, uint64_t t, const uint
     args.ql_inv[i] = host_ql_inv[i];
   }
   const size_t n = (size_t)1 << log_n;
-  const size_t need = (size_t)count * n * 8 * (1 + nr);
   std::lock_guard<std::mutex> lock(ctx->mu);
+  if (log_n >= shg::kBatchMinLogN && log_n <= shg::kBatchMaxLogN &&
+      num_limbs <= shg::kBatchMaxPolyLimbs) {
+    // One workgroup per polynomial: INTT of the last limb into its scratch,
+    // then lift, NTT and combine for each other limb, all in one launch.
+    const int v = resolve_variant(ctx, SHG_NTT_AUTO, primes, num_limbs, log_n,
+                                  (long long)count * num_limbs) & 3;
+    shg::PolyJobDev uni{};
+    uni.data = polys;
+    uni.log_n = log_n;
+    uni.num_limbs = num_limbs;
+    uni.is_ntt = is_ntt ? 1 : 0;
+    for (int i = 0; i < num_limbs; ++i) uni.prime[i] = primes[i];
+    uni.k_mod_ql = args.k_mod_ql;
+    for (int i = 0; i < nr; ++i) {
+      uni.r_mod_qi[i] = args.r_mod_qi[i];
+      uni.t_mod_qi[i] = args.t_mod_qi[i];
+      uni.ql_inv[i] = args.ql_inv[i];
+    }
+    const int grid = shg::modreduce_uniform_grid(v, log_n, count);
+    rc = ensure_scratch(ctx, ((size_t)grid << 16) * 8);
+    if (rc) return rc;
+    hipError_t e = shg::launch_modreduce_uniform(uni, (size_t)num_limbs * n, count, v,
+                                                 ctx->d_primes, (uint64_t*)ctx->scratch, grid,
+                                                 ctx->stream);
+    return e == hipSuccess ? SHG_OK : hip_fail(e, "mod_reduce launch");
+  }
+  const size_t need = (size_t)count * n * 8 * (1 + nr);
   rc = ensure_scratch(ctx, need);
   if (rc) return rc;
   uint64_t* delta_l = (uint64_t*)ctx->scratch;

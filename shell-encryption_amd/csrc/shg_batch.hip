@@ -203,11 +203,14 @@ __global__ void __launch_bounds__(kT)
 // other limb i becomes (r * a_i + t * NTT(lift_i(delta))) * q_l^-1 with the
 // balanced lift of ConvertModulusBalanced (modulus_conversion.h:30-83) and
 // r = q_l mod t.  Limbs 0..L-2 hold the result; the last limb is left as
-// it was.  Each workgroup owns 2^16 words of scratch.
-template <class A>
+// it was.  Each workgroup owns 2^16 words of scratch.  UNIFORM: every job
+// is `uni` with its data at uni.data + j * stride (one call over equal
+// polynomials; the job then lives in the kernel arguments).
+template <class A, bool UNIFORM>
 __global__ void __launch_bounds__(kT)
-    batch_modreduce_kernel(const PolyJobDev* __restrict__ jobs, int num_jobs,
-                           const PrimeDev* __restrict__ primes, uint64_t* __restrict__ scratch) {
+    batch_modreduce_kernel(const PolyJobDev* __restrict__ jobs, int num_jobs, PolyJobDev uni,
+                           size_t stride, const PrimeDev* __restrict__ primes,
+                           uint64_t* __restrict__ scratch) {
   extern __shared__ __align__(16) unsigned char smem[];
   auto* lds = reinterpret_cast<typename A::V*>(smem);
   const gmut<uint64_t> s0 = (gmut<uint64_t>)(scratch + ((size_t)blockIdx.x << 16));
@@ -215,11 +218,11 @@ __global__ void __launch_bounds__(kT)
   for (int j = blockIdx.x; j < num_jobs; j += gridDim.x) {
     int t = threadIdx.x;
     asm volatile("" : "+v"(t));
-    const PolyJobDev& job = jobs[j];
+    const PolyJobDev& job = UNIFORM ? uni : jobs[j];
     const int L = job.num_limbs;
     const int log_n = job.log_n;
     const int n = 1 << log_n;
-    const gmut<uint64_t> a = (gmut<uint64_t>)job.data;
+    const gmut<uint64_t> a = (gmut<uint64_t>)(UNIFORM ? uni.data + (size_t)j * stride : job.data);
     const PrimeDev& Pl = primes[job.prime[L - 1]];
     const gmut<uint64_t> al = a + (size_t)(L - 1) * n;
     const uint64_t k = job.k_mod_ql;
@@ -260,8 +263,8 @@ size_t batch_shmem_bytes(int max_log_n) {
 
 const void* batch_kernel_ptr(int variant, bool modreduce) {
   if (modreduce)
-    return variant == SHG_NTT_F64 ? (const void*)batch_modreduce_kernel<F64Arith>
-                                  : (const void*)batch_modreduce_kernel<IntArith>;
+    return variant == SHG_NTT_F64 ? (const void*)batch_modreduce_kernel<F64Arith, false>
+                                  : (const void*)batch_modreduce_kernel<IntArith, false>;
   return variant == SHG_NTT_F64 ? (const void*)batch_limbs_kernel<F64Arith>
                                 : (const void*)batch_limbs_kernel<IntArith>;
 }
@@ -270,12 +273,48 @@ hipError_t launch_batch_modreduce(const PolyJobDev* jobs, int num_jobs, int vari
                                   const PrimeDev* primes, uint64_t* scratch, int grid,
                                   size_t shmem, hipStream_t s) {
   if (num_jobs <= 0) return hipSuccess;
+  const PolyJobDev none{};
   if (variant == SHG_NTT_F64)
-    hipLaunchKernelGGL(batch_modreduce_kernel<F64Arith>, dim3(grid), dim3(kT), shmem, s, jobs,
-                       num_jobs, primes, scratch);
+    hipLaunchKernelGGL((batch_modreduce_kernel<F64Arith, false>), dim3(grid), dim3(kT), shmem, s,
+                       jobs, num_jobs, none, (size_t)0, primes, scratch);
   else
-    hipLaunchKernelGGL(batch_modreduce_kernel<IntArith>, dim3(grid), dim3(kT), shmem, s, jobs,
-                       num_jobs, primes, scratch);
+    hipLaunchKernelGGL((batch_modreduce_kernel<IntArith, false>), dim3(grid), dim3(kT), shmem, s,
+                       jobs, num_jobs, none, (size_t)0, primes, scratch);
+  return hipGetLastError();
+}
+
+static const void* modreduce_uniform_kernel(int variant) {
+  return variant == SHG_NTT_F64 ? (const void*)batch_modreduce_kernel<F64Arith, true>
+                                : (const void*)batch_modreduce_kernel<IntArith, true>;
+}
+
+int modreduce_uniform_grid(int variant, int log_n, int count) {
+  const void* k = modreduce_uniform_kernel(variant);
+  const size_t shmem = batch_shmem_bytes(log_n);
+  if (shmem > 65536) set_max_dynamic_shmem(k, shmem);
+  int dev = 0, cus = 0, per_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      cus <= 0)
+    cus = 1;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, kT, shmem) != hipSuccess ||
+      per_cu <= 0)
+    per_cu = 1;
+  const long long cap = (long long)cus * per_cu;
+  return (int)(cap < count ? cap : count);
+}
+
+hipError_t launch_modreduce_uniform(const PolyJobDev& uni, size_t stride, int count, int variant,
+                                    const PrimeDev* primes, uint64_t* scratch, int grid,
+                                    hipStream_t s) {
+  if (count <= 0) return hipSuccess;
+  const size_t shmem = batch_shmem_bytes(uni.log_n);
+  if (variant == SHG_NTT_F64)
+    hipLaunchKernelGGL((batch_modreduce_kernel<F64Arith, true>), dim3(grid), dim3(kT), shmem, s,
+                       nullptr, count, uni, stride, primes, scratch);
+  else
+    hipLaunchKernelGGL((batch_modreduce_kernel<IntArith, true>), dim3(grid), dim3(kT), shmem, s,
+                       nullptr, count, uni, stride, primes, scratch);
   return hipGetLastError();
 }
 

@@ -178,6 +178,14 @@ const void* batch_kernel_ptr(int variant, bool modreduce);
 hipError_t launch_batch_modreduce(const PolyJobDev* jobs, int num_jobs, int variant,
                                   const PrimeDev* primes, uint64_t* scratch, int grid,
                                   size_t shmem, hipStream_t s);
+// ModReduceLsb of `count` equal polynomials (job `uni`, data at uni.data +
+// j * stride) with the fused per-polynomial kernel: the grid from
+// modreduce_uniform_grid (which also sets the kernel's LDS limit), scratch
+// of 2^16 words per workgroup.
+int modreduce_uniform_grid(int variant, int log_n, int count);
+hipError_t launch_modreduce_uniform(const PolyJobDev& uni, size_t stride, int count, int variant,
+                                    const PrimeDev* primes, uint64_t* scratch, int grid,
+                                    hipStream_t s);
 hipError_t launch_batch_limbs(const LimbJobDev* jobs, int num_jobs, int op, int variant,
                               const PrimeDev* primes, int grid, size_t shmem, hipStream_t s);
 

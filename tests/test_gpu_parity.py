@@ -165,7 +165,7 @@ def test_substitute(ctx, log_n, power):
 
 
 @pytest.mark.parametrize("is_ntt", [True, False])
-@pytest.mark.parametrize("log_n,L", [(9, 3), (14, 8)])
+@pytest.mark.parametrize("log_n,L", [(9, 3), (11, 2), (12, 4), (14, 8), (15, 3)])
 def test_mod_reduce_lsb(ctx, is_ntt, log_n, L):
     qs, rns = _rns(log_n, L)
     pids = ctx.primes(qs, log_n)
@@ -177,6 +177,24 @@ def test_mod_reduce_lsb(ctx, is_ntt, log_n, L):
     got = d.download()[:, : L - 1]
     exp = np.stack([rns.mod_reduce_lsb(x[c], t, ql_inv, is_ntt) for c in range(2)])
     assert np.array_equal(got, exp)
+
+
+@pytest.mark.parametrize("is_ntt", [True, False])
+def test_mod_reduce_lsb_many(ctx, is_ntt):
+    """More polynomials than resident workgroups: the fused per-polynomial
+    kernel loops; first, middle and last polynomials against the oracle."""
+    log_n, L, count = 12, 4, 600
+    qs, rns = _rns(log_n, L)
+    pids = ctx.primes(qs, log_n)
+    t = 65537
+    x = rns.uniform(22 + int(is_ntt), count)
+    d = ctx.array(x)
+    ql_inv = rns.ql_inv()
+    ctx.mod_reduce_lsb(d, t, ql_inv, is_ntt, count, pids)
+    got = d.download()
+    for c in (0, count // 2, count - 1):
+        assert np.array_equal(got[c, : L - 1], rns.mod_reduce_lsb(x[c], t, ql_inv, is_ntt)), c
+        assert np.array_equal(got[c, L - 1], x[c, L - 1]), c  # the dropped limb is left as it was
 
 
 def test_tensor_product(ctx):
