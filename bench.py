@@ -409,6 +409,19 @@ def main():
                 "avg_launch_ms": round(ks_avg_ms, 4),
                 "algorithmic_bytes_per_launch": ks_bytes}
 
+    # The key switch is bound by FP64 vector arithmetic, not HBM: its
+    # algorithmic FP64 operations per launch (a Cooley-Tukey butterfly = one
+    # exact FMA-based product of 6 operations + 2 additions; stage 0 of a
+    # half-limb = product + 1 addition; a key multiply-accumulate term = 7)
+    # against MI355X's FP64 vector issue rate (78.6 TFLOP/s counting an FMA
+    # as two flops = 39.3 T operations/s).
+    half = n // 2
+    bfly = BATCH * L * k * (2 * (half // 2) * (LOG_N - 1) * 8 + 2 * half * 7)
+    mac = BATCH * L * k * n * 2 * 7
+    ops = bfly + mac
+    fp64_valu = {"achieved": round(ops / (ks_avg_ms * 1e-3) / 1e12, 2), "peak": 39.3,
+                 "unit": "T FP64 ops/s", "frac": round(ops / (ks_avg_ms * 1e-3) / 39.3e12, 4),
+                 "ops_per_launch": ops}
     out = {
         "metric": "key-switches/s & NTTs/s at n=2^14,L=8 batched vs plain (bit-exact); p50/p99",
         "value": round(value, 1), "unit": "key-switches/s", "n_gpus": 1,
@@ -421,6 +434,7 @@ def main():
         "latency_ms": {"p50": round(percentile(lat, 50), 4), "p99": round(percentile(lat, 99), 4)},
         "bit_exact": bit_exact,
         "roofline": roofline,
+        "fp64_valu": fp64_valu,
         "keyswitch_intt_ms_per_step": round(intt_ms / max(intt_launches, 1), 4),
     }
 

@@ -86,6 +86,39 @@ struct shg_context {
 
 namespace {
 
+uint64_t mulmod_host(uint64_t a, uint64_t b, uint64_t q) {
+  return (uint64_t)((unsigned __int128)a * b % q);
+}
+uint64_t powmod_host(uint64_t a, uint64_t e, uint64_t q) {
+  uint64_t r = 1 % q;
+  a %= q;
+  for (; e; e >>= 1, a = mulmod_host(a, a, q))
+    if (e & 1) r = mulmod_host(r, a, q);
+  return r;
+}
+
+// The FP64 constants of a ModReduceLsb job (PolyJobDev::f_*) from its
+// Montgomery-form ones, for primes q (< 2^50, so exact as doubles):
+// f_kl = k * n^-1 * 2^-64 mod q_l, f_c1[i] = r * ql_inv * 2^-128 mod q_i,
+// f_c2[i] = t * ql_inv * 2^-64 mod q_i.
+void fill_modreduce_f64(shg_context* ctx, shg::PolyJobDev& d, uint64_t k) {
+  const int L = d.num_limbs;
+  auto rinv = [](uint64_t q) {
+    const uint64_t r = (uint64_t)(((unsigned __int128)1 << 64) % q);
+    return powmod_host(r, q - 2, q);
+  };
+  const uint64_t ql = ctx->primes[d.prime[L - 1]]->q;
+  const uint64_t ninv = powmod_host((uint64_t)1 << d.log_n, ql - 2, ql);
+  d.f_kl = (double)mulmod_host(mulmod_host(k % ql, ninv, ql), rinv(ql), ql);
+  for (int i = 0; i < L - 1; ++i) {
+    const uint64_t qi = ctx->primes[d.prime[i]]->q;
+    const uint64_t ri = rinv(qi);
+    const uint64_t base = mulmod_host(d.ql_inv[i], ri, qi);
+    d.f_c1[i] = (double)mulmod_host(mulmod_host(d.r_mod_qi[i], base, qi), ri, qi);
+    d.f_c2[i] = (double)mulmod_host(d.t_mod_qi[i], base, qi);
+  }
+}
+
 int ensure_scratch(shg_context* ctx, size_t bytes) {
   if (bytes <= ctx->scratch_size) return SHG_OK;
   if (ctx->scratch) {
@@ -647,6 +680,7 @@ int shg_mod_reduce_lsb(shg_context* ctx, uint64_t* polys, uint64_t t, const uint
       uni.t_mod_qi[i] = args.t_mod_qi[i];
       uni.ql_inv[i] = args.ql_inv[i];
     }
+    if (v == SHG_NTT_F64) fill_modreduce_f64(ctx, uni, k);
     const int grid = shg::modreduce_uniform_grid(v, log_n, count);
     rc = ensure_scratch(ctx, ((size_t)grid << 16) * 8);
     if (rc) return rc;
@@ -1298,6 +1332,7 @@ int shg_batch_mod_reduce_plan_create(shg_context* ctx, const shg_poly_job* host_
         d.t_mod_qi[l] = ModInt::ImportInt(t, Pi).value().GetMontgomeryRepresentation();
         d.ql_inv[l] = j.host_ql_inv[l];
       }
+      if (f64_ok) fill_modreduce_f64(ctx, d, k);
       max_log_n = j.log_n > max_log_n ? j.log_n : max_log_n;
       jobs[i] = d;
     }

@@ -14,6 +14,8 @@
 // graph.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "shg_device.h"
 #include "shg_kernels.h"
 #include "shg_ntt_core.h"
@@ -226,6 +228,41 @@ __global__ void __launch_bounds__(kT)
     const PrimeDev& Pl = primes[job.prime[L - 1]];
     const gmut<uint64_t> al = a + (size_t)(L - 1) * n;
     const uint64_t k = job.k_mod_ql;
+    if constexpr (std::is_same<A, F64Arith>::value) {
+      if (job.is_ntt) {
+        // FP64 form: delta = exported INTT(k * a_l), balanced, kept as a
+        // double; each other limb is a_i * c1 + NTT(delta mod q_i) * c2.
+        A arl(Pl);
+        arl.ninv = job.f_kl;
+        arl.ninvq = job.f_kl * arl.qinv;
+        const uint64_t half_ql = Pl.q >> 1;
+        inv_io(log_n, arl, lds, t, s0, [&](int p) { return (uint64_t)al[p]; },
+               [&](int p, uint64_t v) {
+                 const double d = u52_to_f64(v) - (v > half_ql ? arl.q : 0.0);
+                 s0[p] = (uint64_t)__double_as_longlong(d);
+               });
+        __syncthreads();
+        for (int i = 0; i < L - 1; ++i) {
+          const A ari(primes[job.prime[i]]);
+          const gmut<uint64_t> ai = a + (size_t)i * n;
+          const double c1 = job.f_c1[i], c1q = c1 * ari.qinv;
+          const double c2 = job.f_c2[i], c2q = c2 * ari.qinv;
+          fwd_io(log_n, ari, lds, t, s1,
+                 [&](int p) {
+                   return f64_reduce(__longlong_as_double((long long)(uint64_t)s0[p]), ari.q,
+                                     ari.qinv);
+                 },
+                 [&](int p, uint64_t v) {
+                   const double sum = f64_mulmod(u52_to_f64(ai[p]), c1, c1q, ari.q) +
+                                      f64_mulmod(u52_to_f64(v), c2, c2q, ari.q);
+                   ai[p] = f64_to_u52(f64_canon(f64_reduce(sum, ari.q, ari.qinv), ari.q));
+                 });
+        }
+        // delta (s0) is rewritten by the next job.
+        __syncthreads();
+        continue;
+      }
+    }
     if (job.is_ntt) {
       const A arl(Pl);
       inv_io(log_n, arl, lds, t, s0, [&](int p) { return mont_mul(al[p], k, Pl); },

@@ -171,6 +171,14 @@ struct PolyJobDev {
   uint64_t r_mod_qi[kBatchMaxPolyLimbs];
   uint64_t t_mod_qi[kBatchMaxPolyLimbs];
   uint64_t ql_inv[kBatchMaxPolyLimbs];
+  // FP64 form of the same constants (q < 2^50), plain residues: the
+  // inverse transform of the last limb is scaled by k * n^-1 * 2^-64 (giving
+  // the exported k * a_l directly), and limb i becomes
+  // a_i * c1[i] + NTT(lift_i(delta)) * c2[i] with c1 = r * q_l^-1 * 2^-128
+  // and c2 = t * q_l^-1 * 2^-64 (Montgomery reps r, t, q_l^-1).
+  double f_kl;
+  double f_c1[kBatchMaxPolyLimbs];
+  double f_c2[kBatchMaxPolyLimbs];
 };
 size_t batch_shmem_bytes(int max_log_n);
 const void* batch_kernel_ptr(int variant, bool modreduce);
