@@ -542,7 +542,7 @@ static int ntt_common(shg_context* ctx, bool inverse, uint64_t* polys, int count
   if (count == 0) return SHG_OK;
   if (!aligned16(polys))
     return fail(SHG_E_INVALID_ARGUMENT, "polynomial data must be a non-null 16-byte aligned pointer.");
-  if (variant < 0 || (variant & 3) == 3 || (variant & ~31) != 0)
+  if (variant < 0 || (variant & 3) == 3 || (variant & ~63) != 0)
     return fail(SHG_E_INVALID_ARGUMENT, "unknown transform variant %d.", variant);
   const long long rows = (long long)count * num_limbs;
   if ((rows << (log_n > shg::kMaxLogM ? log_n - shg::kMaxLogM : 0)) > 0x7fffffffLL)

@@ -19,10 +19,13 @@ constexpr int kMaxLogM = 14;
 // (SHG_NTT_INT_SHOUP or SHG_NTT_F64), the alternate register tiling of
 // 2^12..2^14-word slices in kNttCfgAlt, next-item prefetch (FP64 only) in
 // kNttCfgPrefetch, forward transforms of 2^14-word limbs split over two
-// workgroups in kNttCfgSplit.  A bare variant is a valid configuration.
+// workgroups in kNttCfgSplit, FP64 forward transforms of 2^12..2^14-word
+// limbs with one workgroup-wide exchange (the rest inside each wave) in
+// kNttCfgWave.  A bare variant is a valid configuration.
 constexpr int kNttCfgAlt = 4;
 constexpr int kNttCfgPrefetch = 8;
 constexpr int kNttCfgSplit = 16;
+constexpr int kNttCfgWave = 32;
 hipError_t launch_ntt(bool inverse, int cfg, uint64_t* data, const PrimeDev* primes,
                       const LimbPrimes& lp, int rows, int num_limbs, int logn, int max_logm,
                       hipStream_t s, int inv_mode = 1);

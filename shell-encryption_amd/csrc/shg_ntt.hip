@@ -16,6 +16,7 @@
 #include "shg_device.h"
 #include "shg_kernels.h"
 #include "shg_ntt_core.h"
+#include "shg_ntt_wave.h"
 
 // Per-thread word count (log2) for 2^14-word slices: 16 words with 1024
 // threads (four register passes) by default; 32 words with 512 threads
@@ -31,6 +32,15 @@
 // the current item's stores.
 #ifndef SHG_NTT_PF_AT
 #define SHG_NTT_PF_AT 0
+#endif
+// The wave-exchange forward kernel loads the next limb into VGPRs at the
+// start of the current one (its uniform twiddles come through the scalar
+// cache, so only the per-lane twiddles of the last two passes wait behind
+// those loads).  2^14 limbs, 8192 per launch: 0.71 ms without either, 0.65
+// with scalar twiddles, 0.64 with both; the 16-words-per-thread kernel
+// 0.68 ms (one run).
+#ifndef SHG_NTT_WAVE_PF
+#define SHG_NTT_WAVE_PF 1
 #endif
 
 namespace shg {
@@ -199,6 +209,103 @@ __global__ void __launch_bounds__(512, 4)
     // straight from VGPRs, and the pass's own barrier frees LDS.
     FwdPasses<LOGM, LOGE, 0, A, false, 1>::run(x, lds, lds, t, h, ar);
     store_fwd_last<LOGM, LOGE>(g + h * H, x, t, ar);
+  }
+}
+
+// Forward transform of 2^(10+LW)-word limbs with one workgroup-wide LDS
+// exchange per limb (shg_ntt_wave.h), FP64 arithmetic, persistent over
+// limbs like ntt_fwd_kernel.
+template <int LW, class A>
+__global__ void __launch_bounds__(64 << LW)
+    ntt_fwd_wave_kernel(uint64_t* __restrict__ data, const PrimeDev* __restrict__ primes,
+                        LimbPrimes lp, int num_limbs, int items) {
+  using Lay = WaveLayout<LW>;
+  constexpr int LOGM = Lay::LOGM;
+  extern __shared__ __align__(16) unsigned char smem[];
+  double* lds = reinterpret_cast<double*>(smem);
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  using T1 = WavePassTw<LW, 1, 3, 0>;
+  using T2 = WavePassTw<LW, 2, LW - 1, 0>;
+  using T3 = WavePassTw<LW, 3, 3, 0>;
+  using T4 = WavePassTw<LW, 4, 1, 0>;
+  uint64_t pf[16];  // with SHG_NTT_WAVE_PF: the words of the next limb
+  if (SHG_NTT_WAVE_PF && (int)blockIdx.x < items) {
+    const int tp1 = Lay::thread_part(1, lane, wave);
+    const uint64_t* g0 = data + ((size_t)blockIdx.x << LOGM);
+#pragma unroll
+    for (int e = 0; e < 16; ++e) pf[e] = g0[tp1 + Lay::reg_part(1, e)];
+  }
+  for (int item = blockIdx.x; item < items; item += gridDim.x) {
+    // An opaque copy of the lane index per limb keeps the per-thread
+    // twiddle and LDS offsets inside the loop: hoisted out of it they would
+    // all stay live across the loop and spill.
+    int lane_o = lane;
+    asm volatile("" : "+v"(lane_o));
+    const A ar(primes[lp.id[item % num_limbs]]);
+    uint64_t* g = data + ((size_t)item << LOGM);
+    const int tp1 = Lay::thread_part(1, lane_o, wave);
+    double tw1[T1::N];
+    wave_pass_tw<LW, 1, 3, 0, true>(tw1, 0, ar);
+    double x[16];
+    if constexpr (SHG_NTT_WAVE_PF) {
+#pragma unroll
+      for (int e = 0; e < 16; ++e) x[e] = ar.load(pf[e]);
+      const int next = item + gridDim.x;
+      if (next < items) {
+        const uint64_t* gn = data + ((size_t)next << LOGM);
+#pragma unroll
+        for (int e = 0; e < 16; ++e) pf[e] = gn[tp1 + Lay::reg_part(1, e)];
+      }
+    } else {
+#pragma unroll
+      for (int e = 0; e < 16; ++e) x[e] = ar.load(g[tp1 + Lay::reg_part(1, e)]);
+    }
+    wave_pass<LW, 1, 3, 0>(x, tw1, ar);
+    // Each pass's twiddles are issued before the exchange that precedes it
+    // (and not earlier: the scheduling barriers keep one pass's twiddles
+    // live at a time).
+    __builtin_amdgcn_sched_barrier(0);
+    double tw2[T2::N];
+    wave_pass_tw<LW, 2, LW - 1, 0, true>(tw2, wave << 10, ar);
+    // Every wave has finished the previous limb's in-wave exchanges.
+    __syncthreads();
+    wave_to_lds<LW, 1>(x, lds, tp1);
+    __syncthreads();
+    const int tp2 = Lay::thread_part(2, lane_o, wave);
+    wave_from_lds<LW, 2>(x, lds, tp2);
+    wave_pass<LW, 2, LW - 1, 0>(x, tw2, ar);
+    // From here on the wave only touches the slots of its own words.
+    const int tp3 = Lay::thread_part(3, lane_o, wave);
+    __builtin_amdgcn_sched_barrier(0);
+    double tw3[T3::N];
+    wave_pass_tw<LW, 3, 3, 0>(tw3, tp3, ar);
+    wave_to_lds<LW, 2>(x, lds, tp2);
+    wave_sync_lds();
+    wave_from_lds<LW, 3>(x, lds, tp3);
+    wave_pass<LW, 3, 3, 0>(x, tw3, ar);
+    const int tp4 = Lay::thread_part(4, lane_o, wave);
+    __builtin_amdgcn_sched_barrier(0);
+    double tw4[T4::N];
+    wave_pass_tw<LW, 4, 1, 0>(tw4, tp4, ar);
+    wave_to_lds<LW, 3>(x, lds, tp3);
+    wave_sync_lds();
+    wave_from_lds<LW, 4>(x, lds, tp4);
+    // The next limb's exchange writes these slots after its first barrier.
+    pin_words(x);
+    wave_pass<LW, 4, 1, 0>(x, tw4, ar);
+    // Registers (e & 3) hold consecutive words; e >> 2 selects index bits 8, 9.
+#pragma unroll
+    for (int gg = 0; gg < 4; ++gg) {
+      u64x2* o = reinterpret_cast<u64x2*>(g + tp4 + Lay::reg_part(4, gg * 4));
+#pragma unroll
+      for (int e2 = 0; e2 < 4; e2 += 2) {
+        u64x2 v;
+        v.x = ar.fwd_out(x[gg * 4 + e2]);
+        v.y = ar.fwd_out(x[gg * 4 + e2 + 1]);
+        o[e2 / 2] = v;
+      }
+    }
   }
 }
 
@@ -523,6 +630,18 @@ static hipError_t launch_fwd_split(uint64_t* data, const PrimeDev* primes, const
   return hipGetLastError();
 }
 
+template <int LW>
+static hipError_t launch_fwd_wave(uint64_t* data, const PrimeDev* primes, const LimbPrimes& lp,
+                                  int rows, int num_limbs, hipStream_t stream) {
+  constexpr int T = 64 << LW;
+  const size_t shmem = (size_t)lds_words<10 + LW>() * 8;
+  auto k = ntt_fwd_wave_kernel<LW, F64Arith>;
+  if (shmem > 65536) set_max_dynamic_shmem((const void*)k, shmem);
+  const int grid = persistent_grid((const void*)k, T, shmem, rows, num_limbs);
+  hipLaunchKernelGGL(k, dim3(grid), dim3(T), shmem, stream, data, primes, lp, num_limbs, rows);
+  return hipGetLastError();
+}
+
 template <int LOGM, int LOGE, class A, bool PF>
 static hipError_t launch_fused(bool inverse, uint64_t* data, const PrimeDev* primes,
                                const LimbPrimes& lp, int rows, int num_limbs, int logn,
@@ -609,6 +728,15 @@ hipError_t launch_ntt(bool inverse, int cfg, uint64_t* data, const PrimeDev* pri
                       const LimbPrimes& lp, int rows, int num_limbs, int logn, int max_logm,
                       hipStream_t s, int inv_mode) {
   const bool alt = (cfg & kNttCfgAlt) != 0;
+  // FP64 forward transforms of 2^14 limbs default to the wave-exchange
+  // kernel (the prefetching and alternate tilings keep the others).
+  const bool wave_default = logn == 14 && (cfg & (kNttCfgAlt | kNttCfgPrefetch | kNttCfgSplit)) == 0;
+  if (!inverse && ((cfg & kNttCfgWave) || wave_default) && (cfg & 3) == SHG_NTT_F64 &&
+      logn >= 12 && logn <= 14 && logn <= max_logm) {
+    if (logn == 12) return launch_fwd_wave<2>(data, primes, lp, rows, num_limbs, s);
+    if (logn == 13) return launch_fwd_wave<3>(data, primes, lp, rows, num_limbs, s);
+    return launch_fwd_wave<4>(data, primes, lp, rows, num_limbs, s);
+  }
   if (!inverse && logn == 14 && max_logm >= 13 && (cfg & kNttCfgSplit)) {
     if ((cfg & 3) == SHG_NTT_F64)
       return launch_fwd_split<F64Arith>(data, primes, lp, rows, num_limbs, s);

@@ -1,0 +1,184 @@
+// Forward NTT of a 2^(10+LW)-word slice with one workgroup-wide exchange per
+// limb: 2^LW waves x 64 lanes x 16 words per lane.  Four register passes:
+//
+//   pass  stages (index bits)      registers   lanes            waves
+//   1     top 4 bits               9+LW..6+LW  5..0             6+LW-1..6
+//   2     6+LW-1..6 (LW stages)    9..6        5..0             top LW bits
+//   3     5..2                     5..2        9..6, 1, 0       top LW bits
+//   4     1..0                     9, 8, 1, 0  7..2             top LW bits
+//
+// Pass 1's words are loaded coalesced (lanes on the low bits) and its
+// twiddles are uniform over the workgroup; the exchange to pass 2's layout
+// goes through LDS between two barriers, after which each wave owns the
+// words whose top LW bits are its index, so the exchanges to passes 3 and 4
+// stay inside the wave (LDS slots the wave itself read, ordered by a wave
+// barrier): between one limb's exchange and the next, waves run
+// independently and their HBM phases overlap other waves' arithmetic.
+// Pass 2's twiddles are uniform per wave.  Word i always sits at LDS slot
+// lpad(i); lane, wave and register parts of an index occupy disjoint bits,
+// so lpad(index) = lpad(lane/wave part) + lpad(register part).
+#ifndef SHG_NTT_WAVE_H_
+#define SHG_NTT_WAVE_H_
+
+#include "shg_ntt_core.h"
+
+namespace shg {
+
+// Index bit of register bit k (0..3) and of lane bit k (0..5) in each layout.
+template <int LW>
+struct WaveLayout {
+  static constexpr int LOGM = 10 + LW;
+  __host__ __device__ static constexpr int reg(int pass, int k) {
+    return pass == 1 ? LOGM - 4 + k : pass == 2 ? 6 + k : pass == 3 ? 2 + k
+                                                                      : (k < 2 ? k : 6 + k);
+  }
+  __host__ __device__ static constexpr int lane(int pass, int k) {
+    return pass <= 2 ? k : pass == 3 ? (k < 2 ? k : 4 + k) : 2 + k;
+  }
+  __host__ __device__ static constexpr int wave(int pass, int k) {
+    return pass == 1 ? 6 + k : 10 + k;
+  }
+  // Register part of the index of register word e.
+  __host__ __device__ static constexpr int reg_part(int pass, int e) {
+    return ((e & 1) << reg(pass, 0)) | (((e >> 1) & 1) << reg(pass, 1)) |
+           (((e >> 2) & 1) << reg(pass, 2)) | (((e >> 3) & 1) << reg(pass, 3));
+  }
+  // Lane and wave part of a thread's indices.
+  __device__ static int thread_part(int pass, int lane_id, int wave_id) {
+    int v = 0;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) v |= ((lane_id >> k) & 1) << lane(pass, k);
+#pragma unroll
+    for (int k = 0; k < LW; ++k) v |= ((wave_id >> k) & 1) << wave(pass, k);
+    return v;
+  }
+};
+
+// Twiddles of one pass: stages on register bits kb = KHI down to KLO (index
+// bits reg(P, kb)); global stage s = LOGM - 1 - index bit.  The pair (e,
+// e | 1 << kb) uses psi_bitrev[2^s + (index >> (bit + 1))], which depends on
+// the lane/wave part and on e's register bits above the stage's bit only:
+// stage kb has 2^nhigh(kb) of them, stored from offset base(kb).
+template <int LW, int P, int KHI, int KLO>
+struct WavePassTw {
+  using Lay = WaveLayout<LW>;
+  __host__ __device__ static constexpr int nhigh(int kb) {
+    int c = 0;
+    for (int k = 0; k < 4; ++k) c += Lay::reg(P, k) > Lay::reg(P, kb) ? 1 : 0;
+    return c;
+  }
+  // j-th register bit (increasing k) above stage kb's bit.
+  __host__ __device__ static constexpr int high(int kb, int j) {
+    for (int k = 0; k < 4; ++k)
+      if (Lay::reg(P, k) > Lay::reg(P, kb) && j-- == 0) return k;
+    return 0;
+  }
+  __host__ __device__ static constexpr int base(int kb) {
+    int b = 0;
+    for (int k = KHI; k > kb; --k) b += 1 << nhigh(k);
+    return b;
+  }
+  static constexpr int N = base(KLO - 1);
+  // Position of register word e's twiddle for stage kb.
+  __host__ __device__ static constexpr int slot(int kb, int e) {
+    int c = 0;
+    for (int j = 0; j < nhigh(kb); ++j) c |= ((e >> high(kb, j)) & 1) << j;
+    return base(kb) + c;
+  }
+  // Register part of the index for combination c of stage kb's high bits.
+  __host__ __device__ static constexpr int combo_part(int kb, int c) {
+    int v = 0;
+    for (int j = 0; j < nhigh(kb); ++j) v |= ((c >> j) & 1) << Lay::reg(P, high(kb, j));
+    return v;
+  }
+};
+
+template <int LW, int P, int KHI, int KLO, class A>
+__device__ __forceinline__ void wave_pass_tw_vec(double* tw, int tw_tp, const A& ar) {
+  using PT = WavePassTw<LW, P, KHI, KLO>;
+  constexpr int LOGM = WaveLayout<LW>::LOGM;
+#pragma unroll
+  for (int kb = KHI; kb >= KLO; --kb) {
+    const int bit = WaveLayout<LW>::reg(P, kb);
+    const int s = LOGM - 1 - bit;
+#pragma unroll
+    for (int c = 0; c < (1 << PT::nhigh(kb)); ++c)
+      tw[PT::base(kb) + c] =
+          ar.fwd_tw_raw((unsigned)((1 << s) + ((PT::combo_part(kb, c) | tw_tp) >> (bit + 1))));
+  }
+}
+
+#ifndef SHG_NTT_WAVE_SMEM
+#define SHG_NTT_WAVE_SMEM 1
+#endif
+
+// tw_tp: the lane/wave part of the indices as far as it reaches above the
+// pass's bits (0 for pass 1, the wave part for pass 2).  UNIFORM: tw_tp is
+// the same across the wave, and the twiddles are read through the scalar
+// cache (SMEM loads complete independently of vector loads in flight).
+template <int LW, int P, int KHI, int KLO, bool UNIFORM = false, class A>
+__device__ __forceinline__ void wave_pass_tw(double* tw, int tw_tp, const A& ar) {
+  if constexpr (UNIFORM && SHG_NTT_WAVE_SMEM) {
+    using PT = WavePassTw<LW, P, KHI, KLO>;
+    constexpr int LOGM = WaveLayout<LW>::LOGM;
+    using cptr = const double __attribute__((address_space(4)))*;
+    const cptr ct = (cptr)(size_t)(const void*)ar.fw;
+#pragma unroll
+    for (int kb = KHI; kb >= KLO; --kb) {
+      const int bit = WaveLayout<LW>::reg(P, kb);
+      const int s = LOGM - 1 - bit;
+#pragma unroll
+      for (int c = 0; c < (1 << PT::nhigh(kb)); ++c)
+        tw[PT::base(kb) + c] =
+            ct[(unsigned)((1 << s) + ((PT::combo_part(kb, c) | tw_tp) >> (bit + 1)))];
+    }
+    return;
+  }
+  wave_pass_tw_vec<LW, P, KHI, KLO>(tw, tw_tp, ar);
+}
+
+template <int LW, int P, int KHI, int KLO, class A>
+__device__ __forceinline__ void wave_pass(double (&x)[16], const double* tw, const A& ar) {
+  using PT = WavePassTw<LW, P, KHI, KLO>;
+  constexpr int LOGM = WaveLayout<LW>::LOGM;
+#pragma unroll
+  for (int kb = KHI; kb >= KLO; --kb) {
+    const int s = LOGM - 1 - WaveLayout<LW>::reg(P, kb);
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+      if (e & (1 << kb)) continue;
+      ar.ct(x[e], x[e | (1 << kb)], ar.tw(tw[PT::slot(kb, e)]));
+    }
+    if (ar.fwd_reduce_at(s) && !(A::kSkipLastReduce && s == LOGM - 1)) {
+#pragma unroll
+      for (int e = 0; e < 16; ++e) ar.fwd_reduce(x[e]);
+    }
+  }
+}
+
+// Orders a wave's LDS writes before its following reads (other lanes').
+__device__ __forceinline__ void wave_sync_lds() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int LW, int P>
+__device__ __forceinline__ void wave_to_lds(const double (&x)[16], double* lds, int tp) {
+  using Lay = WaveLayout<LW>;
+  const int base = lpad(tp);
+#pragma unroll
+  for (int e = 0; e < 16; ++e) lds[base + cpad(Lay::reg_part(P, e))] = x[e];
+}
+
+template <int LW, int P>
+__device__ __forceinline__ void wave_from_lds(double (&x)[16], const double* lds, int tp) {
+  using Lay = WaveLayout<LW>;
+  const int base = lpad(tp);
+#pragma unroll
+  for (int e = 0; e < 16; ++e) x[e] = lds[base + cpad(Lay::reg_part(P, e))];
+}
+
+}  // namespace shg
+
+#endif  // SHG_NTT_WAVE_H_

@@ -19,7 +19,7 @@ VP = ctypes.c_void_p
 OP_ADD, OP_SUB, OP_MUL, OP_NEG, OP_FMA = 0, 1, 2, 3, 4
 BATCH_NTT_FWD, BATCH_NTT_INV, BATCH_MUL, BATCH_MOD_REDUCE = 0, 1, 2, 3
 NTT_AUTO, NTT_INT, NTT_F64 = 0, 1, 2
-NTT_TILING_ALT, NTT_PREFETCH, NTT_SPLIT_LIMB = 4, 8, 16
+NTT_TILING_ALT, NTT_PREFETCH, NTT_SPLIT_LIMB, NTT_WAVE_EXCHANGE = 4, 8, 16, 32
 
 _lib = None
 

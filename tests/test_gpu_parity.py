@@ -363,15 +363,17 @@ def test_ntt_reference_moduli(ctx, entry):
 
 CONFIGS = [sg.NTT_F64 | sg.NTT_TILING_ALT, sg.NTT_F64 | sg.NTT_PREFETCH,
            sg.NTT_F64 | sg.NTT_TILING_ALT | sg.NTT_PREFETCH, sg.NTT_INT | sg.NTT_TILING_ALT,
-           sg.NTT_F64 | sg.NTT_SPLIT_LIMB, sg.NTT_INT | sg.NTT_SPLIT_LIMB]
+           sg.NTT_F64 | sg.NTT_SPLIT_LIMB, sg.NTT_INT | sg.NTT_SPLIT_LIMB,
+           sg.NTT_F64 | sg.NTT_WAVE_EXCHANGE, sg.NTT_INT | sg.NTT_WAVE_EXCHANGE]
 
 
 @pytest.mark.parametrize("cfg", CONFIGS)
 @pytest.mark.parametrize("log_n,L,count", [(12, 2, 3), (13, 3, 2), (14, 4, 3), (15, 2, 2)])
 def test_ntt_launch_configs(ctx, cfg, log_n, L, count):
-    """Every register tiling, the prefetching loop and the split forward
-    transform (2^14 only; other sizes ignore the flag), with more limbs than
-    resident workgroups at 2^14 so the persistent loops iterate."""
+    """Every register tiling, the prefetching loop, the split forward
+    transform (2^14 only) and the wave-exchange forward transform (FP64,
+    2^12..2^14; other shapes ignore the flags), with more limbs than resident
+    workgroups at 2^14 so the persistent loops iterate."""
     if log_n == 14:
         count = 160  # 640 limbs > 256 CUs at one workgroup each
     qs, rns = _rns(log_n, L)
@@ -390,7 +392,7 @@ def test_autotune_then_auto(ctx):
     qs, rns = _rns(13, 4)
     pids = ctx.primes(qs, 13)
     chosen = ctx.autotune_ntt(13, 256)
-    assert chosen & 3 in (sg.NTT_INT, sg.NTT_F64) and chosen & ~31 == 0
+    assert chosen & 3 in (sg.NTT_INT, sg.NTT_F64) and chosen & ~63 == 0
     x = rns.uniform(77, 64)  # 256 limbs: the tuned shape
     d = ctx.array(x)
     ctx.ntt_fwd(d, 64, pids)
