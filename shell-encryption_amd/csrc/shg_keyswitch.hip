@@ -18,15 +18,21 @@
 // key rows, which the L2 and Infinity Cache then serve.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "shg_device.h"
 #include "shg_kernels.h"
 #include "shg_ntt_core.h"
+#include "shg_ntt_wave.h"
 
 #ifndef SHG_KS_XCD_REMAP
 #define SHG_KS_XCD_REMAP 1
 #endif
 #ifndef SHG_KS_DB
 #define SHG_KS_DB 1
+#endif
+#ifndef SHG_KS_WAVE
+#define SHG_KS_WAVE 1
 #endif
 
 
@@ -279,6 +285,221 @@ __global__ void __launch_bounds__((1 << LOGM) >> LOGE)
   }
 }
 
+// The digit-plane key switch on the wave-exchange layout (shg_ntt_wave.h):
+// slices of 2^(10+LW) words (n = 2^12: LW = 2; 2^13: LW = 3; 2^14: LW = 3
+// with HB = 1), FP64.  Per digit: one workgroup-wide LDS exchange between
+// two barriers, the others inside each wave; uniform twiddles through the
+// scalar cache.  The multiply-accumulate runs in the last pass's layout:
+// thread (wave w, lane l) holds, for gg = 0..3, the four consecutive words
+// tp4 + 256 gg + 0..3 of the slice.  Workgroup order and remap as in
+// keyswitch_kernel.
+template <int LW, int HB>
+__global__ void __launch_bounds__(64 << LW)
+    keyswitch_wave_kernel(KeySwitchArgs ka, GadgetDesc gd, const PrimeDev* __restrict__ primes,
+                          LimbPrimes lp, int count, int num_limbs) {
+  using A = F64KsArith;
+  using V = double;
+  using Lay = WaveLayout<LW>;
+  constexpr int LOGM = Lay::LOGM;
+  constexpr int E = 16;
+  constexpr int T = 64 << LW;
+  constexpr int LOGN = LOGM + HB;
+  constexpr bool kPrep = A::ks_prep(LOGN);
+  constexpr int kAccEvery = A::ks_acc_every(LOGN);
+  using T1 = WavePassTw<LW, 1, 3, 0>;
+  using T2 = WavePassTw<LW, 2, LW - 1, 0>;
+  using T3 = WavePassTw<LW, 3, 3, 0>;
+  using T4 = WavePassTw<LW, 4, 1, 0>;
+  extern __shared__ __align__(16) unsigned char smem[];
+  V* lds = reinterpret_cast<V*>(smem);
+  unsigned b = blockIdx.x;
+#if SHG_KS_XCD_REMAP
+  if ((gridDim.x & 7) == 0) b = (b & 7) * (gridDim.x >> 3) + (b >> 3);
+#endif
+  const int h = HB ? (int)(b & ((1u << HB) - 1)) : 0;
+  const int lc = (int)(b >> HB);  // l * count + ct: limb-major order
+  const int l = lc / count;
+  const int ct = lc - l * count;
+  const int t = threadIdx.x;
+  const int lane = t & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
+  const PrimeDev& P = primes[lp.id[l]];
+  const A ar(P);
+  const size_t limb_off = ((size_t)l << LOGN) + ((size_t)h << LOGM);
+  const size_t poly_words = (size_t)num_limbs << LOGN;
+
+  V acc0[E], acc1[E];
+  uint64_t* o0 = ka.out0 + ct * ka.out_stride + limb_off;
+  uint64_t* o1 = ka.out1 + ct * ka.out_stride + limb_off;
+  {
+    const int tp4 = Lay::thread_part(4, lane, wave);
+    const uint64_t* i0 = ka.accumulate ? o0 : (ka.c0 ? ka.c0 + ct * ka.c0_stride + limb_off : nullptr);
+    const uint64_t* i1 = ka.accumulate ? o1 : (ka.c1 ? ka.c1 + ct * ka.c1_stride + limb_off : nullptr);
+#pragma unroll
+    for (int gg = 0; gg < 4; ++gg) {
+      const size_t base = (size_t)tp4 + Lay::reg_part(4, gg * 4);
+#pragma unroll
+      for (int e = 0; e < 4; e += 2) {
+        u64x2 a = i0 ? *reinterpret_cast<const u64x2*>(i0 + base + e) : u64x2{0, 0};
+        u64x2 b2 = i1 ? *reinterpret_cast<const u64x2*>(i1 + base + e) : u64x2{0, 0};
+        acc0[gg * 4 + e] = ar.acc_in(a.x);
+        acc0[gg * 4 + e + 1] = ar.acc_in(a.y);
+        acc1[gg * 4 + e] = ar.acc_in(b2.x);
+        acc1[gg * 4 + e + 1] = ar.acc_in(b2.y);
+      }
+    }
+  }
+
+  // Digits from 16-bit planes (launch_ntt_inv_digits): thread t's VC digits
+  // of a plane are positions t + T v, v < VC, i.e. pass 1's layout (both
+  // halves of the limb for HB = 1); the next digit's are loaded while this
+  // one is transformed.
+  constexpr int VC = ks_planes_vc(LOGN);
+  constexpr int NW = VC / 8;
+  static_assert(VC == E << HB, "plane layout must match the slice tiling");
+  const uint4* pl = reinterpret_cast<const uint4*>(ka.planes + ((size_t)ct * gd.k << LOGN)) +
+                    (size_t)t * NW;
+  constexpr size_t kPlane = ((size_t)1 << LOGN) / 8;  // uint4 per plane
+  uint4 cur[NW];
+#pragma unroll
+  for (int w = 0; w < NW; ++w) cur[w] = pl[w];
+  for (int j = 0; j < gd.k; ++j) {
+    uint4 nxt[NW];
+    if (j + 1 < gd.k) {
+#pragma unroll
+      for (int w = 0; w < NW; ++w) nxt[w] = pl[(size_t)(j + 1) * kPlane + w];
+    }
+    // An opaque copy of the lane index per digit keeps the per-thread
+    // twiddle and LDS offsets inside the loop (hoisted they would stay live
+    // across it beside the accumulators).
+    int lane_o = lane;
+    asm volatile("" : "+v"(lane_o));
+    const int tp1 = Lay::thread_part(1, lane_o, wave);
+    double tw1[T1::N];
+    wave_pass_tw<LW, 1, 3, 0, true, HB>(tw1, 0, h, ar);
+    uint32_t wd[VC / 2];
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+      wd[4 * w] = cur[w].x;
+      wd[4 * w + 1] = cur[w].y;
+      wd[4 * w + 2] = cur[w].z;
+      wd[4 * w + 3] = cur[w].w;
+    }
+    auto digit = [&](int i) {
+      const uint32_t v = (i & 1) ? (wd[i / 2] >> 16) : (wd[i / 2] & 0xffffu);
+      return ar.digit_in32(v, P);
+    };
+    V x[E];
+    if constexpr (HB == 0) {
+#pragma unroll
+      for (int e = 0; e < E; ++e) x[e] = digit(e);
+    } else {
+      const auto w = ar.fwd_tw(1);
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        V u = digit(e);
+        V v = digit(E + e);
+        ar.ct(u, v, w);
+        x[e] = h ? v : u;
+        if (ar.fwd_reduce_at(0)) ar.fwd_reduce(x[e]);
+      }
+    }
+    wave_pass<LW, 1, 3, 0, HB>(x, tw1, ar);
+    __builtin_amdgcn_sched_barrier(0);
+    double tw2[T2::N];
+    wave_pass_tw<LW, 2, LW - 1, 0, true, HB>(tw2, wave << 10, h, ar);
+    // Every wave has finished the previous digit's in-wave exchanges.
+    __syncthreads();
+    wave_to_lds<LW, 1>(x, lds, tp1);
+    __syncthreads();
+    const int tp2 = Lay::thread_part(2, lane_o, wave);
+    wave_from_lds<LW, 2>(x, lds, tp2);
+    wave_pass<LW, 2, LW - 1, 0, HB>(x, tw2, ar);
+    const int tp3 = Lay::thread_part(3, lane_o, wave);
+    __builtin_amdgcn_sched_barrier(0);
+    double tw3[T3::N];
+    wave_pass_tw<LW, 3, 3, 0, false, HB>(tw3, tp3, h, ar);
+    wave_to_lds<LW, 2>(x, lds, tp2);
+    wave_sync_lds();
+    wave_from_lds<LW, 3>(x, lds, tp3);
+    wave_pass<LW, 3, 3, 0, HB>(x, tw3, ar);
+    const int tp4 = Lay::thread_part(4, lane_o, wave);
+    __builtin_amdgcn_sched_barrier(0);
+    double tw4[T4::N];
+    wave_pass_tw<LW, 4, 1, 0, false, HB>(tw4, tp4, h, ar);
+    wave_to_lds<LW, 3>(x, lds, tp3);
+    wave_sync_lds();
+    wave_from_lds<LW, 4>(x, lds, tp4);
+    // The next digit's exchange writes these slots after its first barrier.
+    pin_words(x);
+    wave_pass<LW, 4, 1, 0, HB>(x, tw4, ar);
+    // Multiply-accumulate with key row j.
+    const uint64_t* kb = ka.key_b + (size_t)(ka.key_base + j) * poly_words + limb_off + tp4;
+    const uint64_t* kap = ka.key_a + (size_t)(ka.key_base + j) * poly_words + limb_off + tp4;
+#pragma unroll
+    for (int gg = 0; gg < 4; ++gg) {
+      const int off = Lay::reg_part(4, gg * 4);
+#pragma unroll
+      for (int e = 0; e < 4; e += 2) {
+        const u64x2 kbv = *reinterpret_cast<const u64x2*>(kb + off + e);
+        const u64x2 kav = *reinterpret_cast<const u64x2*>(kap + off + e);
+        const int r = gg * 4 + e;
+        typename A::Prep d0, d1;
+        if constexpr (kPrep) {
+          d0 = ar.prep(x[r]);
+          d1 = ar.prep(x[r + 1]);
+        } else {
+          d0 = x[r];
+          d1 = x[r + 1];
+        }
+        ar.mac(acc0[r], d0, kbv.x, P);
+        ar.mac(acc0[r + 1], d1, kbv.y, P);
+        ar.mac(acc1[r], d0, kav.x, P);
+        ar.mac(acc1[r + 1], d1, kav.y, P);
+      }
+    }
+    if (((j + 1) % kAccEvery) == 0) {
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        ar.acc_reduce(acc0[e]);
+        ar.acc_reduce(acc1[e]);
+      }
+    }
+    if (j + 1 < gd.k) {
+#pragma unroll
+      for (int w = 0; w < NW; ++w) cur[w] = nxt[w];
+    }
+  }
+  const int tp4 = Lay::thread_part(4, lane, wave);
+#pragma unroll
+  for (int gg = 0; gg < 4; ++gg) {
+    const size_t base = (size_t)tp4 + Lay::reg_part(4, gg * 4);
+#pragma unroll
+    for (int e = 0; e < 4; e += 2) {
+      u64x2 a, b2;
+      a.x = ar.acc_out(acc0[gg * 4 + e]);
+      a.y = ar.acc_out(acc0[gg * 4 + e + 1]);
+      b2.x = ar.acc_out(acc1[gg * 4 + e]);
+      b2.y = ar.acc_out(acc1[gg * 4 + e + 1]);
+      *reinterpret_cast<u64x2*>(o0 + base + e) = a;
+      *reinterpret_cast<u64x2*>(o1 + base + e) = b2;
+    }
+  }
+}
+
+template <int LW, int HB>
+static hipError_t launch_ks_wave(const KeySwitchArgs& ka, const GadgetDesc& gd,
+                                 const PrimeDev* primes, const LimbPrimes& lp, int count,
+                                 int num_limbs, hipStream_t s) {
+  constexpr int T = 64 << LW;
+  const size_t shmem = (size_t)lds_words<10 + LW>() * 8;
+  auto k = keyswitch_wave_kernel<LW, HB>;
+  if (shmem > 65536) set_max_dynamic_shmem((const void*)k, shmem);
+  hipLaunchKernelGGL(k, dim3((unsigned)(count * num_limbs) << HB), dim3(T), shmem, s, ka, gd,
+                     primes, lp, count, num_limbs);
+  return hipGetLastError();
+}
+
 template <int LOGM, int LOGE, class A, int HB, bool PL>
 static hipError_t launch_ks(const KeySwitchArgs& ka, const GadgetDesc& gd, const PrimeDev* primes,
                             const LimbPrimes& lp, int count, int num_limbs, hipStream_t s) {
@@ -296,6 +517,13 @@ static hipError_t dispatch_ks(int logn, const KeySwitchArgs& ka, const GadgetDes
                               const PrimeDev* primes, const LimbPrimes& lp, int count,
                               int num_limbs, hipStream_t s) {
 #define SHG_KS(M, E, HB) launch_ks<M, E, A, HB, PL>(ka, gd, primes, lp, count, num_limbs, s)
+  // The wave-exchange kernel at n = 2^12 (single LDS buffer: three 256-thread
+  // workgroups per CU): 0.35 vs 0.52 us per ciphertext at L = 2, 1.28 vs 1.86
+  // at L = 4.  At 2^13 and 2^14 one 512-thread workgroup per CU is resident
+  // either way and keyswitch_kernel measured 5% and 2.5% faster.
+  if constexpr (PL && std::is_same<A, F64KsArith>::value && SHG_KS_WAVE) {
+    if (logn == 12) return launch_ks_wave<2, 0>(ka, gd, primes, lp, count, num_limbs, s);
+  }
   switch (logn) {
     case 4: return SHG_KS(4, 4, 0);
     case 5: return SHG_KS(5, 4, 0);

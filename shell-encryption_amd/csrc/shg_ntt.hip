@@ -246,7 +246,7 @@ __global__ void __launch_bounds__(64 << LW)
     uint64_t* g = data + ((size_t)item << LOGM);
     const int tp1 = Lay::thread_part(1, lane_o, wave);
     double tw1[T1::N];
-    wave_pass_tw<LW, 1, 3, 0, true>(tw1, 0, ar);
+    wave_pass_tw<LW, 1, 3, 0, true>(tw1, 0, 0, ar);
     double x[16];
     if constexpr (SHG_NTT_WAVE_PF) {
 #pragma unroll
@@ -267,7 +267,7 @@ __global__ void __launch_bounds__(64 << LW)
     // live at a time).
     __builtin_amdgcn_sched_barrier(0);
     double tw2[T2::N];
-    wave_pass_tw<LW, 2, LW - 1, 0, true>(tw2, wave << 10, ar);
+    wave_pass_tw<LW, 2, LW - 1, 0, true>(tw2, wave << 10, 0, ar);
     // Every wave has finished the previous limb's in-wave exchanges.
     __syncthreads();
     wave_to_lds<LW, 1>(x, lds, tp1);
@@ -279,7 +279,7 @@ __global__ void __launch_bounds__(64 << LW)
     const int tp3 = Lay::thread_part(3, lane_o, wave);
     __builtin_amdgcn_sched_barrier(0);
     double tw3[T3::N];
-    wave_pass_tw<LW, 3, 3, 0>(tw3, tp3, ar);
+    wave_pass_tw<LW, 3, 3, 0>(tw3, tp3, 0, ar);
     wave_to_lds<LW, 2>(x, lds, tp2);
     wave_sync_lds();
     wave_from_lds<LW, 3>(x, lds, tp3);
@@ -287,7 +287,7 @@ __global__ void __launch_bounds__(64 << LW)
     const int tp4 = Lay::thread_part(4, lane_o, wave);
     __builtin_amdgcn_sched_barrier(0);
     double tw4[T4::N];
-    wave_pass_tw<LW, 4, 1, 0>(tw4, tp4, ar);
+    wave_pass_tw<LW, 4, 1, 0>(tw4, tp4, 0, ar);
     wave_to_lds<LW, 3>(x, lds, tp3);
     wave_sync_lds();
     wave_from_lds<LW, 4>(x, lds, tp4);

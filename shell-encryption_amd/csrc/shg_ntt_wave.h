@@ -93,8 +93,11 @@ struct WavePassTw {
   }
 };
 
-template <int LW, int P, int KHI, int KLO, class A>
-__device__ __forceinline__ void wave_pass_tw_vec(double* tw, int tw_tp, const A& ar) {
+// The slice may be sub-block h of a limb with LOGM + HB index bits whose
+// first HB stages are already applied: local stage s is global stage s + HB
+// and its twiddle block carries h above the local part.
+template <int LW, int P, int KHI, int KLO, int HB, class A>
+__device__ __forceinline__ void wave_pass_tw_vec(double* tw, int tw_tp, int h, const A& ar) {
   using PT = WavePassTw<LW, P, KHI, KLO>;
   constexpr int LOGM = WaveLayout<LW>::LOGM;
 #pragma unroll
@@ -103,8 +106,8 @@ __device__ __forceinline__ void wave_pass_tw_vec(double* tw, int tw_tp, const A&
     const int s = LOGM - 1 - bit;
 #pragma unroll
     for (int c = 0; c < (1 << PT::nhigh(kb)); ++c)
-      tw[PT::base(kb) + c] =
-          ar.fwd_tw_raw((unsigned)((1 << s) + ((PT::combo_part(kb, c) | tw_tp) >> (bit + 1))));
+      tw[PT::base(kb) + c] = ar.fwd_tw_raw((unsigned)((1 << (s + HB)) + (h << s) +
+                                                      ((PT::combo_part(kb, c) | tw_tp) >> (bit + 1))));
   }
 }
 
@@ -116,8 +119,8 @@ __device__ __forceinline__ void wave_pass_tw_vec(double* tw, int tw_tp, const A&
 // pass's bits (0 for pass 1, the wave part for pass 2).  UNIFORM: tw_tp is
 // the same across the wave, and the twiddles are read through the scalar
 // cache (SMEM loads complete independently of vector loads in flight).
-template <int LW, int P, int KHI, int KLO, bool UNIFORM = false, class A>
-__device__ __forceinline__ void wave_pass_tw(double* tw, int tw_tp, const A& ar) {
+template <int LW, int P, int KHI, int KLO, bool UNIFORM = false, int HB = 0, class A>
+__device__ __forceinline__ void wave_pass_tw(double* tw, int tw_tp, int h, const A& ar) {
   if constexpr (UNIFORM && SHG_NTT_WAVE_SMEM) {
     using PT = WavePassTw<LW, P, KHI, KLO>;
     constexpr int LOGM = WaveLayout<LW>::LOGM;
@@ -129,27 +132,27 @@ __device__ __forceinline__ void wave_pass_tw(double* tw, int tw_tp, const A& ar)
       const int s = LOGM - 1 - bit;
 #pragma unroll
       for (int c = 0; c < (1 << PT::nhigh(kb)); ++c)
-        tw[PT::base(kb) + c] =
-            ct[(unsigned)((1 << s) + ((PT::combo_part(kb, c) | tw_tp) >> (bit + 1)))];
+        tw[PT::base(kb) + c] = ct[(unsigned)((1 << (s + HB)) + (h << s) +
+                                             ((PT::combo_part(kb, c) | tw_tp) >> (bit + 1)))];
     }
     return;
   }
-  wave_pass_tw_vec<LW, P, KHI, KLO>(tw, tw_tp, ar);
+  wave_pass_tw_vec<LW, P, KHI, KLO, HB>(tw, tw_tp, h, ar);
 }
 
-template <int LW, int P, int KHI, int KLO, class A>
+template <int LW, int P, int KHI, int KLO, int HB = 0, class A>
 __device__ __forceinline__ void wave_pass(double (&x)[16], const double* tw, const A& ar) {
   using PT = WavePassTw<LW, P, KHI, KLO>;
   constexpr int LOGM = WaveLayout<LW>::LOGM;
 #pragma unroll
   for (int kb = KHI; kb >= KLO; --kb) {
-    const int s = LOGM - 1 - WaveLayout<LW>::reg(P, kb);
+    const int s = LOGM - 1 - WaveLayout<LW>::reg(P, kb) + HB;  // global stage
 #pragma unroll
     for (int e = 0; e < 16; ++e) {
       if (e & (1 << kb)) continue;
       ar.ct(x[e], x[e | (1 << kb)], ar.tw(tw[PT::slot(kb, e)]));
     }
-    if (ar.fwd_reduce_at(s) && !(A::kSkipLastReduce && s == LOGM - 1)) {
+    if (ar.fwd_reduce_at(s) && !(A::kSkipLastReduce && s == LOGM + HB - 1)) {
 #pragma unroll
       for (int e = 0; e < 16; ++e) ar.fwd_reduce(x[e]);
     }

@@ -223,8 +223,8 @@ def test_gadget_decompose(ctx):
     assert np.array_equal(dg.download(), exp)
 
 
-@pytest.mark.parametrize("log_n,L,log_b,count", [(6, 2, 10, 3), (12, 3, 10, 2), (14, 8, 10, 2),
-                                                 (15, 2, 25, 1)])
+@pytest.mark.parametrize("log_n,L,log_b,count", [(6, 2, 10, 3), (12, 3, 10, 2), (12, 2, 10, 64),
+                                                 (13, 4, 10, 2), (14, 8, 10, 2), (15, 2, 25, 1)])
 def test_relin_apply(ctx, log_n, L, log_b, count):
     qs, rns = _rns(log_n, L)
     pids = ctx.primes(qs, log_n)
@@ -235,10 +235,11 @@ def test_relin_apply(ctx, log_n, L, log_b, count):
     dc, dka, dkb = ctx.array(comps), ctx.array(kas), ctx.array(kbs)
     o0, o1 = ctx.empty((count, L, rns.n)), ctx.empty((count, L, rns.n))
     ctx.relin_apply(dc, 2, 0b111, dka, dkb, k, log_bs, o0, o1, count, pids)
+    g0, g1 = o0.download(), o1.download()
     for c in range(count):
         e0, e1 = rns.relin_apply(comps[c], kas, kbs, log_bs)
-        assert np.array_equal(o0.download()[c], e0)
-        assert np.array_equal(o1.download()[c], e1)
+        assert np.array_equal(g0[c], e0), c
+        assert np.array_equal(g1[c], e1), c
 
 
 def test_relin_degree3_and_coeff_form_component(ctx):
@@ -256,7 +257,7 @@ def test_relin_degree3_and_coeff_form_component(ctx):
     assert np.array_equal(o0.download()[0], e0) and np.array_equal(o1.download()[0], e1)
 
 
-@pytest.mark.parametrize("log_n,L,count", [(7, 2, 2), (14, 8, 2)])
+@pytest.mark.parametrize("log_n,L,count", [(7, 2, 2), (12, 2, 16), (14, 8, 2)])
 def test_galois_apply(ctx, log_n, L, count):
     qs, rns = _rns(log_n, L)
     pids = ctx.primes(qs, log_n)
@@ -267,10 +268,11 @@ def test_galois_apply(ctx, log_n, L, count):
     dc, dka, dkb = ctx.array(comps), ctx.array(kas), ctx.array(kbs)
     o0, o1 = ctx.empty((count, L, rns.n)), ctx.empty((count, L, rns.n))
     ctx.galois_apply(dc, True, dka, dkb, log_bs, o0, o1, count, pids)
+    g0, g1 = o0.download(), o1.download()
     for c in range(count):
         e0, e1 = rns.galois_apply(comps[c, 0], comps[c, 1], kas, kbs, log_bs)
-        assert np.array_equal(o0.download()[c], e0)
-        assert np.array_equal(o1.download()[c], e1)
+        assert np.array_equal(g0[c], e0), c
+        assert np.array_equal(g1[c], e1), c
 
 
 @pytest.mark.parametrize("terms", [1, 5, 64])
