@@ -154,6 +154,17 @@ struct IntArith {
   __device__ __forceinline__ uint64_t acc_out(Acc a) const { return a; }
 };
 
+// Twiddle products take their quotient from fl(v*w) (f64_mulmod_h); the
+// twiddle's w/q is then never formed.
+#ifndef SHG_F64_HQ
+#define SHG_F64_HQ 1
+#endif
+#if SHG_F64_HQ
+#define SHG_F64_MULMOD(x, tw) f64_mulmod_h(x, (tw).w, q, qinv)
+#else
+#define SHG_F64_MULMOD(x, tw) f64_mulmod(x, (tw).w, (tw).wq, q)
+#endif
+
 struct F64Arith {
   using V = double;
   static constexpr bool kSkipLastReduce = true;
@@ -177,19 +188,19 @@ struct F64Arith {
   // Inputs are canonical (<= q < 2^50).
   __device__ __forceinline__ V load(uint64_t x) const { return u52_to_f64(x); }
   __device__ __forceinline__ void ct(V& u, V& v, const Tw& w) const {
-    const double t = f64_mulmod(v, w.w, w.wq, q);
+    const double t = SHG_F64_MULMOD(v, w);
     v = u - t;
     u = u + t;
   }
   __device__ __forceinline__ V ct_half(V u, V v, const Tw& w, bool low) const {
-    const double t = f64_mulmod(v, w.w, w.wq, q);
+    const double t = SHG_F64_MULMOD(v, w);
     return u + (low ? -t : t);
   }
   __device__ __forceinline__ void gs(V& u, V& v, const Tw& w) const {
     const double s = u + v;
     const double d = u - v;
     u = s;
-    v = f64_mulmod(d, w.w, w.wq, q);
+    v = SHG_F64_MULMOD(d, w);
   }
   __device__ __forceinline__ void fwd_reduce(V& x) const { x = f64_reduce(x, q, qinv); }
   __device__ __forceinline__ void inv_reduce(V& x) const { x = f64_reduce(x, q, qinv); }
@@ -221,7 +232,11 @@ struct F64Arith {
   __device__ __forceinline__ Prep prep(V y) const { return f64_reduce(y, q, qinv); }
   __device__ __forceinline__ void mac(Acc& a, Prep d, uint64_t key, const PrimeDev&) const {
     const double w = u52_to_f64(key);
+#if SHG_F64_HQ
+    a += f64_mulmod_h(d, w, q, qinv);
+#else
     a += f64_mulmod(d, w, w * qinv, q);
+#endif
   }
   __device__ __forceinline__ void acc_reduce(Acc& a) const { a = f64_reduce(a, q, qinv); }
   __device__ __forceinline__ uint64_t acc_out(Acc a) const {
