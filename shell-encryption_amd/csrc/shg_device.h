@@ -133,10 +133,11 @@ __device__ __forceinline__ uint64_t convert_balanced(uint64_t rep_q, const Prime
 
 // ---------------------------------------------------------------------------
 // FP64 exact modular arithmetic (q < 2^50).  Integers are held exactly in
-// doubles; x*w = h + l exactly (h = fl(x*w), l = fma(x, w, -h)), the quotient
-// c = rint(x * fl(w/q)) is within 1 + |x|/2^52 of x*w/q, so h - c*q is an
+// doubles; x*w = h + l exactly (h = fl(x*w), l = fma(x, w, -h)).  The
+// quotient c = rint(x * fl(w/q)), or rint(h * fl(1/q)), is within
+// 1/2 + 1.5 * 2^-52 |x| w/q of x*w/q (three roundings), so h - c*q is an
 // exact small integer and t = (h - c*q) + l = x*w - c*q is exact with
-// |t| <= q * (0.5 + |x| * 2^-52).
+// |t| <= q/2 + 1.5 * 2^-52 |x| w.
 // ---------------------------------------------------------------------------
 
 // x < 2^52 -> double, exactly: 2^52 + x has the bits 0x433 << 52 | x.

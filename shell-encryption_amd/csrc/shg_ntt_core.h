@@ -177,8 +177,10 @@ struct F64Arith {
   struct Tw {
     double w, wq;
   };
-  // Tables hold w only (8 bytes per twiddle); w/q = w * fl(1/q) is formed
-  // on use, within 2^-52 relative error, which the bounds above absorb.
+  // Tables hold w only (8 bytes per twiddle).  With SHG_F64_HQ the products
+  // take their quotient from fl(v*w) * fl(1/q) and wq is dead code; without
+  // it wq = w * fl(1/q) is formed on use.  Either way three roundings, which
+  // the bounds below absorb.
   using TwRaw = double;
   __device__ __forceinline__ TwRaw fwd_tw_raw(int i) const { return fw[i]; }
   __device__ __forceinline__ TwRaw inv_tw_raw(int i) const { return iw[i]; }
