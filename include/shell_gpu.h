@@ -76,10 +76,11 @@ typedef enum shg_ntt_variant {
  * workgroups (both compute stage 0; each finishes one half), two of which
  * share a CU. */
 #define SHG_NTT_SPLIT_LIMB 16
-/* FP64 forward transforms of 2^12..2^14-coefficient limbs with a single
- * workgroup-wide exchange per limb; the other exchanges stay inside each
- * wave, so waves run independently between limbs.  The default FP64
- * forward transform at 2^14 (without the other flags) already is this one. */
+/* FP64 forward and inverse transforms of 2^12..2^14-coefficient limbs with
+ * a single workgroup-wide exchange per limb; the other exchanges stay inside
+ * each wave, so waves run independently between limbs.  Without the other
+ * flags the FP64 forward transform at 2^14 and the inverse at 2^13 and 2^14
+ * already are these. */
 #define SHG_NTT_WAVE_EXCHANGE 32
 
 /* ---- context, memory, streams ------------------------------------------ */

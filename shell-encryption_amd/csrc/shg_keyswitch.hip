@@ -34,6 +34,10 @@
 #ifndef SHG_KS_WAVE
 #define SHG_KS_WAVE 1
 #endif
+// Ring degrees (bit log n) that take the wave-exchange kernel.
+#ifndef SHG_KS_WAVE_LOGN
+#define SHG_KS_WAVE_LOGN (1 << 12)
+#endif
 
 
 namespace shg {
@@ -520,9 +524,14 @@ static hipError_t dispatch_ks(int logn, const KeySwitchArgs& ka, const GadgetDes
   // The wave-exchange kernel at n = 2^12 (single LDS buffer: three 256-thread
   // workgroups per CU): 0.35 vs 0.52 us per ciphertext at L = 2, 1.28 vs 1.86
   // at L = 4.  At 2^13 and 2^14 one 512-thread workgroup per CU is resident
-  // either way and keyswitch_kernel measured 5% and 2.5% faster.
+  // either way and keyswitch_kernel measured 5% and 6% faster (2^14, L = 8:
+  // 10.55 vs 11.2 ms per 512 ciphertexts).
   if constexpr (PL && std::is_same<A, F64KsArith>::value && SHG_KS_WAVE) {
-    if (logn == 12) return launch_ks_wave<2, 0>(ka, gd, primes, lp, count, num_limbs, s);
+    if ((SHG_KS_WAVE_LOGN >> logn) & 1) {
+      if (logn == 12) return launch_ks_wave<2, 0>(ka, gd, primes, lp, count, num_limbs, s);
+      if (logn == 13) return launch_ks_wave<3, 0>(ka, gd, primes, lp, count, num_limbs, s);
+      if (logn == 14) return launch_ks_wave<3, 1>(ka, gd, primes, lp, count, num_limbs, s);
+    }
   }
   switch (logn) {
     case 4: return SHG_KS(4, 4, 0);

@@ -42,6 +42,13 @@
 #ifndef SHG_NTT_WAVE_PF
 #define SHG_NTT_WAVE_PF 1
 #endif
+// FP64 inverse transforms of 2^13 and 2^14 limbs on the wave-exchange
+// kernel by default (kNttCfgWave selects it at 2^12..2^14 either way):
+// 0.70 vs 0.75 ms per 8192 limbs of 2^14, 0.67 vs 0.72 at 2^13; at 2^12 the
+// two are level (0.62-0.63 ms per 2^25 words).
+#ifndef SHG_NTT_INV_WAVE
+#define SHG_NTT_INV_WAVE 1
+#endif
 
 namespace shg {
 
@@ -306,6 +313,102 @@ __global__ void __launch_bounds__(64 << LW)
         o[e2 / 2] = v;
       }
     }
+  }
+}
+
+// Inverse transform on the wave-exchange layout, FP64: the forward wave
+// kernel's passes in the opposite order.  Words are loaded in pass 4's
+// layout (index bits 0 and 1 in registers, so each register group is four
+// consecutive words read with two 16-byte loads), two exchanges inside each
+// wave lead to pass 2's layout and one workgroup-wide exchange to pass 1's,
+// whose words are scaled and stored coalesced.  The next limb is loaded
+// while the current one is transformed.  Barriers: one before the first
+// in-wave write of a limb (other waves may still read the previous limb's
+// workgroup-wide exchange there) and one inside that exchange.
+template <int LW, class A>
+__global__ void __launch_bounds__(64 << LW)
+    ntt_inv_wave_kernel(uint64_t* __restrict__ data, const PrimeDev* __restrict__ primes,
+                        LimbPrimes lp, int num_limbs, int items, int scale) {
+  using Lay = WaveLayout<LW>;
+  constexpr int LOGM = Lay::LOGM;
+  extern __shared__ __align__(16) unsigned char smem[];
+  double* lds = reinterpret_cast<double*>(smem);
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  using T1 = WavePassTw<LW, 4, 1, 0>;
+  using T2 = WavePassTw<LW, 3, 3, 0>;
+  using T3 = WavePassTw<LW, 2, LW - 1, 0>;
+  using T4 = WavePassTw<LW, 1, 3, 0>;
+  // Register 4 gg + j holds word tp4 + reg_part(4, 4 gg) + j.
+  auto load4 = [&](const uint64_t* g, int tp4, uint64_t (&w)[16]) {
+#pragma unroll
+    for (int gg = 0; gg < 4; ++gg) {
+      const u64x2* p = reinterpret_cast<const u64x2*>(g + tp4 + Lay::reg_part(4, gg * 4));
+      const u64x2 a = p[0];
+      const u64x2 b = p[1];
+      w[gg * 4] = a.x;
+      w[gg * 4 + 1] = a.y;
+      w[gg * 4 + 2] = b.x;
+      w[gg * 4 + 3] = b.y;
+    }
+  };
+  uint64_t pf[16];
+  if ((int)blockIdx.x < items)
+    load4(data + ((size_t)blockIdx.x << LOGM), Lay::thread_part(4, lane, wave), pf);
+  for (int item = blockIdx.x; item < items; item += gridDim.x) {
+    // Opaque lane copy per limb, as in the forward kernel.
+    int lane_o = lane;
+    asm volatile("" : "+v"(lane_o));
+    const A ar(primes[lp.id[item % num_limbs]]);
+    uint64_t* g = data + ((size_t)item << LOGM);
+    const int tp4 = Lay::thread_part(4, lane_o, wave);
+    double tw1[T1::N];
+    wave_pass_tw_inv<LW, 4, 1, 0, false>(tw1, tp4, ar);
+    double x[16];
+#pragma unroll
+    for (int e = 0; e < 16; ++e) x[e] = ar.load(pf[e]);
+    const int next = item + gridDim.x;
+    if (next < items) load4(data + ((size_t)next << LOGM), tp4, pf);
+    wave_pass_inv<LW, 4, 1, 0>(x, tw1, ar);
+    __builtin_amdgcn_sched_barrier(0);
+    const int tp3 = Lay::thread_part(3, lane_o, wave);
+    double tw2[T2::N];
+    wave_pass_tw_inv<LW, 3, 3, 0, false>(tw2, tp3, ar);
+    // Every wave has finished reading the previous limb's exchange.
+    __syncthreads();
+    wave_to_lds<LW, 4>(x, lds, tp4);
+    wave_sync_lds();
+    wave_from_lds<LW, 3>(x, lds, tp3);
+    wave_pass_inv<LW, 3, 3, 0>(x, tw2, ar);
+    __builtin_amdgcn_sched_barrier(0);
+    const int tp2 = Lay::thread_part(2, lane_o, wave);
+    double tw3[T3::N];
+    wave_pass_tw_inv<LW, 2, LW - 1, 0, true>(tw3, wave << 10, ar);
+    wave_to_lds<LW, 3>(x, lds, tp3);
+    wave_sync_lds();
+    wave_from_lds<LW, 2>(x, lds, tp2);
+    wave_pass_inv<LW, 2, LW - 1, 0>(x, tw3, ar);
+    __builtin_amdgcn_sched_barrier(0);
+    const int tp1 = Lay::thread_part(1, lane_o, wave);
+    double tw4[T4::N];
+    wave_pass_tw_inv<LW, 1, 3, 0, true>(tw4, 0, ar);
+    // Each wave writes only its own words (pass 2's waves own the top bits),
+    // which no other wave reads before the barrier.
+    wave_to_lds<LW, 2>(x, lds, tp2);
+    __syncthreads();
+    wave_from_lds<LW, 1>(x, lds, tp1);
+    wave_pass_inv<LW, 1, 3, 0>(x, tw4, ar);
+    auto store_all = [&](auto mode) {
+#pragma unroll
+      for (int e = 0; e < 16; ++e)
+        g[tp1 + Lay::reg_part(1, e)] = ar.inv_out(x[e], decltype(mode)::value);
+    };
+    if (scale == 1)
+      store_all(std::integral_constant<int, 1>());
+    else if (scale == 2)
+      store_all(std::integral_constant<int, 2>());
+    else
+      store_all(std::integral_constant<int, 0>());
   }
 }
 
@@ -642,6 +745,19 @@ static hipError_t launch_fwd_wave(uint64_t* data, const PrimeDev* primes, const 
   return hipGetLastError();
 }
 
+template <int LW>
+static hipError_t launch_inv_wave(uint64_t* data, const PrimeDev* primes, const LimbPrimes& lp,
+                                  int rows, int num_limbs, int scale, hipStream_t stream) {
+  constexpr int T = 64 << LW;
+  const size_t shmem = (size_t)lds_words<10 + LW>() * 8;
+  auto k = ntt_inv_wave_kernel<LW, F64Arith>;
+  if (shmem > 65536) set_max_dynamic_shmem((const void*)k, shmem);
+  const int grid = persistent_grid((const void*)k, T, shmem, rows, num_limbs);
+  hipLaunchKernelGGL(k, dim3(grid), dim3(T), shmem, stream, data, primes, lp, num_limbs, rows,
+                     scale);
+  return hipGetLastError();
+}
+
 template <int LOGM, int LOGE, class A, bool PF>
 static hipError_t launch_fused(bool inverse, uint64_t* data, const PrimeDev* primes,
                                const LimbPrimes& lp, int rows, int num_limbs, int logn,
@@ -736,6 +852,14 @@ hipError_t launch_ntt(bool inverse, int cfg, uint64_t* data, const PrimeDev* pri
     if (logn == 12) return launch_fwd_wave<2>(data, primes, lp, rows, num_limbs, s);
     if (logn == 13) return launch_fwd_wave<3>(data, primes, lp, rows, num_limbs, s);
     return launch_fwd_wave<4>(data, primes, lp, rows, num_limbs, s);
+  }
+  const bool inv_wave_default = SHG_NTT_INV_WAVE && (logn == 13 || logn == 14) &&
+                                (cfg & (kNttCfgAlt | kNttCfgPrefetch | kNttCfgSplit)) == 0;
+  if (inverse && ((cfg & kNttCfgWave) || inv_wave_default) &&
+      (cfg & 3) == SHG_NTT_F64 && logn >= 12 && logn <= 14 && logn <= max_logm) {
+    if (logn == 12) return launch_inv_wave<2>(data, primes, lp, rows, num_limbs, inv_mode, s);
+    if (logn == 13) return launch_inv_wave<3>(data, primes, lp, rows, num_limbs, inv_mode, s);
+    return launch_inv_wave<4>(data, primes, lp, rows, num_limbs, inv_mode, s);
   }
   if (!inverse && logn == 14 && max_logm >= 13 && (cfg & kNttCfgSplit)) {
     if ((cfg & 3) == SHG_NTT_F64)

@@ -159,6 +159,59 @@ __device__ __forceinline__ void wave_pass(double (&x)[16], const double* tw, con
   }
 }
 
+// Inverse (Gentleman-Sande) passes run on the same layouts in the opposite
+// order: pass 4's (index bits 0, 1), 3's (5..2), 2's (6+LW-1..6), 1's (the
+// top 4 bits).  The stage on index bit b pairs word i (bit b clear) with
+// i + 2^b and uses psi_inv_bitrev[n - n/2^b + (i >> (b + 1))], which depends
+// on the same register bits above b as the forward twiddle of that bit, so
+// WavePassTw's slots serve both.  UNIFORM as in wave_pass_tw.
+template <int LW, int P, int KHI, int KLO, bool UNIFORM, class A>
+__device__ __forceinline__ void wave_pass_tw_inv(double* tw, int tw_tp, const A& ar) {
+  using PT = WavePassTw<LW, P, KHI, KLO>;
+  constexpr int n = 1 << WaveLayout<LW>::LOGM;
+  if constexpr (UNIFORM && SHG_NTT_WAVE_SMEM) {
+    using cptr = const double __attribute__((address_space(4)))*;
+    const cptr ct = (cptr)(size_t)(const void*)ar.iw;
+#pragma unroll
+    for (int kb = KLO; kb <= KHI; ++kb) {
+      const int bit = WaveLayout<LW>::reg(P, kb);
+#pragma unroll
+      for (int c = 0; c < (1 << PT::nhigh(kb)); ++c)
+        tw[PT::base(kb) + c] =
+            ct[(unsigned)(n - (n >> bit) + ((PT::combo_part(kb, c) | tw_tp) >> (bit + 1)))];
+    }
+  } else {
+#pragma unroll
+    for (int kb = KLO; kb <= KHI; ++kb) {
+      const int bit = WaveLayout<LW>::reg(P, kb);
+#pragma unroll
+      for (int c = 0; c < (1 << PT::nhigh(kb)); ++c)
+        tw[PT::base(kb) + c] = ar.inv_tw_raw(
+            (unsigned)(n - (n >> bit) + ((PT::combo_part(kb, c) | tw_tp) >> (bit + 1))));
+    }
+  }
+}
+
+// Stages on register bits KLO up to KHI; the reduction schedule follows the
+// index bit (the inverse stage number).
+template <int LW, int P, int KHI, int KLO, class A>
+__device__ __forceinline__ void wave_pass_inv(double (&x)[16], const double* tw, const A& ar) {
+  using PT = WavePassTw<LW, P, KHI, KLO>;
+#pragma unroll
+  for (int kb = KLO; kb <= KHI; ++kb) {
+    const int bit = WaveLayout<LW>::reg(P, kb);
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+      if (e & (1 << kb)) continue;
+      ar.gs(x[e], x[e | (1 << kb)], ar.tw(tw[PT::slot(kb, e)]));
+    }
+    if (ar.inv_reduce_at(bit)) {
+#pragma unroll
+      for (int e = 0; e < 16; ++e) ar.inv_reduce(x[e]);
+    }
+  }
+}
+
 // Orders a wave's LDS writes before its following reads (other lanes').
 __device__ __forceinline__ void wave_sync_lds() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");

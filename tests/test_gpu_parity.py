@@ -373,9 +373,11 @@ CONFIGS = [sg.NTT_F64 | sg.NTT_TILING_ALT, sg.NTT_F64 | sg.NTT_PREFETCH,
 @pytest.mark.parametrize("log_n,L,count", [(12, 2, 3), (13, 3, 2), (14, 4, 3), (15, 2, 2)])
 def test_ntt_launch_configs(ctx, cfg, log_n, L, count):
     """Every register tiling, the prefetching loop, the split forward
-    transform (2^14 only) and the wave-exchange forward transform (FP64,
-    2^12..2^14; other shapes ignore the flags), with more limbs than resident
-    workgroups at 2^14 so the persistent loops iterate."""
+    transform (2^14 only) and the wave-exchange forward and inverse
+    transforms (FP64, 2^12..2^14; other shapes ignore the flags), with more
+    limbs than resident workgroups at 2^14 so the persistent loops iterate.
+    The inverse is checked against the oracle on its own input as well as by
+    the round trip."""
     if log_n == 14:
         count = 160  # 640 limbs > 256 CUs at one workgroup each
     qs, rns = _rns(log_n, L)
@@ -388,6 +390,13 @@ def test_ntt_launch_configs(ctx, cfg, log_n, L, count):
         assert np.array_equal(got[c], rns.ntt_fwd(x[c])), c
     ctx.ntt_inv(d, count, pids, cfg)
     assert np.array_equal(d.download(), x)
+    # The inverse of uniform (not transform-output) words.
+    y = rns.uniform(900 + log_n + cfg, 2)
+    d2 = ctx.array(y)
+    ctx.ntt_inv(d2, 2, pids, cfg)
+    got = d2.download()
+    for c in range(2):
+        assert np.array_equal(got[c], rns.ntt_inv(y[c])), c
 
 
 def test_autotune_then_auto(ctx):
