@@ -97,9 +97,9 @@ def stream_copy_gbs():
 
 
 def ntt_probe():
-    """Only the forward FP64 transform of the NTT leg's shape (1024 RNS
-    polynomials at n=2^14, L=8), four launches: the workload of the NTT
-    counter runs."""
+    """Only the FP64 transforms of the NTT leg's shape (1024 RNS polynomials
+    at n=2^14, L=8), four forward then four inverse launches: the workload of
+    the NTT counter runs."""
     import shell_gpu as sg
     qs = primes_for(LOG_N, L)
     rng = np.random.default_rng(20261203)
@@ -108,9 +108,11 @@ def ntt_probe():
     d_x = ctx.array(uniform(rng, qs, (NTT_BATCH,), 1 << LOG_N))
     for _ in range(4):
         ctx.ntt_fwd(d_x, NTT_BATCH, pids, sg.NTT_F64)
+    for _ in range(4):
+        ctx.ntt_inv(d_x, NTT_BATCH, pids, sg.NTT_F64)
     ctx.sync()
-    print(json.dumps({"ntt_probe": "forward FP64", "polys": NTT_BATCH, "n": 1 << LOG_N, "L": L,
-                      "launches": 4}))
+    print(json.dumps({"ntt_probe": "forward and inverse FP64", "polys": NTT_BATCH,
+                      "n": 1 << LOG_N, "L": L, "launches": 8}))
     return 0
 
 
@@ -330,7 +332,7 @@ def main():
                     help="only run the known-size streaming kernel used to scale byte counters")
     ap.add_argument("--copy-words", type=int, default=1 << 25)
     ap.add_argument("--ntt-probe", action="store_true",
-                    help="only run four forward transforms of the NTT leg (for counter runs)")
+                    help="only run four forward and four inverse transforms of the NTT leg (for counter runs)")
     args = ap.parse_args()
     if args.copy_probe:
         return copy_probe(args.copy_words)

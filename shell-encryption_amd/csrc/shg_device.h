@@ -50,6 +50,22 @@ struct PrimeDev {
 // LDS wait counter and so serialise with the LDS exchanges.
 template <class T>
 using gptr = const T __attribute__((address_space(1)))*;
+// A table of doubles read through buffer loads: the descriptor lives in
+// SGPRs and each lane supplies a 32-bit byte offset, so an indexed read
+// costs one shift where a 64-bit address costs three VALU operations per
+// lane.  Descriptor word 3 (0x00020000) selects the raw 32-bit data format
+// of gfx9-family targets; num_records is not used as a bound (callers index
+// within the table).
+struct F64Table {
+  __amdgpu_buffer_rsrc_t r;
+  __device__ __forceinline__ explicit F64Table(const void* p)
+      : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, 0x7fffffff,
+                                            0x00020000)) {}
+  __device__ __forceinline__ double operator[](uint32_t i) const {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)(i * 8u), 0, 0));
+  }
+};
+
 // Writable global pointer: data pointers read from device tables (rather
 // than kernel arguments) are otherwise flat, and flat stores may alias LDS,
 // which serialises them with the LDS exchanges.

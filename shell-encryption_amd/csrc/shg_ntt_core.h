@@ -154,6 +154,11 @@ struct IntArith {
   __device__ __forceinline__ uint64_t acc_out(Acc a) const { return a; }
 };
 
+// FP64 twiddles read with buffer loads (F64Table).
+#ifndef SHG_TW_BUFFER
+#define SHG_TW_BUFFER 1
+#endif
+
 // Twiddle products take their quotient from fl(v*w) (f64_mulmod_h); the
 // twiddle's w/q is then never formed.
 #ifndef SHG_F64_HQ
@@ -171,9 +176,16 @@ struct F64Arith {
   double q, qinv, ninv, ninvq, ninvx, ninvxq;
   gptr<double> fw;
   gptr<double> iw;
+#if SHG_TW_BUFFER
+  F64Table fwt, iwt;
+#endif
   __device__ __forceinline__ explicit F64Arith(const PrimeDev& P)
       : q(P.qd), qinv(P.qinv), ninv(P.n_inv_d), ninvq(P.n_inv_dq), ninvx(P.n_inv_x_d),
-        ninvxq(P.n_inv_x_dq), fw((gptr<double>)P.fwd), iw((gptr<double>)P.iwd) {}
+        ninvxq(P.n_inv_x_dq), fw((gptr<double>)P.fwd), iw((gptr<double>)P.iwd)
+#if SHG_TW_BUFFER
+        , fwt(P.fwd), iwt(P.iwd)
+#endif
+  {}
   struct Tw {
     double w, wq;
   };
@@ -182,8 +194,13 @@ struct F64Arith {
   // it wq = w * fl(1/q) is formed on use.  Either way three roundings, which
   // the bounds below absorb.
   using TwRaw = double;
+#if SHG_TW_BUFFER
+  __device__ __forceinline__ TwRaw fwd_tw_raw(int i) const { return fwt[(uint32_t)i]; }
+  __device__ __forceinline__ TwRaw inv_tw_raw(int i) const { return iwt[(uint32_t)i]; }
+#else
   __device__ __forceinline__ TwRaw fwd_tw_raw(int i) const { return fw[i]; }
   __device__ __forceinline__ TwRaw inv_tw_raw(int i) const { return iw[i]; }
+#endif
   __device__ __forceinline__ Tw tw(TwRaw r) const { return Tw{r, r * qinv}; }
   __device__ __forceinline__ Tw fwd_tw(int i) const { return tw(fw[i]); }
   __device__ __forceinline__ Tw inv_tw(int i) const { return tw(iw[i]); }
