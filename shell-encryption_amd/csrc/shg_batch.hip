@@ -19,6 +19,12 @@
 #include "shg_device.h"
 #include "shg_kernels.h"
 #include "shg_ntt_core.h"
+#include "shg_ntt_wave.h"
+
+// 2^14-word slices of FP64 jobs on the wave-exchange layout (shg_ntt_wave.h).
+#ifndef SHG_BATCH_WAVE
+#define SHG_BATCH_WAVE 1
+#endif
 
 namespace shg {
 
@@ -33,6 +39,10 @@ constexpr int kT = kBatchThreads;
 template <int LOGM, int HB, class A, class Ld, class St>
 __device__ __forceinline__ void fwd_slice_io(int h, const A& ar, typename A::V* lds, int t,
                                              const Ld& ld, const St& st) {
+  if constexpr (LOGM == 14 && std::is_same<A, F64Arith>::value && SHG_BATCH_WAVE) {
+    wave_fwd_slice<HB>(h, ar, lds, ld, st);
+    return;
+  }
   constexpr int LOGE = LOGM - 10;
   constexpr int E = 1 << LOGE;
   asm volatile("" : "+v"(t));  // keep this body's addressing local to it
@@ -61,6 +71,10 @@ __device__ __forceinline__ void fwd_slice_io(int h, const A& ar, typename A::V* 
 template <int LOGM, int HB, class A, class Ld, class St>
 __device__ __forceinline__ void inv_slice_io(int h, const A& ar, typename A::V* lds, int scale,
                                              int t, const Ld& ld, const St& st) {
+  if constexpr (LOGM == 14 && std::is_same<A, F64Arith>::value && SHG_BATCH_WAVE) {
+    wave_inv_slice<HB>(h, ar, lds, scale, ld, st);
+    return;
+  }
   using V = typename A::V;
   constexpr int LOGE = LOGM - 10;
   constexpr int E = 1 << LOGE;

@@ -363,7 +363,7 @@ __global__ void __launch_bounds__(64 << LW)
     uint64_t* g = data + ((size_t)item << LOGM);
     const int tp4 = Lay::thread_part(4, lane_o, wave);
     double tw1[T1::N];
-    wave_pass_tw_inv<LW, 4, 1, 0, false>(tw1, tp4, ar);
+    wave_pass_tw_inv<LW, 4, 1, 0, false>(tw1, tp4, 0, ar);
     double x[16];
 #pragma unroll
     for (int e = 0; e < 16; ++e) x[e] = ar.load(pf[e]);
@@ -373,7 +373,7 @@ __global__ void __launch_bounds__(64 << LW)
     __builtin_amdgcn_sched_barrier(0);
     const int tp3 = Lay::thread_part(3, lane_o, wave);
     double tw2[T2::N];
-    wave_pass_tw_inv<LW, 3, 3, 0, false>(tw2, tp3, ar);
+    wave_pass_tw_inv<LW, 3, 3, 0, false>(tw2, tp3, 0, ar);
     // Every wave has finished reading the previous limb's exchange.
     __syncthreads();
     wave_to_lds<LW, 4>(x, lds, tp4);
@@ -383,7 +383,7 @@ __global__ void __launch_bounds__(64 << LW)
     __builtin_amdgcn_sched_barrier(0);
     const int tp2 = Lay::thread_part(2, lane_o, wave);
     double tw3[T3::N];
-    wave_pass_tw_inv<LW, 2, LW - 1, 0, true>(tw3, wave << 10, ar);
+    wave_pass_tw_inv<LW, 2, LW - 1, 0, true>(tw3, wave << 10, 0, ar);
     wave_to_lds<LW, 3>(x, lds, tp3);
     wave_sync_lds();
     wave_from_lds<LW, 2>(x, lds, tp2);
@@ -391,7 +391,7 @@ __global__ void __launch_bounds__(64 << LW)
     __builtin_amdgcn_sched_barrier(0);
     const int tp1 = Lay::thread_part(1, lane_o, wave);
     double tw4[T4::N];
-    wave_pass_tw_inv<LW, 1, 3, 0, true>(tw4, 0, ar);
+    wave_pass_tw_inv<LW, 1, 3, 0, true>(tw4, 0, 0, ar);
     // Each wave writes only its own words (pass 2's waves own the top bits),
     // which no other wave reads before the barrier.
     wave_to_lds<LW, 2>(x, lds, tp2);

@@ -165,10 +165,15 @@ __device__ __forceinline__ void wave_pass(double (&x)[16], const double* tw, con
 // i + 2^b and uses psi_inv_bitrev[n - n/2^b + (i >> (b + 1))], which depends
 // on the same register bits above b as the forward twiddle of that bit, so
 // WavePassTw's slots serve both.  UNIFORM as in wave_pass_tw.
-template <int LW, int P, int KHI, int KLO, bool UNIFORM, class A>
-__device__ __forceinline__ void wave_pass_tw_inv(double* tw, int tw_tp, const A& ar) {
+//
+// HB: the slice is sub-block h of a limb of 2^(LOGM + HB) words (its last HB
+// stages run elsewhere); the twiddle index then carries h above the local
+// part: n - n/2^b + (h << (LOGM - 1 - b)) + (local index >> (b + 1)).
+template <int LW, int P, int KHI, int KLO, bool UNIFORM, int HB = 0, class A>
+__device__ __forceinline__ void wave_pass_tw_inv(double* tw, int tw_tp, int h, const A& ar) {
   using PT = WavePassTw<LW, P, KHI, KLO>;
-  constexpr int n = 1 << WaveLayout<LW>::LOGM;
+  constexpr int LOGM = WaveLayout<LW>::LOGM;
+  constexpr int n = 1 << (LOGM + HB);
   if constexpr (UNIFORM && SHG_NTT_WAVE_SMEM) {
     using cptr = const double __attribute__((address_space(4)))*;
     const cptr ct = (cptr)(size_t)(const void*)ar.iw;
@@ -177,8 +182,8 @@ __device__ __forceinline__ void wave_pass_tw_inv(double* tw, int tw_tp, const A&
       const int bit = WaveLayout<LW>::reg(P, kb);
 #pragma unroll
       for (int c = 0; c < (1 << PT::nhigh(kb)); ++c)
-        tw[PT::base(kb) + c] =
-            ct[(unsigned)(n - (n >> bit) + ((PT::combo_part(kb, c) | tw_tp) >> (bit + 1)))];
+        tw[PT::base(kb) + c] = ct[(unsigned)(n - (n >> bit) + (h << (LOGM - 1 - bit)) +
+                                             ((PT::combo_part(kb, c) | tw_tp) >> (bit + 1)))];
     }
   } else {
 #pragma unroll
@@ -186,8 +191,9 @@ __device__ __forceinline__ void wave_pass_tw_inv(double* tw, int tw_tp, const A&
       const int bit = WaveLayout<LW>::reg(P, kb);
 #pragma unroll
       for (int c = 0; c < (1 << PT::nhigh(kb)); ++c)
-        tw[PT::base(kb) + c] = ar.inv_tw_raw(
-            (unsigned)(n - (n >> bit) + ((PT::combo_part(kb, c) | tw_tp) >> (bit + 1))));
+        tw[PT::base(kb) + c] =
+            ar.inv_tw_raw((unsigned)(n - (n >> bit) + (h << (LOGM - 1 - bit)) +
+                                     ((PT::combo_part(kb, c) | tw_tp) >> (bit + 1))));
     }
   }
 }
@@ -233,6 +239,110 @@ __device__ __forceinline__ void wave_from_lds(double (&x)[16], const double* lds
   const int base = lpad(tp);
 #pragma unroll
   for (int e = 0; e < 16; ++e) x[e] = lds[base + cpad(Lay::reg_part(P, e))];
+}
+
+// Whole-workgroup transforms of one 2^14-word slice (LW = 4: 1024 threads)
+// for kernels that run many differently sized jobs in one launch.  Words
+// come from ld(p) and go to st(p, v) by slice position p: forward ld gives
+// the policy's representation and st takes canonical words; inverse ld gives
+// the raw residue and st takes inv_out(x, scale).  Both start with no LDS
+// write before a barrier that other waves' earlier reads would need, and end
+// with a barrier after their last LDS reads, so the caller may run any other
+// LDS job right after (as after the register/LDS passes of shg_ntt_core.h).
+template <int HB, class A, class Ld, class St>
+__device__ __forceinline__ void wave_fwd_slice(int h, const A& ar, double* lds, const Ld& ld,
+                                               const St& st) {
+  using Lay = WaveLayout<4>;
+  using T1 = WavePassTw<4, 1, 3, 0>;
+  using T2 = WavePassTw<4, 2, 3, 0>;
+  using T3 = WavePassTw<4, 3, 3, 0>;
+  using T4 = WavePassTw<4, 4, 1, 0>;
+  int lane_o = threadIdx.x & 63;
+  asm volatile("" : "+v"(lane_o));
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int tp1 = Lay::thread_part(1, lane_o, wave);
+  double tw1[T1::N];
+  wave_pass_tw<4, 1, 3, 0, true, HB>(tw1, 0, h, ar);
+  double x[16];
+#pragma unroll
+  for (int e = 0; e < 16; ++e) x[e] = ld(tp1 + Lay::reg_part(1, e));
+  wave_pass<4, 1, 3, 0, HB>(x, tw1, ar);
+  __builtin_amdgcn_sched_barrier(0);
+  double tw2[T2::N];
+  wave_pass_tw<4, 2, 3, 0, true, HB>(tw2, wave << 10, h, ar);
+  __syncthreads();
+  wave_to_lds<4, 1>(x, lds, tp1);
+  __syncthreads();
+  const int tp2 = Lay::thread_part(2, lane_o, wave);
+  wave_from_lds<4, 2>(x, lds, tp2);
+  wave_pass<4, 2, 3, 0, HB>(x, tw2, ar);
+  const int tp3 = Lay::thread_part(3, lane_o, wave);
+  __builtin_amdgcn_sched_barrier(0);
+  double tw3[T3::N];
+  wave_pass_tw<4, 3, 3, 0, false, HB>(tw3, tp3, h, ar);
+  wave_to_lds<4, 2>(x, lds, tp2);
+  wave_sync_lds();
+  wave_from_lds<4, 3>(x, lds, tp3);
+  wave_pass<4, 3, 3, 0, HB>(x, tw3, ar);
+  const int tp4 = Lay::thread_part(4, lane_o, wave);
+  __builtin_amdgcn_sched_barrier(0);
+  double tw4[T4::N];
+  wave_pass_tw<4, 4, 1, 0, false, HB>(tw4, tp4, h, ar);
+  wave_to_lds<4, 3>(x, lds, tp3);
+  wave_sync_lds();
+  wave_from_lds<4, 4>(x, lds, tp4);
+  pin_words(x);
+  wave_pass<4, 4, 1, 0, HB>(x, tw4, ar);
+#pragma unroll
+  for (int e = 0; e < 16; ++e) st(tp4 + Lay::reg_part(4, e & ~3) + (e & 3), ar.fwd_out(x[e]));
+  __syncthreads();
+}
+
+template <int HB, class A, class Ld, class St>
+__device__ __forceinline__ void wave_inv_slice(int h, const A& ar, double* lds, int scale,
+                                               const Ld& ld, const St& st) {
+  using Lay = WaveLayout<4>;
+  using T1 = WavePassTw<4, 4, 1, 0>;
+  using T2 = WavePassTw<4, 3, 3, 0>;
+  using T3 = WavePassTw<4, 2, 3, 0>;
+  using T4 = WavePassTw<4, 1, 3, 0>;
+  int lane_o = threadIdx.x & 63;
+  asm volatile("" : "+v"(lane_o));
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int tp4 = Lay::thread_part(4, lane_o, wave);
+  double tw1[T1::N];
+  wave_pass_tw_inv<4, 4, 1, 0, false, HB>(tw1, tp4, h, ar);
+  double x[16];
+#pragma unroll
+  for (int e = 0; e < 16; ++e) x[e] = ar.load(ld(tp4 + Lay::reg_part(4, e & ~3) + (e & 3)));
+  wave_pass_inv<4, 4, 1, 0>(x, tw1, ar);
+  __builtin_amdgcn_sched_barrier(0);
+  const int tp3 = Lay::thread_part(3, lane_o, wave);
+  double tw2[T2::N];
+  wave_pass_tw_inv<4, 3, 3, 0, false, HB>(tw2, tp3, h, ar);
+  wave_to_lds<4, 4>(x, lds, tp4);
+  wave_sync_lds();
+  wave_from_lds<4, 3>(x, lds, tp3);
+  wave_pass_inv<4, 3, 3, 0>(x, tw2, ar);
+  __builtin_amdgcn_sched_barrier(0);
+  const int tp2 = Lay::thread_part(2, lane_o, wave);
+  double tw3[T3::N];
+  wave_pass_tw_inv<4, 2, 3, 0, true, HB>(tw3, wave << 10, h, ar);
+  wave_to_lds<4, 3>(x, lds, tp3);
+  wave_sync_lds();
+  wave_from_lds<4, 2>(x, lds, tp2);
+  wave_pass_inv<4, 2, 3, 0>(x, tw3, ar);
+  __builtin_amdgcn_sched_barrier(0);
+  const int tp1 = Lay::thread_part(1, lane_o, wave);
+  double tw4[T4::N];
+  wave_pass_tw_inv<4, 1, 3, 0, true, HB>(tw4, 0, h, ar);
+  wave_to_lds<4, 2>(x, lds, tp2);
+  __syncthreads();
+  wave_from_lds<4, 1>(x, lds, tp1);
+  wave_pass_inv<4, 1, 3, 0>(x, tw4, ar);
+#pragma unroll
+  for (int e = 0; e < 16; ++e) st(tp1 + Lay::reg_part(1, e), ar.inv_out(x[e], scale));
+  __syncthreads();
 }
 
 }  // namespace shg
