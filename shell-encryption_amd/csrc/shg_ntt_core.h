@@ -215,6 +215,12 @@ struct F64Arith {
     const double t = SHG_F64_MULMOD(v, w);
     return u + (low ? -t : t);
   }
+  // One output of a Cooley-Tukey butterfly with the side chosen by a
+  // uniform sign (+1: u + w v, -1: u - w v): one FMA, where a select of -t
+  // or t costs a sign flip and a 64-bit select per word.
+  __device__ __forceinline__ V ct_side(V u, V v, const Tw& w, double sgn) const {
+    return fma(sgn, SHG_F64_MULMOD(v, w), u);
+  }
   __device__ __forceinline__ void gs(V& u, V& v, const Tw& w) const {
     const double s = u + v;
     const double d = u - v;
