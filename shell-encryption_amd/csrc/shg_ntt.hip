@@ -397,18 +397,8 @@ __global__ void __launch_bounds__(64 << LW)
     wave_to_lds<LW, 2>(x, lds, tp2);
     __syncthreads();
     wave_from_lds<LW, 1>(x, lds, tp1);
-    wave_pass_inv<LW, 1, 3, 0>(x, tw4, ar);
-    auto store_all = [&](auto mode) {
-#pragma unroll
-      for (int e = 0; e < 16; ++e)
-        g[tp1 + Lay::reg_part(1, e)] = ar.inv_out(x[e], decltype(mode)::value);
-    };
-    if (scale == 1)
-      store_all(std::integral_constant<int, 1>());
-    else if (scale == 2)
-      store_all(std::integral_constant<int, 2>());
-    else
-      store_all(std::integral_constant<int, 0>());
+    wave_inv_last_pass_store<LW>(x, tw4, scale, ar,
+                                 [&](int e, uint64_t v) { g[tp1 + Lay::reg_part(1, e)] = v; });
   }
 }
 

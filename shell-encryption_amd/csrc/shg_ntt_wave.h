@@ -198,13 +198,14 @@ __device__ __forceinline__ void wave_pass_tw_inv(double* tw, int tw_tp, int h, c
   }
 }
 
-// Stages on register bits KLO up to KHI; the reduction schedule follows the
-// index bit (the inverse stage number).
-template <int LW, int P, int KHI, int KLO, class A>
+// Stages on register bits KLO up to KRUN (default KHI; the twiddles are laid
+// out for KHI..KLO); the reduction schedule follows the index bit (the
+// inverse stage number).
+template <int LW, int P, int KHI, int KLO, int KRUN = KHI, class A>
 __device__ __forceinline__ void wave_pass_inv(double (&x)[16], const double* tw, const A& ar) {
   using PT = WavePassTw<LW, P, KHI, KLO>;
 #pragma unroll
-  for (int kb = KLO; kb <= KHI; ++kb) {
+  for (int kb = KLO; kb <= KRUN; ++kb) {
     const int bit = WaveLayout<LW>::reg(P, kb);
 #pragma unroll
     for (int e = 0; e < 16; ++e) {
@@ -216,6 +217,39 @@ __device__ __forceinline__ void wave_pass_inv(double (&x)[16], const double* tw,
       for (int e = 0; e < 16; ++e) ar.inv_reduce(x[e]);
     }
   }
+}
+
+// The last inverse stage (the top index bit: register bit 3 of pass 1's
+// layout, a single twiddle w) with an output scale s folded in:
+// u' = (u + v) s, v' = (u - v)(w s), one product per word where the stage
+// and a separate scaling take three per pair.
+template <class A>
+__device__ __forceinline__ void wave_last_inv_scaled(double (&x)[16], double w, double s,
+                                                     const A& ar) {
+  const double ws = f64_mulmod_h(w, s, ar.q, ar.qinv);
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    const double a = x[e] + x[e + 8];
+    const double b = x[e] - x[e + 8];
+    x[e] = f64_mulmod_h(a, s, ar.q, ar.qinv);
+    x[e + 8] = f64_mulmod_h(b, ws, ar.q, ar.qinv);
+  }
+}
+
+// Pass 1 of an inverse (top 4 bits) followed by the stores, with the
+// scale (0: none, 1: n^-1, 2: n^-1 2^-64) folded into the last stage.
+template <int LW, class A, class St>
+__device__ __forceinline__ void wave_inv_last_pass_store(double (&x)[16], const double* tw4,
+                                                         int scale, const A& ar, const St& st) {
+  using T4 = WavePassTw<LW, 1, 3, 0>;
+  if (scale == 0) {
+    wave_pass_inv<LW, 1, 3, 0>(x, tw4, ar);
+  } else {
+    wave_pass_inv<LW, 1, 3, 0, 2>(x, tw4, ar);
+    wave_last_inv_scaled(x, tw4[T4::slot(3, 0)], scale == 1 ? ar.ninv : ar.ninvx, ar);
+  }
+#pragma unroll
+  for (int e = 0; e < 16; ++e) st(e, ar.fwd_out(x[e]));
 }
 
 // Orders a wave's LDS writes before its following reads (other lanes').
@@ -339,9 +373,8 @@ __device__ __forceinline__ void wave_inv_slice(int h, const A& ar, double* lds, 
   wave_to_lds<4, 2>(x, lds, tp2);
   __syncthreads();
   wave_from_lds<4, 1>(x, lds, tp1);
-  wave_pass_inv<4, 1, 3, 0>(x, tw4, ar);
-#pragma unroll
-  for (int e = 0; e < 16; ++e) st(tp1 + Lay::reg_part(1, e), ar.inv_out(x[e], scale));
+  wave_inv_last_pass_store<4>(x, tw4, scale, ar,
+                              [&](int e, uint64_t v) { st(tp1 + Lay::reg_part(1, e), v); });
   __syncthreads();
 }
 
