@@ -223,8 +223,11 @@ def test_gadget_decompose(ctx):
     assert np.array_equal(dg.download(), exp)
 
 
+# Bases above 2^16 take the path without digit planes: an inverse transform
+# with the ExportInt scaling fused (n^-1 2^-64), then the key switch.
 @pytest.mark.parametrize("log_n,L,log_b,count", [(6, 2, 10, 3), (12, 3, 10, 2), (12, 2, 10, 64),
-                                                 (13, 4, 10, 2), (14, 8, 10, 2), (15, 2, 25, 1)])
+                                                 (13, 4, 10, 2), (14, 8, 10, 2), (15, 2, 25, 1),
+                                                 (13, 2, 20, 2), (14, 3, 20, 1)])
 def test_relin_apply(ctx, log_n, L, log_b, count):
     qs, rns = _rns(log_n, L)
     pids = ctx.primes(qs, log_n)
