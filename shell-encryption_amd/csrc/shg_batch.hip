@@ -103,7 +103,12 @@ __device__ __forceinline__ void inv_slice_io(int h, const A& ar, typename A::V* 
 // the limb itself when ld reads it), then the halves run as 2^14-word
 // slices; __syncthreads makes every thread's global writes visible to the
 // whole workgroup.
-template <class A, class Ld, class St>
+//
+// HOLD (FP64 wave path): for n = 2^15 each thread reads the pairs (p, p + H)
+// of its pass-1 positions, applies stage 0, runs half 0 from the sums and
+// keeps the differences in VGPRs for half 1: the limb is read and written
+// once, with no pass through tmp.
+template <bool HOLD = false, class A, class Ld, class St>
 __device__ __forceinline__ void fwd_io(int log_n, const A& ar, typename A::V* lds, int t,
                                        gmut<uint64_t> tmp, const Ld& ld, const St& st) {
   switch (log_n) {
@@ -113,6 +118,34 @@ __device__ __forceinline__ void fwd_io(int log_n, const A& ar, typename A::V* ld
     case 14: fwd_slice_io<14, 0>(0, ar, lds, t, ld, st); break;
     case 15: {
       constexpr int H = 1 << 14;
+      if constexpr (HOLD && std::is_same<A, F64Arith>::value && SHG_BATCH_WAVE) {
+        using Lay = WaveLayout<4>;
+        const auto w = ar.fwd_tw(1);
+        const int tp1 = Lay::thread_part(1, (int)threadIdx.x & 63,
+                                         __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6));
+        double x[16], held[16];
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+          const int p = tp1 + Lay::reg_part(1, e);
+          double u = ld(p);
+          double v = ld(p + H);
+          ar.ct(u, v, w);
+          x[e] = u;
+          held[e] = v;
+        }
+        auto none = [](int) { return 0.0; };
+        // Every thread reads its inputs before the core's first barrier, so
+        // half 0's stores cannot reach a word that is still to be read.
+        const int tp4 = wave_fwd_core<1, true>(0, ar, lds, none, x);
+#pragma unroll
+        for (int e = 0; e < 16; ++e) st(tp4 + Lay::reg_part(4, e), ar.fwd_out(x[e]));
+        __syncthreads();
+        wave_fwd_core<1, true>(1, ar, lds, none, held);
+#pragma unroll
+        for (int e = 0; e < 16; ++e) st(H + tp4 + Lay::reg_part(4, e), ar.fwd_out(held[e]));
+        __syncthreads();
+        break;
+      }
       const auto w = ar.fwd_tw(1);
       for (int i = t; i < H; i += kT) {
         auto u = ld(i);
@@ -135,7 +168,12 @@ __device__ __forceinline__ void fwd_io(int log_n, const A& ar, typename A::V* ld
 // (raw residue) and whose output goes to st(p, v).  For n = 2^15 the halves
 // run as slices into tmp, then the last Gentleman-Sande stage (pairs i,
 // i + 2^14, twiddle index n - 2) applies the scaling.
-template <class A, class Ld, class St>
+//
+// HOLD (FP64 wave path): for n = 2^15 both halves' slices end with their
+// words in VGPRs at the same positions, so the last stage (pairs i,
+// i + 2^14) runs in registers with the n^-1 scaling folded in, and the limb
+// is read and written once.
+template <bool HOLD = false, class A, class Ld, class St>
 __device__ __forceinline__ void inv_io(int log_n, const A& ar, typename A::V* lds, int t,
                                        gmut<uint64_t> tmp, const Ld& ld, const St& st) {
   switch (log_n) {
@@ -145,6 +183,28 @@ __device__ __forceinline__ void inv_io(int log_n, const A& ar, typename A::V* ld
     case 14: inv_slice_io<14, 0>(0, ar, lds, 1, t, ld, st); break;
     case 15: {
       constexpr int H = 1 << 14;
+      if constexpr (HOLD && std::is_same<A, F64Arith>::value && SHG_BATCH_WAVE) {
+        using Lay = WaveLayout<4>;
+        double y0[16], y1[16], tw4[15];
+        const int tp1 = wave_inv_core<1>(0, ar, lds, ld, y0, tw4);
+        wave_pass_inv<4, 1, 3, 0>(y0, tw4, ar);
+        __syncthreads();
+        wave_inv_core<1>(1, ar, lds, [&](int p) { return ld(H + p); }, y1, tw4);
+        wave_pass_inv<4, 1, 3, 0>(y1, tw4, ar);
+        __syncthreads();
+        // Stage on index bit 14 (twiddle index 2n' - 2 of the 2^15 table)
+        // with the n^-1 scale: (u + v) s, (u - v)(w s).  Inputs are four
+        // Gentleman-Sande stages past the reduction at bit 10, below 8q.
+        const double s = ar.ninv;
+        const double ws = f64_mulmod_h(ar.inv_tw_raw(2 * H - 2), s, ar.q, ar.qinv);
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+          const int p = tp1 + Lay::reg_part(1, e);
+          st(p, ar.fwd_out(f64_mulmod_h(y0[e] + y1[e], s, ar.q, ar.qinv)));
+          st(H + p, ar.fwd_out(f64_mulmod_h(y0[e] - y1[e], ws, ar.q, ar.qinv)));
+        }
+        break;
+      }
       inv_slice_io<14, 1>(0, ar, lds, 0, t, ld, [&](int p, uint64_t v) { tmp[p] = v; });
       inv_slice_io<14, 1>(1, ar, lds, 0, t, [&](int p) { return ld(H + p); },
                           [&](int p, uint64_t v) { tmp[H + p] = v; });
@@ -166,15 +226,15 @@ __device__ __forceinline__ void inv_io(int log_n, const A& ar, typename A::V* ld
 template <class A>
 __device__ __forceinline__ void fwd_job(gmut<uint64_t> g, int log_n, const A& ar,
                                         typename A::V* lds, int t) {
-  fwd_io(log_n, ar, lds, t, g, [&](int p) { return ar.load(g[p]); },
-         [&](int p, uint64_t v) { g[p] = v; });
+  fwd_io<true>(log_n, ar, lds, t, g, [&](int p) { return ar.load(g[p]); },
+               [&](int p, uint64_t v) { g[p] = v; });
 }
 
 template <class A>
 __device__ __forceinline__ void inv_job(gmut<uint64_t> g, int log_n, const A& ar,
                                         typename A::V* lds, int t) {
-  inv_io(log_n, ar, lds, t, g, [&](int p) { return (uint64_t)g[p]; },
-         [&](int p, uint64_t v) { g[p] = v; });
+  inv_io<true>(log_n, ar, lds, t, g, [&](int p) { return (uint64_t)g[p]; },
+               [&](int p, uint64_t v) { g[p] = v; });
 }
 
 template <class A>

@@ -283,9 +283,14 @@ __device__ __forceinline__ void wave_from_lds(double (&x)[16], const double* lds
 // write before a barrier that other waves' earlier reads would need, and end
 // with a barrier after their last LDS reads, so the caller may run any other
 // LDS job right after (as after the register/LDS passes of shg_ntt_core.h).
-template <int HB, class A, class Ld, class St>
-__device__ __forceinline__ void wave_fwd_slice(int h, const A& ar, double* lds, const Ld& ld,
-                                               const St& st) {
+// The forward slice's passes: x receives the transformed (unreduced) words,
+// word e at slice position (return value) + reg_part(4, e).  No barrier
+// after the last LDS reads.  PRELOADED: x already holds the input word of
+// slice position thread_part(1, lane, wave) + reg_part(1, e) in x[e], and
+// ld is not called.
+template <int HB, bool PRELOADED = false, class A, class Ld>
+__device__ __forceinline__ int wave_fwd_core(int h, const A& ar, double* lds, const Ld& ld,
+                                             double (&x)[16]) {
   using Lay = WaveLayout<4>;
   using T1 = WavePassTw<4, 1, 3, 0>;
   using T2 = WavePassTw<4, 2, 3, 0>;
@@ -297,9 +302,10 @@ __device__ __forceinline__ void wave_fwd_slice(int h, const A& ar, double* lds, 
   const int tp1 = Lay::thread_part(1, lane_o, wave);
   double tw1[T1::N];
   wave_pass_tw<4, 1, 3, 0, true, HB>(tw1, 0, h, ar);
-  double x[16];
+  if constexpr (!PRELOADED) {
 #pragma unroll
-  for (int e = 0; e < 16; ++e) x[e] = ld(tp1 + Lay::reg_part(1, e));
+    for (int e = 0; e < 16; ++e) x[e] = ld(tp1 + Lay::reg_part(1, e));
+  }
   wave_pass<4, 1, 3, 0, HB>(x, tw1, ar);
   __builtin_amdgcn_sched_barrier(0);
   double tw2[T2::N];
@@ -327,28 +333,39 @@ __device__ __forceinline__ void wave_fwd_slice(int h, const A& ar, double* lds, 
   wave_from_lds<4, 4>(x, lds, tp4);
   pin_words(x);
   wave_pass<4, 4, 1, 0, HB>(x, tw4, ar);
-#pragma unroll
-  for (int e = 0; e < 16; ++e) st(tp4 + Lay::reg_part(4, e & ~3) + (e & 3), ar.fwd_out(x[e]));
-  __syncthreads();
+  return tp4;
 }
 
 template <int HB, class A, class Ld, class St>
-__device__ __forceinline__ void wave_inv_slice(int h, const A& ar, double* lds, int scale,
-                                               const Ld& ld, const St& st) {
+__device__ __forceinline__ void wave_fwd_slice(int h, const A& ar, double* lds, const Ld& ld,
+                                               const St& st) {
+  double x[16];
+  const int tp4 = wave_fwd_core<HB>(h, ar, lds, ld, x);
+#pragma unroll
+  for (int e = 0; e < 16; ++e) st(tp4 + WaveLayout<4>::reg_part(4, e), ar.fwd_out(x[e]));
+  __syncthreads();
+}
+
+// The inverse slice's passes up to pass 1's words in VGPRs: x holds word e
+// of slice position (return value) + reg_part(1, e), tw4 pass 1's twiddles.
+// The caller runs pass 1 (wave_pass_inv<4, 1, 3, 0> or
+// wave_inv_last_pass_store) and a barrier before the next LDS write.
+template <int HB, class A, class Ld>
+__device__ __forceinline__ int wave_inv_core(int h, const A& ar, double* lds, const Ld& ld,
+                                             double (&x)[16], double (&tw4)[15]) {
   using Lay = WaveLayout<4>;
   using T1 = WavePassTw<4, 4, 1, 0>;
   using T2 = WavePassTw<4, 3, 3, 0>;
   using T3 = WavePassTw<4, 2, 3, 0>;
-  using T4 = WavePassTw<4, 1, 3, 0>;
+  static_assert(WavePassTw<4, 1, 3, 0>::N == 15, "pass 1 has 15 twiddles");
   int lane_o = threadIdx.x & 63;
   asm volatile("" : "+v"(lane_o));
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int tp4 = Lay::thread_part(4, lane_o, wave);
   double tw1[T1::N];
   wave_pass_tw_inv<4, 4, 1, 0, false, HB>(tw1, tp4, h, ar);
-  double x[16];
 #pragma unroll
-  for (int e = 0; e < 16; ++e) x[e] = ar.load(ld(tp4 + Lay::reg_part(4, e & ~3) + (e & 3)));
+  for (int e = 0; e < 16; ++e) x[e] = ar.load(ld(tp4 + Lay::reg_part(4, e)));
   wave_pass_inv<4, 4, 1, 0>(x, tw1, ar);
   __builtin_amdgcn_sched_barrier(0);
   const int tp3 = Lay::thread_part(3, lane_o, wave);
@@ -368,13 +385,21 @@ __device__ __forceinline__ void wave_inv_slice(int h, const A& ar, double* lds, 
   wave_pass_inv<4, 2, 3, 0>(x, tw3, ar);
   __builtin_amdgcn_sched_barrier(0);
   const int tp1 = Lay::thread_part(1, lane_o, wave);
-  double tw4[T4::N];
   wave_pass_tw_inv<4, 1, 3, 0, true, HB>(tw4, 0, h, ar);
   wave_to_lds<4, 2>(x, lds, tp2);
   __syncthreads();
   wave_from_lds<4, 1>(x, lds, tp1);
-  wave_inv_last_pass_store<4>(x, tw4, scale, ar,
-                              [&](int e, uint64_t v) { st(tp1 + Lay::reg_part(1, e), v); });
+  return tp1;
+}
+
+template <int HB, class A, class Ld, class St>
+__device__ __forceinline__ void wave_inv_slice(int h, const A& ar, double* lds, int scale,
+                                               const Ld& ld, const St& st) {
+  double x[16], tw4[15];
+  const int tp1 = wave_inv_core<HB>(h, ar, lds, ld, x, tw4);
+  wave_inv_last_pass_store<4>(x, tw4, scale, ar, [&](int e, uint64_t v) {
+    st(tp1 + WaveLayout<4>::reg_part(1, e), v);
+  });
   __syncthreads();
 }
 
