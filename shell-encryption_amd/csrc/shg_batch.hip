@@ -310,7 +310,7 @@ __global__ void __launch_bounds__(kT)
         arl.ninv = job.f_kl;
         arl.ninvq = job.f_kl * arl.qinv;
         const uint64_t half_ql = Pl.q >> 1;
-        inv_io(log_n, arl, lds, t, s0, [&](int p) { return (uint64_t)al[p]; },
+        inv_io<true>(log_n, arl, lds, t, s0, [&](int p) { return (uint64_t)al[p]; },
                [&](int p, uint64_t v) {
                  const double d = u52_to_f64(v) - (v > half_ql ? arl.q : 0.0);
                  s0[p] = (uint64_t)__double_as_longlong(d);
@@ -321,7 +321,7 @@ __global__ void __launch_bounds__(kT)
           const gmut<uint64_t> ai = a + (size_t)i * n;
           const double c1 = job.f_c1[i], c1q = c1 * ari.qinv;
           const double c2 = job.f_c2[i], c2q = c2 * ari.qinv;
-          fwd_io(log_n, ari, lds, t, s1,
+          fwd_io<true>(log_n, ari, lds, t, s1,
                  [&](int p) {
                    return f64_reduce(__longlong_as_double((long long)(uint64_t)s0[p]), ari.q,
                                      ari.qinv);
