@@ -42,6 +42,10 @@
 #ifndef SHG_NTT_WAVE_PF
 #define SHG_NTT_WAVE_PF 1
 #endif
+// The key switch's inverse + digit planes at 2^14 (FP64) on the wave layout.
+#ifndef SHG_DIG_WAVE
+#define SHG_DIG_WAVE 1
+#endif
 // FP64 inverse transforms of 2^13 and 2^14 limbs on the wave-exchange
 // kernel by default (kNttCfgWave selects it at 2^12..2^14 either way):
 // 0.70 vs 0.75 ms per 8192 limbs of 2^14, 0.67 vs 0.72 at 2^13; at 2^12 the
@@ -506,20 +510,34 @@ __global__ void __launch_bounds__((1 << LOGM) >> LOGE)
   const int t = threadIdx.x;
   const A ar(primes[lp.id[l]]);
   const uint64_t* g = in + (size_t)ct * in_stride + ((size_t)l << LOGM);
+  if constexpr (LOGM == 14 && std::is_same<A, F64Arith>::value && SHG_DIG_WAVE) {
+    // The wave-exchange inverse (the exported scale n^-1 2^-64 folded into
+    // its last stage); its words reach LDS in natural order once every
+    // thread has read the workgroup-wide exchange.
+    using Lay = WaveLayout<4>;
+    double x[16], tw4[15];
+    uint64_t outv[16];
+    const int tp1 = wave_inv_core<0>(0, ar, lds, [&](int p) { return g[p]; }, x, tw4);
+    wave_inv_last_pass_store<4>(x, tw4, 2, ar, [&](int e, uint64_t v) { outv[e] = v; });
+    __syncthreads();
 #pragma unroll
-  for (int e = 0; e < E; ++e) lds[lpad(e * T + t)] = ar.load(g[e * T + t]);
-  __syncthreads();
-  V x[E];
-  InvPasses<LOGM, LOGE, 0, A, false, LOGM>::run(x, lds, t, 0, ar);
-  // The last pass read its exchange before its barrier: LDS is free.
-  constexpr int I0 = InvLast<LOGM, LOGE, 0, A>::value;
-  constexpr int R = LOGM - I0;
-  constexpr int G = E >> R;
+    for (int e = 0; e < 16; ++e) lds_u[lpad(tp1 + Lay::reg_part(1, e))] = outv[e];
+  } else {
 #pragma unroll
-  for (int gg = 0; gg < G; ++gg) {
+    for (int e = 0; e < E; ++e) lds[lpad(e * T + t)] = ar.load(g[e * T + t]);
+    __syncthreads();
+    V x[E];
+    InvPasses<LOGM, LOGE, 0, A, false, LOGM>::run(x, lds, t, 0, ar);
+    // The last pass read its exchange before its barrier: LDS is free.
+    constexpr int I0 = InvLast<LOGM, LOGE, 0, A>::value;
+    constexpr int R = LOGM - I0;
+    constexpr int G = E >> R;
 #pragma unroll
-    for (int e = 0; e < (1 << R); ++e)
-      lds_u[lpad((e << I0) | (gg * T + t))] = ar.inv_out(x[gg * (1 << R) + e], 2);
+    for (int gg = 0; gg < G; ++gg) {
+#pragma unroll
+      for (int e = 0; e < (1 << R); ++e)
+        lds_u[lpad((e << I0) | (gg * T + t))] = ar.inv_out(x[gg * (1 << R) + e], 2);
+    }
   }
   __syncthreads();
   uint64_t v[E];
