@@ -457,15 +457,30 @@ __global__ void __launch_bounds__(64 << LW)
     pin_words(x);
     wave_pass<LW, 4, 1, 0, HB>(x, tw4, ar);
     // Multiply-accumulate with key row j.
+#if SHG_KS_KEY_BUF
+    const __amdgpu_buffer_rsrc_t rb = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<uint64_t*>(ka.key_b + (size_t)(ka.key_base + j) * poly_words + limb_off),
+        (short)0, 0x7fffffff, 0x00020000);
+    const __amdgpu_buffer_rsrc_t ra = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<uint64_t*>(ka.key_a + (size_t)(ka.key_base + j) * poly_words + limb_off),
+        (short)0, 0x7fffffff, 0x00020000);
+#else
     const uint64_t* kb = ka.key_b + (size_t)(ka.key_base + j) * poly_words + limb_off + tp4;
     const uint64_t* kap = ka.key_a + (size_t)(ka.key_base + j) * poly_words + limb_off + tp4;
+#endif
 #pragma unroll
     for (int gg = 0; gg < 4; ++gg) {
       const int off = Lay::reg_part(4, gg * 4);
 #pragma unroll
       for (int e = 0; e < 4; e += 2) {
+#if SHG_KS_KEY_BUF
+        const int bo = (int)((uint32_t)(tp4 + off + e) * 8u);
+        const u64x2 kbv = __builtin_bit_cast(u64x2, __builtin_amdgcn_raw_buffer_load_b128(rb, bo, 0, 0));
+        const u64x2 kav = __builtin_bit_cast(u64x2, __builtin_amdgcn_raw_buffer_load_b128(ra, bo, 0, 0));
+#else
         const u64x2 kbv = *reinterpret_cast<const u64x2*>(kb + off + e);
         const u64x2 kav = *reinterpret_cast<const u64x2*>(kap + off + e);
+#endif
         const int r = gg * 4 + e;
         typename A::Prep d0, d1;
         if constexpr (kPrep) {
