@@ -210,6 +210,39 @@ def test_tensor_product(ctx):
     assert np.array_equal(out.download(), exp)
 
 
+def test_tensor_product_batch_n8192_decrypts(ctx):
+    """The batched tensor-product workload at its own shape: n = 2^13, L = 4
+    primes, t = 65537, 1024 two-component ciphertexts (eight encrypted pairs
+    tiled) multiplied in NTT form and brought back by the inverse NTT on the
+    GPU; sampled ciphertexts bit-exact against the oracle and decrypting to
+    the negacyclic product of the messages mod t."""
+    log_n, L, t, B, distinct = 13, 4, 65537, 1024, 8
+    qs, rns = _rns(log_n, L)
+    pids = ctx.primes(qs, log_n)
+    bgv = Bgv(rns, t, seed=11)
+    r = np.random.default_rng(12)
+    ms = [(r.integers(0, t, rns.n), r.integers(0, t, rns.n)) for _ in range(distinct)]
+    c1 = np.stack([bgv.encrypt(m[0]) for m in ms])
+    c2 = np.stack([bgv.encrypt(m[1]) for m in ms])
+    reps = B // distinct
+    da, db = ctx.array(np.tile(c1, (reps, 1, 1, 1))), ctx.array(np.tile(c2, (reps, 1, 1, 1)))
+    prod = ctx.empty((B, 3, L, rns.n))
+    ctx.tensor_product(da, 2, db, 2, prod, B, pids)
+    ntt_form = prod.download()
+    ctx.ntt_inv(prod, B * 3, pids)
+    coeff = prod.download()
+    for c in (0, 5, B - 1):
+        i = c % distinct
+        exp = rns.tensor_product(c1[i], c2[i])
+        assert np.array_equal(ntt_form[c], exp), c
+        for j in range(3):
+            assert np.array_equal(coeff[c, j], rns.ntt_inv(exp[j])), (c, j)
+    for c in (3, B - 2):
+        i = c % distinct
+        expect = np.array(negacyclic_mul_mod_t(ms[i][0], ms[i][1], t), dtype=np.int64)
+        assert np.array_equal(bgv.decrypt(ntt_form[c]), expect), c
+
+
 def test_gadget_decompose(ctx):
     qs, rns = _rns(10, 3)
     pids = ctx.primes(qs, 10)
