@@ -210,6 +210,30 @@ def test_tensor_product(ctx):
     assert np.array_equal(out.download(), exp)
 
 
+@pytest.mark.parametrize("variant", [sg.NTT_F64, sg.NTT_INT])
+def test_polynomial_product_n4096(ctx, variant):
+    """The plain negacyclic product at its own shape: n = 2^12, one prime
+    (the largest below 2^50 with q = 1 mod 2^13), 64 pairs: forward NTT of
+    both, pointwise Montgomery product, inverse NTT, every word against the
+    oracle; and the NTT -> INTT round trip is the identity."""
+    qs, rns = _rns(12, 1)
+    pids = ctx.primes(qs, 12)
+    pairs = 64
+    a, b = rns.uniform(41, pairs), rns.uniform(42, pairs)
+    da, db = ctx.array(a), ctx.array(b)
+    ctx.ntt_fwd(da, pairs, pids, variant)
+    ctx.ntt_fwd(db, pairs, pids, variant)
+    assert np.array_equal(db.download()[pairs - 1], rns.ntt_fwd(b[pairs - 1]))
+    ctx.pointwise(sg.OP_MUL, da, db, None, pairs, pids)  # MulInPlace
+    ctx.ntt_inv(da, pairs, pids, variant)
+    got = da.download()
+    for c in range(pairs):
+        exp = rns.ntt_inv(rns.pointwise("mul", rns.ntt_fwd(a[c]), rns.ntt_fwd(b[c])))
+        assert np.array_equal(got[c], exp), c
+    ctx.ntt_inv(db, pairs, pids, variant)  # round trip of b
+    assert np.array_equal(db.download(), b)
+
+
 def test_tensor_product_batch_n8192_decrypts(ctx):
     """The batched tensor-product workload at its own shape: n = 2^13, L = 4
     primes, t = 65537, 1024 two-component ciphertexts (eight encrypted pairs
