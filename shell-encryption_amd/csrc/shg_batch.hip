@@ -365,7 +365,50 @@ __global__ void __launch_bounds__(kT)
   }
 }
 
+// Whole 2^15 limbs with FP64 arithmetic, persistent over the limbs: the
+// register-held forms of fwd_io / inv_io (a limb crosses HBM once each way,
+// the inverse scaled by n^-1).  Limb `item` uses prime lp.id[item % num_limbs].
+__global__ void __launch_bounds__(kT)
+    ntt15_kernel(uint64_t* __restrict__ data, const PrimeDev* __restrict__ primes,
+                 LimbPrimes lp, int num_limbs, int items, int inverse) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  auto* lds = reinterpret_cast<double*>(smem);
+  for (int item = blockIdx.x; item < items; item += gridDim.x) {
+    int t = threadIdx.x;
+    asm volatile("" : "+v"(t));
+    const F64Arith ar(primes[lp.id[item % num_limbs]]);
+    const gmut<uint64_t> g = (gmut<uint64_t>)(data + ((size_t)item << 15));
+    if (inverse)
+      inv_job(g, 15, ar, lds, t);
+    else
+      fwd_job(g, 15, ar, lds, t);
+  }
+}
+
 }  // namespace
+
+hipError_t launch_ntt15(bool inverse, uint64_t* data, const PrimeDev* primes,
+                        const LimbPrimes& lp, int rows, int num_limbs, hipStream_t s) {
+  if (rows <= 0) return hipSuccess;
+  const size_t shmem = batch_shmem_bytes(15);
+  const void* k = (const void*)ntt15_kernel;
+  set_max_dynamic_shmem(k, shmem);
+  int dev = 0, cus = 0, per_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      cus <= 0)
+    cus = 1;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, kT, shmem) != hipSuccess ||
+      per_cu <= 0)
+    per_cu = 1;
+  long long grid = (long long)cus * per_cu;
+  // A multiple of the limb count keeps each workgroup on one prime.
+  if (grid >= num_limbs) grid -= grid % num_limbs;
+  if (grid > rows) grid = rows;
+  hipLaunchKernelGGL(ntt15_kernel, dim3((unsigned)grid), dim3(kT), shmem, s, data, primes, lp,
+                     num_limbs, rows, inverse ? 1 : 0);
+  return hipGetLastError();
+}
 
 size_t batch_shmem_bytes(int max_log_n) {
   const int logm = max_log_n < 14 ? max_log_n : 14;

@@ -199,6 +199,10 @@ hipError_t launch_modreduce_uniform(const PolyJobDev& uni, size_t stride, int co
                                     hipStream_t s);
 hipError_t launch_batch_limbs(const LimbJobDev* jobs, int num_jobs, int op, int variant,
                               const PrimeDev* primes, int grid, size_t shmem, hipStream_t s);
+// FP64 forward / inverse (x n^-1) transforms of `rows` limbs of 2^15 words,
+// each limb read and written once (shg_batch.hip).
+hipError_t launch_ntt15(bool inverse, uint64_t* data, const PrimeDev* primes,
+                        const LimbPrimes& lp, int rows, int num_limbs, hipStream_t s);
 
 // Oblivious expansion and the inner product against a database (shg_pir.hip).
 hipError_t launch_expand_substitute(const uint64_t* const* cts, int count, int m, uint64_t* S,

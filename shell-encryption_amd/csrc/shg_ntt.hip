@@ -42,6 +42,9 @@
 #ifndef SHG_NTT_WAVE_PF
 #define SHG_NTT_WAVE_PF 1
 #endif
+#ifndef SHG_NTT15_HELD
+#define SHG_NTT15_HELD 1
+#endif
 // The key switch's inverse + digit planes at 2^14 (FP64) on the wave layout.
 #ifndef SHG_DIG_WAVE
 #define SHG_DIG_WAVE 1
@@ -869,6 +872,12 @@ hipError_t launch_ntt(bool inverse, int cfg, uint64_t* data, const PrimeDev* pri
     if (logn == 13) return launch_inv_wave<3>(data, primes, lp, rows, num_limbs, inv_mode, s);
     return launch_inv_wave<4>(data, primes, lp, rows, num_limbs, inv_mode, s);
   }
+  // 2^15 limbs: one kernel with both halves held in VGPRs (instead of a
+  // streaming stage over HBM followed by the 2^14-slice kernel).
+  if (SHG_NTT15_HELD && logn == 15 && max_logm >= 14 && (cfg & 3) == SHG_NTT_F64 &&
+      (cfg & (kNttCfgAlt | kNttCfgPrefetch | kNttCfgSplit | kNttCfgWave)) == 0 &&
+      (!inverse || inv_mode == 1))
+    return launch_ntt15(inverse, data, primes, lp, rows, num_limbs, s);
   if (!inverse && logn == 14 && max_logm >= 13 && (cfg & kNttCfgSplit)) {
     if ((cfg & 3) == SHG_NTT_F64)
       return launch_fwd_split<F64Arith>(data, primes, lp, rows, num_limbs, s);

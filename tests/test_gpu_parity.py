@@ -210,6 +210,29 @@ def test_tensor_product(ctx):
     assert np.array_equal(out.download(), exp)
 
 
+def test_ntt_n32768_more_limbs_than_workgroups(ctx):
+    """2^15 limbs (FP64: one persistent kernel with both halves held in
+    VGPRs) with 320 limbs, more than one per resident workgroup: sampled
+    forward outputs and every inverse word against the oracle."""
+    qs, rns = _rns(15, 4)
+    pids = ctx.primes(qs, 15)
+    count = 80
+    x = rns.uniform(1515, count)
+    d = ctx.array(x)
+    ctx.ntt_fwd(d, count, pids, sg.NTT_F64)
+    got = d.download()
+    for c in (0, 41, count - 1):
+        assert np.array_equal(got[c], rns.ntt_fwd(x[c])), c
+    y = rns.uniform(1516, count)
+    d2 = ctx.array(y)
+    ctx.ntt_inv(d2, count, pids, sg.NTT_F64)
+    got2 = d2.download()
+    for c in (0, 17, count - 1):
+        assert np.array_equal(got2[c], rns.ntt_inv(y[c])), c
+    ctx.ntt_inv(d, count, pids, sg.NTT_F64)
+    assert np.array_equal(d.download(), x)
+
+
 @pytest.mark.parametrize("variant", [sg.NTT_F64, sg.NTT_INT])
 def test_polynomial_product_n4096(ctx, variant):
     """The plain negacyclic product at its own shape: n = 2^12, one prime
