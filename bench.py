@@ -339,8 +339,8 @@ def main():
     if args.ntt_probe:
         return ntt_probe()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: every box here has one MI355X, and the "
-              "multi-GPU path (batch sharding, section on sharding in DESIGN.md) is not "
+        print("bench.py measures one GPU only (--gpus 1): the multi-GPU path, which "
+              "would shard the batch of ciphertexts with a key replica per GPU, is not "
               "built in this repository.", file=sys.stderr)
         return 2
 
