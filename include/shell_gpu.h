@@ -108,6 +108,13 @@ int shg_prime_create(shg_context* ctx, uint64_t q, int log_n, int* prime);
 int shg_prime_info(shg_context* ctx, int prime, uint64_t* params,
                    uint64_t* tables);
 
+/* The ModReduceLsb constants of a chain: host_ql_inv[i] = Montgomery rep of
+ * q_l^-1 mod q_i for i < num_limbs - 1, where q_l = primes[num_limbs - 1] is
+ * the modulus being dropped (RnsContext::MainPrimeModulusInverseResidues,
+ * rns/rns_context.h:145-148, built at rns/rns_context.cc:169-199).  Host
+ * only; no device work. */
+int shg_rns_ql_inv(shg_context* ctx, const int* primes, int num_limbs, uint64_t* host_ql_inv);
+
 /* ---- transforms: RnsPolynomial::ConvertToNttForm / ConvertToCoeffForm
  *      (rns/rns_polynomial.cc:171-207), i.e. ForwardNumberTheoreticTransform
  *      and InverseNumberTheoreticTransform (dft_transformations.h:158-201)
@@ -324,10 +331,22 @@ int shg_batch_limbs(shg_context* ctx, const shg_limb_job* host_jobs,
 
 /* ---- HIP graphs of op chains --------------------------------------------
  * Captures every launch the context makes between begin and end into a
- * graph (stream capture on the context's stream); launch replays it.  Only
- * entry points that launch work without host synchronisation or allocation
- * may be called while capturing: the transforms, pointwise ops, substitute,
- * tensor product, lazy FMA and shg_batch_run. */
+ * graph (stream capture on the context's stream); launch replays it.
+ * May be captured: shg_copy, the transforms, pointwise ops, scalar
+ * multiplies, substitute, tensor product, lazy FMA, gadget decomposition,
+ * shg_batch_run, shg_mod_reduce_lsb, shg_relin_apply and shg_galois_apply.
+ * The last three use work buffers held by the context: make the same call
+ * once outside capture at the largest size the graph will use, so that no
+ * buffer has to grow; a call that would have to grow one during capture
+ * fails with SHG_E_FAILED_PRECONDITION.  While any graph of the context is
+ * alive, a buffer that an eager call grows is retired rather than freed, so
+ * replays never touch freed memory; retired buffers are released when the
+ * last graph is destroyed.  Entry points that synchronise with the host,
+ * allocate, or copy from host memory (shg_sync, shg_alloc, shg_free,
+ * shg_upload, shg_download, shg_prime_create, plan creation and
+ * destruction, shg_batch_limbs, oblivious expansion, the inner product,
+ * profiling queries, the autotuner, shg_graph_destroy) fail with
+ * SHG_E_FAILED_PRECONDITION while the stream is being captured. */
 typedef struct shg_graph shg_graph;
 int shg_graph_begin(shg_context* ctx);
 int shg_graph_end(shg_context* ctx, shg_graph** out);

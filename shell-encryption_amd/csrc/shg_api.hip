@@ -73,6 +73,12 @@ struct shg_context {
   };
   DevBuf exp_s, exp_sub, exp_ptrs, exp_ptrs2, pir_part, ks_planes;
   std::map<std::pair<std::vector<int>, int>, void*> monomials;
+  // Captured graphs hold raw pointers to the work buffers above.  While any
+  // graph instantiated on this context is alive, a buffer that has to grow
+  // is retired here instead of freed, and released once the last graph is
+  // destroyed.
+  int live_graphs = 0;
+  std::vector<void*> retired;
   std::mutex mu;
   // Optional per-kernel timing: HIP events recorded around launches.
   bool profiling = false;
@@ -119,16 +125,46 @@ void fill_modreduce_f64(shg_context* ctx, shg::PolyJobDev& d, uint64_t k) {
   }
 }
 
+// True while the context's stream is being captured into a graph.
+bool capturing(shg_context* ctx) {
+  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+  return hipStreamIsCapturing(ctx->stream, &st) == hipSuccess && st == hipStreamCaptureStatusActive;
+}
+
+// Entry points that synchronise with the host, allocate or copy from host
+// memory cannot be recorded into a graph: they fail before doing anything.
+int refuse_capture(shg_context* ctx, const char* what) {
+  return fail(SHG_E_FAILED_PRECONDITION, "%s cannot be called while the context's stream is "
+              "captured into a graph.", what);
+}
+
+// Releases a work buffer that is being replaced: retired while a graph may
+// still replay launches that use it, else freed once queued work is done.
+int release_work_buffer(shg_context* ctx, void* p) {
+  if (!p) return SHG_OK;
+  if (ctx->live_graphs > 0) {
+    ctx->retired.push_back(p);
+    return SHG_OK;
+  }
+  hipError_t se = hipStreamSynchronize(ctx->stream);  // queued work may still use it
+  if (se == hipSuccess) se = hipFree(p);
+  return se == hipSuccess ? SHG_OK : hip_fail(se, "releasing a previous work buffer");
+}
+
+int grow_refused(size_t bytes) {
+  return fail(SHG_E_FAILED_PRECONDITION,
+              "this call needs a larger work buffer (%zu bytes) than the context holds, which "
+              "cannot be allocated during graph capture; make the same call once outside "
+              "capture first.", bytes);
+}
+
 int ensure_scratch(shg_context* ctx, size_t bytes) {
   if (bytes <= ctx->scratch_size) return SHG_OK;
-  if (ctx->scratch) {
-    // Work queued on the stream may still use the old buffer.
-    hipError_t se = hipStreamSynchronize(ctx->stream);
-    if (se == hipSuccess) se = hipFree(ctx->scratch);
-    if (se != hipSuccess) return hip_fail(se, "releasing the previous scratch buffer");
-    ctx->scratch = nullptr;
-    ctx->scratch_size = 0;
-  }
+  if (capturing(ctx)) return grow_refused(bytes);
+  int rc = release_work_buffer(ctx, ctx->scratch);
+  if (rc) return rc;
+  ctx->scratch = nullptr;
+  ctx->scratch_size = 0;
   hipError_t e = hipMalloc(&ctx->scratch, bytes);
   if (e != hipSuccess) return fail(SHG_E_NO_MEMORY, "scratch allocation of %zu bytes failed", bytes);
   ctx->scratch_size = bytes;
@@ -137,14 +173,11 @@ int ensure_scratch(shg_context* ctx, size_t bytes) {
 
 int ensure_buf(shg_context* ctx, shg_context::DevBuf& b, size_t bytes) {
   if (bytes <= b.size) return SHG_OK;
-  if (b.p) {
-    // Work queued on the stream may still use the old buffer.
-    hipError_t se = hipStreamSynchronize(ctx->stream);
-    if (se == hipSuccess) se = hipFree(b.p);
-    if (se != hipSuccess) return hip_fail(se, "releasing a previous work buffer");
-    b.p = nullptr;
-    b.size = 0;
-  }
+  if (capturing(ctx)) return grow_refused(bytes);
+  int rc = release_work_buffer(ctx, b.p);
+  if (rc) return rc;
+  b.p = nullptr;
+  b.size = 0;
   if (hipMalloc(&b.p, bytes) != hipSuccess)
     return fail(SHG_E_NO_MEMORY, "work buffer allocation of %zu bytes failed", bytes);
   b.size = bytes;
@@ -157,7 +190,7 @@ struct ProfScope {
   shg_context* ctx;
   shg_context::ProfRec rec;
   bool on;
-  ProfScope(shg_context* c, const char* name) : ctx(c), on(c && c->profiling) {
+  ProfScope(shg_context* c, const char* name) : ctx(c), on(c && c->profiling && !capturing(c)) {
     if (!on) return;
     rec.name = name;
     if (hipEventCreate(&rec.start) != hipSuccess || hipEventCreate(&rec.stop) != hipSuccess) {
@@ -296,6 +329,10 @@ int build_gadget(shg_context* ctx, const int* primes, int num_limbs, const int* 
     gd->limb_log_b[i] = (unsigned char)b;
     max_b = b > max_b ? b : max_b;
     const uint64_t q = ctx->primes[primes[i]]->q;
+    // Largest digit of this limb: min(2^b - 1, q - 1).
+    const uint64_t max_digit = ((1ull << b) - 1) < q - 1 ? ((1ull << b) - 1) : q - 1;
+    gd->limb_balance[i] = max_digit > (q >> 1) ? 1 : 0;
+    gd->any_balance |= gd->limb_balance[i];
     min_q = q < min_q ? q : min_q;
   }
   gd->k = k;
@@ -358,6 +395,7 @@ int shg_destroy(shg_context* ctx) {
                   &ctx->ks_planes})
     if (b->p) note(hipFree(b->p));
   for (auto& m : ctx->monomials) note(hipFree(m.second));
+  for (void* p : ctx->retired) note(hipFree(p));
   if (ctx->own_stream) note(hipStreamDestroy(ctx->own_stream));
   delete ctx;
   return first == hipSuccess ? SHG_OK : hip_fail(first, "context teardown");
@@ -373,12 +411,14 @@ void* shg_get_stream(shg_context* ctx) { return ctx ? (void*)ctx->stream : nullp
 
 int shg_sync(shg_context* ctx) {
   if (!ctx) return fail(SHG_E_INVALID_ARGUMENT, "context must not be null.");
+  if (capturing(ctx)) return refuse_capture(ctx, "shg_sync");
   hipError_t e = hipStreamSynchronize(ctx->stream);
   return e == hipSuccess ? SHG_OK : hip_fail(e, "hipStreamSynchronize");
 }
 
 int shg_alloc(shg_context* ctx, size_t bytes, void** dptr) {
   if (!ctx || !dptr) return fail(SHG_E_INVALID_ARGUMENT, "null argument.");
+  if (capturing(ctx)) return refuse_capture(ctx, "shg_alloc");
   hipError_t e = hipMalloc(dptr, bytes ? bytes : 8);
   return e == hipSuccess ? SHG_OK : fail(SHG_E_NO_MEMORY, "hipMalloc(%zu) failed", bytes);
 }
@@ -386,6 +426,7 @@ int shg_alloc(shg_context* ctx, size_t bytes, void** dptr) {
 int shg_free(shg_context* ctx, void* dptr) {
   if (!ctx) return fail(SHG_E_INVALID_ARGUMENT, "context must not be null.");
   if (!dptr) return SHG_OK;
+  if (capturing(ctx)) return refuse_capture(ctx, "shg_free");
   // Work queued on the stream may still use the buffer.
   hipError_t e = hipStreamSynchronize(ctx->stream);
   if (e == hipSuccess) e = hipFree(dptr);
@@ -394,6 +435,7 @@ int shg_free(shg_context* ctx, void* dptr) {
 
 int shg_upload(shg_context* ctx, void* dptr, const void* host, size_t bytes) {
   if (!ctx || (!dptr && bytes) || (!host && bytes)) return fail(SHG_E_INVALID_ARGUMENT, "null argument.");
+  if (capturing(ctx)) return refuse_capture(ctx, "shg_upload");
   hipError_t e = hipMemcpyAsync(dptr, host, bytes, hipMemcpyHostToDevice, ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   return e == hipSuccess ? SHG_OK : hip_fail(e, "upload");
@@ -401,6 +443,7 @@ int shg_upload(shg_context* ctx, void* dptr, const void* host, size_t bytes) {
 
 int shg_download(shg_context* ctx, void* host, const void* dptr, size_t bytes) {
   if (!ctx || (!dptr && bytes) || (!host && bytes)) return fail(SHG_E_INVALID_ARGUMENT, "null argument.");
+  if (capturing(ctx)) return refuse_capture(ctx, "shg_download");
   hipError_t e = hipMemcpyAsync(host, dptr, bytes, hipMemcpyDeviceToHost, ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   return e == hipSuccess ? SHG_OK : hip_fail(e, "download");
@@ -415,6 +458,7 @@ int shg_copy(shg_context* ctx, void* dst, const void* src, size_t bytes) {
 
 int shg_prime_create(shg_context* ctx, uint64_t q, int log_n, int* prime) {
   if (!ctx || !prime) return fail(SHG_E_INVALID_ARGUMENT, "null argument.");
+  if (capturing(ctx)) return refuse_capture(ctx, "shg_prime_create");
   std::lock_guard<std::mutex> lock(ctx->mu);
   auto key = std::make_pair(q, log_n);
   auto it = ctx->prime_index.find(key);
@@ -500,6 +544,28 @@ int shg_prime_create(shg_context* ctx, uint64_t q, int log_n, int* prime) {
   ctx->primes.push_back(std::move(rec));
   ctx->prime_index[key] = id;
   *prime = id;
+  return SHG_OK;
+}
+
+int shg_rns_ql_inv(shg_context* ctx, const int* primes, int num_limbs, uint64_t* host_ql_inv) {
+  LimbPrimes lp{};
+  int log_n = 0;
+  int rc = check_primes(ctx, primes, num_limbs, &log_n, &lp);
+  if (rc) return rc;
+  if (num_limbs < 2)
+    return fail(SHG_E_INVALID_ARGUMENT, "ModReduceLsb cannot apply with only one prime modulus.");
+  if (!host_ql_inv) return fail(SHG_E_INVALID_ARGUMENT, "`ql_inv` must not be null.");
+  // q_l^-1 mod q_i (RnsContext::GenerateCrtConstantsForMainModulus,
+  // rns/rns_context.cc:169-199): ImportInt(q_l) then its inverse.
+  const uint64_t ql = ctx->primes[primes[num_limbs - 1]]->q;
+  for (int i = 0; i < num_limbs - 1; ++i) {
+    const Params* Pi = ctx->primes[primes[i]]->mp.get();
+    if (Pi->modulus == ql)
+      return fail(SHG_E_INVALID_ARGUMENT, "the RNS moduli must be distinct.");
+    auto v = ModInt::ImportInt(ql, Pi);
+    if (!v.ok()) return fail(SHG_E_INVALID_ARGUMENT, "%s", v.status().message().c_str());
+    host_ql_inv[i] = v.value().MultiplicativeInverse(Pi).GetMontgomeryRepresentation();
+  }
   return SHG_OK;
 }
 
@@ -753,8 +819,9 @@ int shg_gadget_decompose(shg_context* ctx, const uint64_t* a, const int* host_lo
 // Shared body of the two key-switch entry points: for each component index
 // in `srcs`, copy it to scratch, bring it to coefficient form, and run the
 // fused key-switch kernel (first call initialises from c0/c1, later ones
-// accumulate).
-static int keyswitch_common(shg_context* ctx, const uint64_t* comps, int comps_per_ct,
+// accumulate).  The caller holds ctx->mu: the work buffers are the
+// context's.
+static int keyswitch_locked(shg_context* ctx, const uint64_t* comps, int comps_per_ct,
                             const int* src_comp, const int* src_is_ntt, const int* key_base,
                             int num_src, bool init_c1, const uint64_t* key_as,
                             const uint64_t* key_bs, const int* log_bs, uint64_t* out_c0,
@@ -762,10 +829,8 @@ static int keyswitch_common(shg_context* ctx, const uint64_t* comps, int comps_p
                             const LimbPrimes& lp, int log_n, const shg::GadgetDesc& gd) {
   const size_t n = (size_t)1 << log_n;
   const size_t poly_words = (size_t)num_limbs * n;
-  std::lock_guard<std::mutex> lock(ctx->mu);
-  int rc = ensure_scratch(ctx, (size_t)count * poly_words * 8);
-  if (rc) return rc;
-  uint64_t* src = (uint64_t*)ctx->scratch;
+  int rc = SHG_OK;
+  uint64_t* src = nullptr;  // coefficient-form copy of a source component (scratch)
   hipStream_t s = ctx->stream;
   const int v = resolve_variant(ctx, SHG_NTT_AUTO, primes, num_limbs, log_n,
                                 (long long)count * num_limbs);
@@ -807,6 +872,9 @@ static int keyswitch_common(shg_context* ctx, const uint64_t* comps, int comps_p
       e = shg::launch_keyswitch(ka, gd, v & 3, ctx->d_primes, lp, count, num_limbs, log_n, s);
       continue;
     }
+    rc = ensure_scratch(ctx, (size_t)count * poly_words * 8);
+    if (rc) return rc;
+    src = (uint64_t*)ctx->scratch;
     e = hipMemcpy2DAsync(src, poly_words * 8, comps + (size_t)src_comp[i] * poly_words,
                          (size_t)comps_per_ct * poly_words * 8, poly_words * 8, count,
                          hipMemcpyDeviceToDevice, s);
@@ -844,16 +912,20 @@ static int keyswitch_common(shg_context* ctx, const uint64_t* comps, int comps_p
       const size_t need = (size_t)count * poly_words * 8 * (1 + (size_t)gd.k);
       if (need > ctx->scratch_size) {
         // Grow while keeping the coefficient-form source.
+        if (capturing(ctx)) return grow_refused(need);
         void* bigger = nullptr;
         if (hipMalloc(&bigger, need) != hipSuccess)
           return fail(SHG_E_NO_MEMORY, "scratch allocation of %zu bytes failed", need);
         hipError_t ge =
             hipMemcpyAsync(bigger, src, (size_t)count * poly_words * 8, hipMemcpyDeviceToDevice, s);
-        if (ge == hipSuccess) ge = hipStreamSynchronize(s);
-        if (ge == hipSuccess) ge = hipFree(ctx->scratch);
         if (ge != hipSuccess) {
           (void)hipFree(bigger);  // already failing; report the copy
           return hip_fail(ge, "scratch growth");
+        }
+        rc = release_work_buffer(ctx, ctx->scratch);
+        if (rc) {
+          (void)hipFree(bigger);  // already failing; report the release
+          return rc;
         }
         ctx->scratch = bigger;
         ctx->scratch_size = need;
@@ -874,7 +946,7 @@ static int keyswitch_common(shg_context* ctx, const uint64_t* comps, int comps_p
                                  (size_t)comps_per_ct * poly_words * 8, poly_words * 8, count,
                                  hipMemcpyDeviceToDevice, s);
           else
-            e = hipMemsetAsync(out_c1, 0, (size_t)count * poly_words * 8, s);
+            e = shg::launch_zero(out_c1, (size_t)count * poly_words, s);
         }
       }
       for (int j = 0; j < gd.k && e == hipSuccess; ++j) {
@@ -891,6 +963,18 @@ static int keyswitch_common(shg_context* ctx, const uint64_t* comps, int comps_p
   }
   (void)log_bs;
   return e == hipSuccess ? SHG_OK : hip_fail(e, "key switch launch");
+}
+
+static int keyswitch_common(shg_context* ctx, const uint64_t* comps, int comps_per_ct,
+                            const int* src_comp, const int* src_is_ntt, const int* key_base,
+                            int num_src, bool init_c1, const uint64_t* key_as,
+                            const uint64_t* key_bs, const int* log_bs, uint64_t* out_c0,
+                            uint64_t* out_c1, int count, int num_limbs, const int* primes,
+                            const LimbPrimes& lp, int log_n, const shg::GadgetDesc& gd) {
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  return keyswitch_locked(ctx, comps, comps_per_ct, src_comp, src_is_ntt, key_base, num_src,
+                          init_c1, key_as, key_bs, log_bs, out_c0, out_c1, count, num_limbs, primes,
+                          lp, log_n, gd);
 }
 
 int shg_relin_apply(shg_context* ctx, const uint64_t* comps, int degree, int ntt_mask,
@@ -1002,6 +1086,7 @@ int shg_oblivious_expand_level(shg_context* ctx, uint64_t* const* host_cts, int 
     return fail(SHG_E_INVALID_ARGUMENT, "queries must be a non-negative count with pointers.");
   if (level < 0 || level >= log_n)
     return fail(SHG_E_INVALID_ARGUMENT, "level must be in [0, log_n).");
+  if (capturing(ctx)) return refuse_capture(ctx, "shg_oblivious_expand_level");
   if (log_n < 4 || log_n > shg::kMaxLogM)
     return fail(SHG_E_INVALID_ARGUMENT, "expansion needs 2^4 <= n <= 2^14.");
   shg::GadgetDesc gd{};
@@ -1020,8 +1105,10 @@ int shg_oblivious_expand_level(shg_context* ctx, uint64_t* const* host_cts, int 
   const uint64_t* mono = nullptr;
   uint64_t *S = nullptr, *sub0 = nullptr, *sub1 = nullptr;
   uint64_t** d_cts = nullptr;
+  // The whole level holds the context's lock: its three launches share the
+  // context's work buffers (S, sub0/sub1 and the pointer table).
+  std::lock_guard<std::mutex> lock(ctx->mu);
   {
-    std::lock_guard<std::mutex> lock(ctx->mu);
     rc = expand_monomial(ctx, primes, num_limbs, log_n, level, lp, &mono);
     if (!rc) rc = ensure_buf(ctx, ctx->exp_s, cts * 2 * poly * 8);
     if (!rc) rc = ensure_buf(ctx, ctx->exp_sub, cts * 2 * poly * 8);
@@ -1038,7 +1125,7 @@ int shg_oblivious_expand_level(shg_context* ctx, uint64_t* const* host_cts, int 
     if (e != hipSuccess) return hip_fail(e, "expansion substitute");
   }
   const int src_comp = 1, src_ntt = 1, base = 0;
-  rc = keyswitch_common(ctx, S, 2, &src_comp, &src_ntt, &base, 1, false, key_as, key_bs,
+  rc = keyswitch_locked(ctx, S, 2, &src_comp, &src_ntt, &base, 1, false, key_as, key_bs,
                         host_log_bs, sub0, sub1, (int)cts, num_limbs, primes, lp, log_n, gd);
   if (rc) return rc;
   hipError_t e = shg::launch_expand_update(d_cts, count, m, sub0, sub1, mono, ctx->d_primes, lp,
@@ -1106,6 +1193,7 @@ int shg_pir_inner_product(shg_context* ctx, const uint64_t* const* host_expanded
     return fail(SHG_E_INVALID_ARGUMENT, "queries must be a non-negative count with pointers.");
   if (levels < 0 || levels > 20 || num_columns <= 0)
     return fail(SHG_E_INVALID_ARGUMENT, "levels must be in [0, 20] and num_columns positive.");
+  if (capturing(ctx)) return refuse_capture(ctx, "shg_pir_inner_product");
   if (count == 0) return SHG_OK;
   if (!aligned_ptr(db_exported)) return fail(SHG_E_INVALID_ARGUMENT, "database must not be null.");
   for (int g = 0; g < count; ++g)
@@ -1158,6 +1246,7 @@ int shg_lazy_fma(shg_context* ctx, const uint64_t* as, size_t as_stride, const u
 
 int shg_profile_enable(shg_context* ctx, int enable) {
   if (!ctx) return fail(SHG_E_INVALID_ARGUMENT, "context must not be null.");
+  if (capturing(ctx)) return refuse_capture(ctx, "shg_profile_enable");
   std::lock_guard<std::mutex> lock(ctx->mu);
   prof_collect(ctx);
   ctx->profiling = enable != 0;
@@ -1166,6 +1255,7 @@ int shg_profile_enable(shg_context* ctx, int enable) {
 
 int shg_profile_reset(shg_context* ctx) {
   if (!ctx) return fail(SHG_E_INVALID_ARGUMENT, "context must not be null.");
+  if (capturing(ctx)) return refuse_capture(ctx, "shg_profile_reset");
   std::lock_guard<std::mutex> lock(ctx->mu);
   prof_collect(ctx);
   ctx->prof_totals.clear();
@@ -1175,6 +1265,7 @@ int shg_profile_reset(shg_context* ctx) {
 int shg_profile_query(shg_context* ctx, const char* kernel, double* total_ms, long long* launches) {
   if (!ctx || !kernel || !total_ms || !launches)
     return fail(SHG_E_INVALID_ARGUMENT, "null argument.");
+  if (capturing(ctx)) return refuse_capture(ctx, "shg_profile_query");
   std::lock_guard<std::mutex> lock(ctx->mu);
   prof_collect(ctx);
   auto it = ctx->prof_totals.find(kernel);
@@ -1210,6 +1301,7 @@ int shg_batch_plan_create(shg_context* ctx, const shg_limb_job* host_jobs, int n
     return fail(SHG_E_INVALID_ARGUMENT, "jobs must be a non-negative count with a job array.");
   if (variant < SHG_NTT_AUTO || variant > SHG_NTT_F64)
     return fail(SHG_E_INVALID_ARGUMENT, "unknown transform variant %d.", variant);
+  if (capturing(ctx)) return refuse_capture(ctx, "shg_batch_plan_create");
   std::vector<shg::LimbJobDev> jobs(num_jobs);
   bool f64_ok = true, has_other = true;
   int max_log_n = shg::kBatchMinLogN;
@@ -1285,6 +1377,7 @@ int shg_batch_mod_reduce_plan_create(shg_context* ctx, const shg_poly_job* host_
   if (t == 0) return fail(SHG_E_INVALID_ARGUMENT, "plaintext modulus must be positive.");
   if (variant < SHG_NTT_AUTO || variant > SHG_NTT_F64)
     return fail(SHG_E_INVALID_ARGUMENT, "unknown transform variant %d.", variant);
+  if (capturing(ctx)) return refuse_capture(ctx, "shg_batch_mod_reduce_plan_create");
   std::vector<shg::PolyJobDev> jobs(num_jobs);
   bool f64_ok = true;
   int max_log_n = shg::kBatchMinLogN;
@@ -1383,6 +1476,7 @@ int shg_batch_mod_reduce_plan_create(shg_context* ctx, const shg_poly_job* host_
 
 int shg_batch_plan_destroy(shg_context* ctx, shg_batch_plan* plan) {
   if (!ctx || !plan) return fail(SHG_E_INVALID_ARGUMENT, "null argument.");
+  if (capturing(ctx)) return refuse_capture(ctx, "shg_batch_plan_destroy");
   if (plan->d_jobs || plan->scratch) (void)hipStreamSynchronize(ctx->stream);
   if (plan->d_jobs) (void)hipFree(plan->d_jobs);
   if (plan->scratch) (void)hipFree(plan->scratch);
@@ -1413,6 +1507,7 @@ int shg_batch_run(shg_context* ctx, const shg_batch_plan* plan, int op) {
 }
 
 int shg_batch_limbs(shg_context* ctx, const shg_limb_job* host_jobs, int num_jobs, int op) {
+  if (ctx && capturing(ctx)) return refuse_capture(ctx, "shg_batch_limbs");
   shg_batch_plan* plan = nullptr;
   int rc = shg_batch_plan_create(ctx, host_jobs, num_jobs, SHG_NTT_AUTO, &plan);
   if (rc) return rc;
@@ -1429,6 +1524,7 @@ int shg_batch_limbs(shg_context* ctx, const shg_limb_job* host_jobs, int num_job
 
 int shg_graph_begin(shg_context* ctx) {
   if (!ctx) return fail(SHG_E_INVALID_ARGUMENT, "context must not be null.");
+  if (capturing(ctx)) return fail(SHG_E_FAILED_PRECONDITION, "a capture is already in progress.");
   hipError_t e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
   return e == hipSuccess ? SHG_OK : hip_fail(e, "begin capture");
 }
@@ -1443,6 +1539,10 @@ int shg_graph_end(shg_context* ctx, shg_graph** out) {
     (void)hipGraphDestroy(g->graph);
     return hip_fail(e, "graph instantiate");
   }
+  {
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    ++ctx->live_graphs;
+  }
   *out = g.release();
   return SHG_OK;
 }
@@ -1455,10 +1555,17 @@ int shg_graph_launch(shg_context* ctx, const shg_graph* graph) {
 
 int shg_graph_destroy(shg_context* ctx, shg_graph* graph) {
   if (!ctx || !graph) return fail(SHG_E_INVALID_ARGUMENT, "null argument.");
+  if (capturing(ctx)) return refuse_capture(ctx, "shg_graph_destroy");
   (void)hipStreamSynchronize(ctx->stream);
   if (graph->exec) (void)hipGraphExecDestroy(graph->exec);
   if (graph->graph) (void)hipGraphDestroy(graph->graph);
   delete graph;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  if (ctx->live_graphs > 0 && --ctx->live_graphs == 0) {
+    // No graph can replay a launch that uses a retired buffer any more.
+    for (void* p : ctx->retired) (void)hipFree(p);
+    ctx->retired.clear();
+  }
   return SHG_OK;
 }
 
@@ -1466,6 +1573,7 @@ int shg_autotune_ntt(shg_context* ctx, int log_n, int limbs, int* chosen) {
   if (!ctx || !chosen) return fail(SHG_E_INVALID_ARGUMENT, "null argument.");
   if (log_n < 1 || log_n > 15) return fail(SHG_E_INVALID_ARGUMENT, "log_n must be in [1, 15].");
   if (limbs <= 0) return fail(SHG_E_INVALID_ARGUMENT, "limbs must be positive.");
+  if (capturing(ctx)) return refuse_capture(ctx, "shg_autotune_ntt");
   // A modulus of the shape the configurations serve: the largest prime
   // below 2^50 with q = 1 mod 2n.
   const uint64_t two_n = 2ull << log_n;

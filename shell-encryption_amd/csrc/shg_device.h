@@ -133,6 +133,17 @@ __device__ __forceinline__ uint64_t mod_sub(uint64_t a, uint64_t b, uint64_t q) 
   return csub(csub(a + (q - b), q), q);
 }
 
+// ImportInt mod P.q of the integer a value d in [0, qs) stands for under the
+// balanced rule of ConvertModulusBalanced (modulus_conversion.h:49-81):
+// d > qs/2 is d - qs.  For P.q == qs this is ImportInt(d) itself.
+__device__ __forceinline__ uint64_t import_balanced(uint64_t d, uint64_t qs, const PrimeDev& P) {
+  if (d > (qs >> 1)) {
+    const uint64_t m = mont_import(qs - d, P);
+    return m ? P.q - m : 0;
+  }
+  return mont_import(d, P);
+}
+
 // ConvertModulusBalanced (modulus_conversion.h:30-83) from q_l to each q_i.
 __device__ __forceinline__ uint64_t convert_balanced(uint64_t rep_q, const PrimeDev& Pq,
                                                      const PrimeDev& Pp) {

@@ -31,6 +31,8 @@ hipError_t launch_ntt(bool inverse, int cfg, uint64_t* data, const PrimeDev* pri
                       hipStream_t s, int inv_mode = 1);
 hipError_t launch_scale_ninv(uint64_t* data, const PrimeDev* primes, const LimbPrimes& lp,
                              int rows, int num_limbs, int logn, int mode, hipStream_t s);
+// x[0 .. words) <- 0 with a kernel (never a memset node in a captured graph).
+hipError_t launch_zero(uint64_t* x, size_t words, hipStream_t s);
 // x <- ExportInt(x) (montgomery.h:156-164) over count polynomials.
 hipError_t launch_export(uint64_t* x, const PrimeDev* primes, const LimbPrimes& lp, int count,
                          int num_limbs, int logn, hipStream_t s);
@@ -89,6 +91,12 @@ struct GadgetDesc {
   // 2^max_log_b * 64 <= every modulus: digits enter the transform far below
   // q, which lets the FP64 key switch defer its first reduction.
   int small_digits;
+  // Per source limb: a digit can exceed q_i / 2 (base 2^b with
+  // 2^b - 1 > q_i / 2), so its lift to the other limbs must take the
+  // balanced rule of ConvertModulusBalanced (modulus_conversion.h:30-83):
+  // d > q_i / 2 stands for d - q_i.
+  unsigned char limb_balance[SHG_MAX_LIMBS];
+  int any_balance;
 };
 hipError_t launch_gadget_decompose(const uint64_t* a, uint64_t* digits, const GadgetDesc& gd,
                                    const PrimeDev* primes, const LimbPrimes& lp, int count,

@@ -282,7 +282,10 @@ __global__ void __launch_bounds__((1 << LOGM) >> LOGE)
             }
           }
         };
-        if (digits32) {
+        if (gd.limb_balance[i]) {
+          const uint64_t qs = primes[lp.id[i]].q;
+          load_digits([&](uint64_t s64) { return ar.digit_in_bal(s64 & mask, qs, P); });
+        } else if (digits32) {
           const uint32_t m32 = (uint32_t)mask;
           load_digits([&](uint64_t s64) { return ar.digit_in32((uint32_t)s64 & m32, P); });
         } else {

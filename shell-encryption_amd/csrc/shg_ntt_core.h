@@ -145,6 +145,10 @@ struct IntArith {
   __device__ __forceinline__ V digit_in32(uint32_t d, const PrimeDev& P) const {
     return mont_import(d, P);
   }
+  // A digit of the limb of modulus qs that may exceed qs / 2: balanced lift.
+  __device__ __forceinline__ V digit_in_bal(uint64_t d, uint64_t qs, const PrimeDev& P) const {
+    return import_balanced(d, qs, P);
+  }
   __device__ __forceinline__ Acc acc_in(uint64_t rep) const { return rep; }
   __device__ __forceinline__ Prep prep(V y) const { return fwd_out(y); }
   __device__ __forceinline__ void mac(Acc& a, Prep d, uint64_t key, const PrimeDev& P) const {
@@ -253,6 +257,11 @@ struct F64Arith {
     return u52_to_f64(d);
   }
   __device__ __forceinline__ V digit_in32(uint32_t d, const PrimeDev&) const { return (double)d; }
+  // Balanced lift of a digit of the limb of modulus qs (< 2^50): d > qs / 2
+  // enters as the exact negative d - qs, congruent to d modulo qs itself.
+  __device__ __forceinline__ V digit_in_bal(uint64_t d, uint64_t qs, const PrimeDev&) const {
+    return u52_to_f64(d) - (d > (qs >> 1) ? u52_to_f64(qs) : 0.0);
+  }
   __device__ __forceinline__ Acc acc_in(uint64_t rep) const { return u52_to_f64(rep); }
   __device__ __forceinline__ Prep prep(V y) const { return f64_reduce(y, q, qinv); }
   __device__ __forceinline__ void mac(Acc& a, Prep d, uint64_t key, const PrimeDev&) const {

@@ -37,6 +37,20 @@ hipError_t launch_scale_ninv(uint64_t* data, const PrimeDev* primes, const LimbP
   return hipGetLastError();
 }
 
+// Zero fill by vector stores: a kernel node rather than a memset node when the
+// launch is captured into a graph.
+__global__ void zero_kernel(uint64_t* __restrict__ x, long long words) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < words) x[i] = 0;
+}
+
+hipError_t launch_zero(uint64_t* x, size_t words, hipStream_t s) {
+  if (words == 0) return hipSuccess;
+  hipLaunchKernelGGL(zero_kernel, dim3(grid_for((long long)words, 256)), dim3(256), 0, s, x,
+                     (long long)words);
+  return hipGetLastError();
+}
+
 __global__ void export_kernel(uint64_t* __restrict__ x, const PrimeDev* __restrict__ primes,
                               LimbPrimes lp, int num_limbs, int logn, long long total) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -287,8 +301,9 @@ hipError_t launch_tensor_product(const uint64_t* a, int len_a, const uint64_t* b
 
 // ---------------------------------------------------------------------------
 // Gadget decomposition (rns/rns_gadget.cc:108-153 with gadget.h:52-79 and the
-// balanced lift of modulus_conversion.h:30-83).  A digit is < 2^log_b, far
-// below q/2, so its balanced lift to q_l is ImportInt(digit) mod q_l.
+// balanced lift of modulus_conversion.h:30-83).  A digit of limb i is below
+// min(2^log_b, q_i); one above q_i / 2 (possible when log_b >= log q_i - 1)
+// stands for digit - q_i on the other limbs.
 // ---------------------------------------------------------------------------
 __global__ void gadget_decompose_kernel(const uint64_t* __restrict__ a,
                                         uint64_t* __restrict__ digits, GadgetDesc gd,
@@ -307,7 +322,7 @@ __global__ void gadget_decompose_kernel(const uint64_t* __restrict__ a,
   const PrimeDev& Ps = primes[lp.id[src]];
   const uint64_t v = mont_export(a[((ct * num_limbs + src) << logn) + off], Ps);
   const uint64_t digit = (v >> gd.shift[j]) & ((1ull << gd.log_b[j]) - 1);
-  digits[i] = mont_import(digit, primes[lp.id[l]]);
+  digits[i] = import_balanced(digit, Ps.q, primes[lp.id[l]]);
 }
 
 hipError_t launch_gadget_decompose(const uint64_t* a, uint64_t* digits, const GadgetDesc& gd,

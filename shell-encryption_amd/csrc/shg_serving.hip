@@ -26,6 +26,7 @@
 #include <memory>
 #include <mutex>
 #include <random>
+#include <string>
 #include <thread>
 #include <vector>
 
@@ -44,6 +45,7 @@ struct Request {
   Clock::time_point submitted, finished;
   bool done = false;
   int rc = SHG_OK;
+  std::string error;  // the worker's shg_last_error() message when rc != SHG_OK
 };
 
 struct Slot {
@@ -164,6 +166,7 @@ void shg_pir_server::run() {
           rc = shg_upload(ctx, free_slot->expanded, req->query.data(), ct_words * 8);
         if (rc) {
           req->rc = rc;
+          req->error = shg_last_error();
           req->done = true;
           req->finished = Clock::now();
           cv_done.notify_all();
@@ -194,8 +197,10 @@ void shg_pir_server::run() {
         req->answer.resize((size_t)C * ct_words);
         r = shg_download(ctx, req->answer.data(), sl->answer, req->answer.size() * 8);
       }
+      const std::string msg = r != SHG_OK ? std::string(shg_last_error()) : std::string();
       std::lock_guard<std::mutex> lock(mu);
       req->rc = r;
+      req->error = msg;
       req->done = true;
       req->finished = Clock::now();
       sl->req.reset();
@@ -207,6 +212,7 @@ void shg_pir_server::run() {
   std::lock_guard<std::mutex> lock(mu);
   for (Slot* sl : active) {
     sl->req->rc = SHG_E_FAILED_PRECONDITION;
+    sl->req->error = "the server stopped before the request finished.";
     sl->req->done = true;
     sl->req.reset();
   }
@@ -304,7 +310,9 @@ int shg_pir_server_wait(shg_pir_server* s, long long request, uint64_t* host_ans
     s->cv_done.wait(lock, [&] { return req->done; });
     s->requests.erase(it);
   }
-  if (req->rc) return req->rc;
+  // The worker thread recorded its error message with the request; report it
+  // on the waiting thread, whose shg_last_error() the caller reads.
+  if (req->rc) return shg::set_error(req->rc, req->error.c_str());
   if (host_answer) std::memcpy(host_answer, req->answer.data(), req->answer.size() * 8);
   if (latency_ms)
     *latency_ms = std::chrono::duration<double, std::milli>(req->finished - req->submitted).count();
@@ -366,6 +374,7 @@ int shg_pir_server_destroy(shg_pir_server* s) {
     s->stop = true;
     for (auto& r : s->pending) {
       r->rc = SHG_E_FAILED_PRECONDITION;
+      r->error = "the server stopped before the request started.";
       r->done = true;
     }
     s->pending.clear();

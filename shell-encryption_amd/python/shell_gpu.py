@@ -66,6 +66,7 @@ def lib():
         "shg_download": [VP, VP, VP, sz],
         "shg_prime_create": [VP, ctypes.c_uint64, i, INTP],
         "shg_prime_info": [VP, i, U64P, U64P],
+        "shg_rns_ql_inv": [VP, INTP, i, U64P],
         "shg_ntt_fwd": [VP, VP, i, i, INTP, i],
         "shg_ntt_inv": [VP, VP, i, i, INTP, i],
         "shg_pointwise": [VP, i, VP, VP, sz, VP, sz, i, i, INTP],
@@ -238,6 +239,12 @@ class Context:
         _check(lib().shg_prime_info(self.h, int(pid), params.ctypes.data_as(U64P),
                                     tables.ctypes.data_as(U64P)))
         return params, tables
+
+    def ql_inv(self, primes):
+        """Montgomery reps of q_l^-1 mod q_i (i < L-1) for dropping primes[-1]."""
+        out = np.zeros(max(len(primes) - 1, 1), dtype=np.uint64)
+        _check(lib().shg_rns_ql_inv(self.h, _ints(primes), len(primes), out.ctypes.data_as(U64P)))
+        return out[:len(primes) - 1]
 
     # ---- ops ----------------------------------------------------------------
     def ntt_fwd(self, buf, count, primes, variant=NTT_AUTO, offset=0):
