@@ -179,6 +179,24 @@ int shg_relin_apply(shg_context* ctx, const uint64_t* comps, int degree,
                     const int* host_log_bs, uint64_t* out_c0,
                     uint64_t* out_c1, int count, int num_limbs,
                     const int* primes);
+/* shg_relin_apply without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, hosts that forbid
+ * hipMalloc).  Same arguments, checks and results as shg_relin_apply, for
+ * ring degrees 2^4..2^14.  Its work space is a fixed 64 MiB area per GPU,
+ * held from the library's load until the process ends; ciphertexts are
+ * switched in chunks that fit it, and a call fails with
+ * SHG_E_INVALID_ARGUMENT if one ciphertext alone needs more (gadget digit
+ * planes of k*n*2 bytes, or L*n*8 bytes for bases above 2^16).  Every call
+ * on one GPU, from any context or thread, shares that area: calls that
+ * overlap on one GPU (different streams, or threads without ordering) race
+ * and must be serialised by the caller. */
+int shg_relin_apply_noalloc(shg_context* ctx, const uint64_t* comps, int degree,
+                            int ntt_mask, const uint64_t* key_as,
+                            const uint64_t* key_bs, int num_key_polys,
+                            const int* host_log_bs, uint64_t* out_c0,
+                            uint64_t* out_c1, int count, int num_limbs,
+                            const int* primes);
 /* RnsGaloisKey::ApplyToRlweCiphertext (rns/rns_galois_key.cc:238-287) for
  * `count` degree-1 ciphertexts (c0 then c1 per ciphertext). */
 int shg_galois_apply(shg_context* ctx, const uint64_t* comps, int c1_is_ntt,

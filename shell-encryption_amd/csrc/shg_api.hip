@@ -25,6 +25,13 @@ using shg::PrimeDev;
 using ModInt = rlwe::MontgomeryInt<uint64_t>;
 using Params = rlwe::MontgomeryIntParams<uint64_t>;
 
+// Work area of shg_relin_apply_noalloc, one per GPU, held from the library's
+// load on that GPU until the process ends.  The name is a host-side stand-in
+// for the symbol, not a device address: host code uses only the address
+// that noalloc_work_address() obtains with hipGetSymbolAddress.
+static constexpr size_t kNoAllocWorkBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocWorkBytes];
+
 namespace {
 
 thread_local std::string g_last_error;
@@ -79,6 +86,10 @@ struct shg_context {
   // destroyed.
   int live_graphs = 0;
   std::vector<void*> retired;
+  // Device address of this GPU's no-allocation work area (or null, with
+  // the reason in noalloc_error).
+  void* noalloc_work = nullptr;
+  std::string noalloc_error;
   std::mutex mu;
   // Optional per-kernel timing: HIP events recorded around launches.
   bool profiling = false;
@@ -91,6 +102,29 @@ struct shg_context {
 };
 
 namespace {
+
+// The address of `scratch` on `device` (which must be current), looked up
+// once per device; each GPU holds its own copy of the array.
+void* noalloc_work_address(int device, std::string* error) {
+  static std::mutex mu;
+  static std::map<int, void*> addr;
+  static std::map<int, std::string> err;
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = addr.find(device);
+  if (it != addr.end()) {
+    *error = err[device];
+    return it->second;
+  }
+  void* p = nullptr;
+  hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) {
+    p = nullptr;
+    err[device] = std::string("hipGetSymbolAddress: ") + hipGetErrorString(e);
+  }
+  addr[device] = p;
+  *error = err[device];
+  return p;
+}
 
 uint64_t mulmod_host(uint64_t a, uint64_t b, uint64_t q) {
   return (uint64_t)((unsigned __int128)a * b % q);
@@ -367,6 +401,7 @@ int shg_create(int device, shg_context** out) {
     return hip_fail(e, "hipStreamCreate");
   }
   ctx->stream = ctx->own_stream;
+  ctx->noalloc_work = noalloc_work_address(device, &ctx->noalloc_error);
   ctx->d_capacity = 256;
   e = hipMalloc(&ctx->d_primes, sizeof(PrimeDev) * ctx->d_capacity);
   if (e != hipSuccess) {
@@ -816,11 +851,72 @@ int shg_gadget_decompose(shg_context* ctx, const uint64_t* a, const int* host_lo
   return e == hipSuccess ? SHG_OK : hip_fail(e, "decompose launch");
 }
 
+// One source component of a fused key switch (4 <= log n <= 14) for `count`
+// ciphertexts: its gadget digits as 16-bit planes in `planes` (NTT-form
+// source, when `planes` is given), or an exported coefficient-form copy in
+// `src`, then the key-switch kernel, which initialises out_c0/out_c1 from
+// c0/c1 or accumulates into them.
+static hipError_t keyswitch_fused_one(shg_context* ctx, int v, uint16_t* planes, uint64_t* src,
+                                      const uint64_t* comps, int comps_per_ct, int comp,
+                                      bool comp_is_ntt, int key_base, bool init_c1,
+                                      bool accumulate, const uint64_t* key_as,
+                                      const uint64_t* key_bs, uint64_t* out_c0, uint64_t* out_c1,
+                                      int count, int num_limbs, const LimbPrimes& lp, int log_n,
+                                      const shg::GadgetDesc& gd) {
+  const size_t poly_words = (size_t)num_limbs << log_n;
+  const size_t ct_words = (size_t)comps_per_ct * poly_words;
+  hipStream_t s = ctx->stream;
+  hipError_t e = hipSuccess;
+  shg::KeySwitchArgs ka{};
+  if (planes) {
+    ProfScope prof(ctx, "keyswitch_intt");
+    e = shg::launch_ntt_inv_digits(v, comps + (size_t)comp * poly_words, ct_words, planes, gd,
+                                   ctx->d_primes, lp, count, num_limbs, log_n, s);
+    ka.planes = planes;
+  } else {
+    e = hipMemcpy2DAsync(src, poly_words * 8, comps + (size_t)comp * poly_words, ct_words * 8,
+                         poly_words * 8, count, hipMemcpyDeviceToDevice, s);
+    // The kernel reads exported residues: ExportInt folded into the inverse
+    // transform's scaling, or applied in place to coefficient-form input.
+    if (e == hipSuccess && comp_is_ntt) {
+      ProfScope prof(ctx, "keyswitch_intt");
+      e = shg::launch_ntt(true, v, src, ctx->d_primes, lp, count * num_limbs, num_limbs, log_n,
+                          shg::kMaxLogM, s, 2);
+    } else if (e == hipSuccess) {
+      e = shg::launch_export(src, ctx->d_primes, lp, count, num_limbs, log_n, s);
+    }
+    ka.src = src;
+    ka.src_stride = poly_words;
+  }
+  if (e != hipSuccess) return e;
+  ka.c0 = comps;
+  ka.c0_stride = ct_words;
+  ka.c1 = init_c1 ? comps + poly_words : nullptr;
+  ka.c1_stride = ct_words;
+  ka.key_a = key_as;
+  ka.key_b = key_bs;
+  ka.key_base = key_base;
+  ka.out0 = out_c0;
+  ka.out1 = out_c1;
+  ka.out_stride = poly_words;
+  ka.accumulate = accumulate;
+  ProfScope prof(ctx, "keyswitch");
+  return shg::launch_keyswitch(ka, gd, v & 3, ctx->d_primes, lp, count, num_limbs, log_n, s);
+}
+
+// Digit planes: the inverse transform writes the gadget digits as 16-bit
+// planes read straight from the components (FP64 key switch, bases up to
+// 2^16, NTT-form source).
+static bool keyswitch_planes_ok(int v, int log_n, const shg::GadgetDesc& gd) {
+  return log_n >= 4 && log_n <= shg::kMaxLogM && (v & 3) == SHG_NTT_F64 && gd.small_digits &&
+         gd.max_log_b <= 16;
+}
+
 // Shared body of the two key-switch entry points: for each component index
-// in `srcs`, copy it to scratch, bring it to coefficient form, and run the
-// fused key-switch kernel (first call initialises from c0/c1, later ones
-// accumulate).  The caller holds ctx->mu: the work buffers are the
-// context's.
+// in `srcs`, bring it to coefficient form (as digit planes or an exported
+// copy) and run the fused key-switch kernel (first call initialises from
+// c0/c1, later ones accumulate).  The caller holds ctx->mu: the work buffers
+// are the context's.
 static int keyswitch_locked(shg_context* ctx, const uint64_t* comps, int comps_per_ct,
                             const int* src_comp, const int* src_is_ntt, const int* key_base,
                             int num_src, bool init_c1, const uint64_t* key_as,
@@ -836,10 +932,7 @@ static int keyswitch_locked(shg_context* ctx, const uint64_t* comps, int comps_p
                                 (long long)count * num_limbs);
   hipError_t e = hipSuccess;
   const bool fused = log_n <= shg::kMaxLogM && log_n >= 4;
-  // Digit planes: the inverse transform writes the gadget digits as 16-bit
-  // planes read straight from the components (FP64 key switch, bases up to
-  // 2^16, NTT-form source).
-  const bool planes_ok = fused && (v & 3) == SHG_NTT_F64 && gd.small_digits && gd.max_log_b <= 16;
+  const bool planes_ok = keyswitch_planes_ok(v, log_n, gd);
   uint16_t* planes = nullptr;
   if (planes_ok) {
     rc = ensure_buf(ctx, ctx->ks_planes, (size_t)count * gd.k * n * 2);
@@ -847,66 +940,30 @@ static int keyswitch_locked(shg_context* ctx, const uint64_t* comps, int comps_p
     planes = (uint16_t*)ctx->ks_planes.p;
   }
   for (int i = 0; i < num_src && e == hipSuccess; ++i) {
-    if (planes_ok && src_is_ntt[i]) {
-      {
-        ProfScope prof(ctx, "keyswitch_intt");
-        e = shg::launch_ntt_inv_digits(v, comps + (size_t)src_comp[i] * poly_words,
-                                       (size_t)comps_per_ct * poly_words, planes, gd,
-                                       ctx->d_primes, lp, count, num_limbs, log_n, s);
-      }
-      if (e != hipSuccess) break;
-      shg::KeySwitchArgs ka{};
-      ka.planes = planes;
-      ka.c0 = comps;
-      ka.c0_stride = (size_t)comps_per_ct * poly_words;
-      ka.c1 = init_c1 ? comps + poly_words : nullptr;
-      ka.c1_stride = (size_t)comps_per_ct * poly_words;
-      ka.key_a = key_as;
-      ka.key_b = key_bs;
-      ka.key_base = key_base[i];
-      ka.out0 = out_c0;
-      ka.out1 = out_c1;
-      ka.out_stride = poly_words;
-      ka.accumulate = i > 0;
-      ProfScope prof(ctx, "keyswitch");
-      e = shg::launch_keyswitch(ka, gd, v & 3, ctx->d_primes, lp, count, num_limbs, log_n, s);
+    const bool use_planes = planes_ok && src_is_ntt[i];
+    if (!use_planes) {
+      rc = ensure_scratch(ctx, (size_t)count * poly_words * 8);
+      if (rc) return rc;
+      src = (uint64_t*)ctx->scratch;
+    }
+    if (fused) {
+      e = keyswitch_fused_one(ctx, v, use_planes ? planes : nullptr, src, comps, comps_per_ct,
+                              src_comp[i], src_is_ntt[i] != 0, key_base[i], init_c1, i > 0,
+                              key_as, key_bs, out_c0, out_c1, count, num_limbs, lp, log_n, gd);
       continue;
     }
-    rc = ensure_scratch(ctx, (size_t)count * poly_words * 8);
-    if (rc) return rc;
-    src = (uint64_t*)ctx->scratch;
+    // The unfused path decomposes Montgomery representations of the
+    // coefficient-form source.
     e = hipMemcpy2DAsync(src, poly_words * 8, comps + (size_t)src_comp[i] * poly_words,
                          (size_t)comps_per_ct * poly_words * 8, poly_words * 8, count,
                          hipMemcpyDeviceToDevice, s);
-    // The fused kernel reads exported residues: fold ExportInt into the
-    // inverse transform's scaling (or export in place for coefficient-form
-    // input).  The unfused path decomposes Montgomery representations.
     if (e == hipSuccess && src_is_ntt[i]) {
       ProfScope prof(ctx, "keyswitch_intt");
       e = shg::launch_ntt(true, v, src, ctx->d_primes, lp, count * num_limbs, num_limbs, log_n,
-                          shg::kMaxLogM, s, fused ? 2 : 1);
+                          shg::kMaxLogM, s, 1);
     }
-    else if (e == hipSuccess && fused)
-      e = shg::launch_export(src, ctx->d_primes, lp, count, num_limbs, log_n, s);
     if (e != hipSuccess) break;
-    if (fused) {
-      shg::KeySwitchArgs ka{};
-      ka.src = src;
-      ka.src_stride = poly_words;
-      ka.c0 = comps;
-      ka.c0_stride = (size_t)comps_per_ct * poly_words;
-      ka.c1 = init_c1 ? comps + poly_words : nullptr;
-      ka.c1_stride = (size_t)comps_per_ct * poly_words;
-      ka.key_a = key_as;
-      ka.key_b = key_bs;
-      ka.key_base = key_base[i];
-      ka.out0 = out_c0;
-      ka.out1 = out_c1;
-      ka.out_stride = poly_words;
-      ka.accumulate = i > 0;
-      ProfScope prof(ctx, "keyswitch");
-      e = shg::launch_keyswitch(ka, gd, v & 3, ctx->d_primes, lp, count, num_limbs, log_n, s);
-    } else {
+    {
       // Unfused form for ring degrees outside the fused kernel's range:
       // materialise the digits, transform them, and multiply-accumulate.
       const size_t need = (size_t)count * poly_words * 8 * (1 + (size_t)gd.k);
@@ -977,22 +1034,27 @@ static int keyswitch_common(shg_context* ctx, const uint64_t* comps, int comps_p
                           lp, log_n, gd);
 }
 
-int shg_relin_apply(shg_context* ctx, const uint64_t* comps, int degree, int ntt_mask,
-                    const uint64_t* key_as, const uint64_t* key_bs, int num_key_polys,
-                    const int* host_log_bs, uint64_t* out_c0, uint64_t* out_c1, int count,
-                    int num_limbs, const int* primes) {
-  LimbPrimes lp{};
-  int log_n = 0;
-  int rc = check_primes(ctx, primes, num_limbs, &log_n, &lp);
+}  // extern "C"
+
+// Argument checks of RnsRelinKey::ApplyToRlweCiphertext
+// (rns/rns_relinearization_key.cc:184-232) shared by both relinearization
+// entry points; degree 1 is a plain copy (nothing to switch).  *done is set
+// when the call is complete (an error, count 0, or the copy).
+static int relin_prepare(shg_context* ctx, const uint64_t* comps, int degree, int ntt_mask,
+                         const uint64_t* key_as, const uint64_t* key_bs, int num_key_polys,
+                         const int* host_log_bs, uint64_t* out_c0, uint64_t* out_c1, int count,
+                         int num_limbs, const int* primes, LimbPrimes* lp, int* log_n,
+                         shg::GadgetDesc* gd, bool* done) {
+  *done = true;
+  int rc = check_primes(ctx, primes, num_limbs, log_n, lp);
   if (rc) return rc;
-  shg::GadgetDesc gd{};
-  rc = build_gadget(ctx, primes, num_limbs, host_log_bs, &gd);
+  rc = build_gadget(ctx, primes, num_limbs, host_log_bs, gd);
   if (rc) return rc;
   if (degree < 1 || degree > 16)
     return fail(SHG_E_INVALID_ARGUMENT, "ciphertext degree must be in [1, 16].");
   if (!(ntt_mask & 1)) return fail(SHG_E_INVALID_ARGUMENT, "RNS polynomial `this` must be in NTT form.");
   if (!(ntt_mask & 2)) return fail(SHG_E_INVALID_ARGUMENT, "RNS polynomial `this` must be in NTT form.");
-  const int needed = degree >= 2 ? (degree - 2) * degree + gd.k : 0;
+  const int needed = degree >= 2 ? (degree - 2) * degree + gd->k : 0;
   if (num_key_polys < needed)
     return fail(SHG_E_INVALID_ARGUMENT,
                 "`ciphertext` degree is larger than degree of this relinearization key.");
@@ -1000,7 +1062,7 @@ int shg_relin_apply(shg_context* ctx, const uint64_t* comps, int degree, int ntt
   if (!aligned16(comps) || !aligned16(out_c0) || !aligned16(out_c1) ||
       (degree >= 2 && (!aligned16(key_as) || !aligned16(key_bs))))
     return fail(SHG_E_INVALID_ARGUMENT, "data must be non-null 16-byte aligned pointers.");
-  const size_t poly_words = (size_t)num_limbs << log_n;
+  const size_t poly_words = (size_t)num_limbs << *log_n;
   if (degree == 1) {
     hipStream_t s = ctx->stream;
     hipError_t e = hipMemcpy2DAsync(out_c0, poly_words * 8, comps, 2 * poly_words * 8,
@@ -1010,15 +1072,89 @@ int shg_relin_apply(shg_context* ctx, const uint64_t* comps, int degree, int ntt
                            poly_words * 8, count, hipMemcpyDeviceToDevice, s);
     return e == hipSuccess ? SHG_OK : hip_fail(e, "copy");
   }
-  std::vector<int> src_comp, src_ntt, base;
+  *done = false;
+  return SHG_OK;
+}
+
+extern "C" {
+
+int shg_relin_apply(shg_context* ctx, const uint64_t* comps, int degree, int ntt_mask,
+                    const uint64_t* key_as, const uint64_t* key_bs, int num_key_polys,
+                    const int* host_log_bs, uint64_t* out_c0, uint64_t* out_c1, int count,
+                    int num_limbs, const int* primes) {
+  LimbPrimes lp{};
+  int log_n = 0;
+  shg::GadgetDesc gd{};
+  bool done = true;
+  int rc = relin_prepare(ctx, comps, degree, ntt_mask, key_as, key_bs, num_key_polys, host_log_bs,
+                         out_c0, out_c1, count, num_limbs, primes, &lp, &log_n, &gd, &done);
+  if (rc || done) return rc;
+  int src_comp[16], src_ntt[16], base[16];
   for (int i = 2; i <= degree; ++i) {
-    src_comp.push_back(i);
-    src_ntt.push_back((ntt_mask >> i) & 1);
-    base.push_back((i - 2) * degree);
+    src_comp[i - 2] = i;
+    src_ntt[i - 2] = (ntt_mask >> i) & 1;
+    base[i - 2] = (i - 2) * degree;
   }
-  return keyswitch_common(ctx, comps, degree + 1, src_comp.data(), src_ntt.data(), base.data(),
-                          (int)src_comp.size(), true, key_as, key_bs, host_log_bs, out_c0, out_c1,
-                          count, num_limbs, primes, lp, log_n, gd);
+  return keyswitch_common(ctx, comps, degree + 1, src_comp, src_ntt, base, degree - 1, true,
+                          key_as, key_bs, host_log_bs, out_c0, out_c1, count, num_limbs, primes,
+                          lp, log_n, gd);
+}
+
+int shg_relin_apply_noalloc(shg_context* ctx, const uint64_t* comps, int degree, int ntt_mask,
+                            const uint64_t* key_as, const uint64_t* key_bs, int num_key_polys,
+                            const int* host_log_bs, uint64_t* out_c0, uint64_t* out_c1,
+                            int count, int num_limbs, const int* primes) {
+  LimbPrimes lp{};
+  int log_n = 0;
+  shg::GadgetDesc gd{};
+  bool done = true;
+  int rc = relin_prepare(ctx, comps, degree, ntt_mask, key_as, key_bs, num_key_polys, host_log_bs,
+                         out_c0, out_c1, count, num_limbs, primes, &lp, &log_n, &gd, &done);
+  if (rc || done) return rc;
+  if (log_n < 4 || log_n > shg::kMaxLogM)
+    return fail(SHG_E_UNSUPPORTED,
+                "shg_relin_apply_noalloc supports ring degrees 2^4 to 2^%d; use shg_relin_apply.",
+                shg::kMaxLogM);
+  if (!ctx->noalloc_work)
+    return fail(SHG_E_DEVICE, "the no-allocation work area of device %d is unavailable: %s",
+                ctx->device, ctx->noalloc_error.c_str());
+  std::lock_guard<std::mutex> lock(ctx->mu);  // the autotuner's table
+  const int v = resolve_variant(ctx, SHG_NTT_AUTO, primes, num_limbs, log_n,
+                                (long long)count * num_limbs);
+  const bool planes_ok = keyswitch_planes_ok(v, log_n, gd);
+  const size_t n = (size_t)1 << log_n;
+  const size_t poly_words = (size_t)num_limbs * n;
+  // Work space per ciphertext: the digit planes of an NTT-form source, or
+  // the exported copy of a source that takes the other path.  Sources run
+  // one after the other on the stream and share the area.
+  size_t per_ct = 0;
+  for (int i = 2; i <= degree; ++i) {
+    const bool use_planes = planes_ok && ((ntt_mask >> i) & 1);
+    const size_t need = use_planes ? (size_t)gd.k * n * 2 : poly_words * 8;
+    per_ct = need > per_ct ? need : per_ct;
+  }
+  if (per_ct > kNoAllocWorkBytes)
+    return fail(SHG_E_INVALID_ARGUMENT,
+                "one ciphertext needs %zu bytes of work space, above the %zu bytes the "
+                "no-allocation key switch holds per device.", per_ct, kNoAllocWorkBytes);
+  // Ciphertexts are switched in chunks that fit the work area.
+  const int chunk = (int)std::min<size_t>((size_t)count, kNoAllocWorkBytes / per_ct);
+  const size_t ct_words = (size_t)(degree + 1) * poly_words;
+  hipError_t e = hipSuccess;
+  for (int c0 = 0; c0 < count && e == hipSuccess; c0 += chunk) {
+    const int cnt = std::min(chunk, count - c0);
+    for (int i = 2; i <= degree && e == hipSuccess; ++i) {
+      const bool is_ntt = (ntt_mask >> i) & 1;
+      const bool use_planes = planes_ok && is_ntt;
+      e = keyswitch_fused_one(ctx, v, use_planes ? (uint16_t*)ctx->noalloc_work : nullptr,
+                              use_planes ? nullptr : (uint64_t*)ctx->noalloc_work,
+                              comps + (size_t)c0 * ct_words, degree + 1, i, is_ntt,
+                              (i - 2) * degree, true, i > 2, key_as, key_bs,
+                              out_c0 + (size_t)c0 * poly_words, out_c1 + (size_t)c0 * poly_words,
+                              cnt, num_limbs, lp, log_n, gd);
+    }
+  }
+  return e == hipSuccess ? SHG_OK : hip_fail(e, "key switch launch");
 }
 
 int shg_galois_apply(shg_context* ctx, const uint64_t* comps, int c1_is_ntt, const uint64_t* key_as,

@@ -76,6 +76,7 @@ def lib():
         "shg_mod_reduce_lsb": [VP, VP, ctypes.c_uint64, U64P, i, i, i, INTP],
         "shg_tensor_product": [VP, VP, i, VP, i, VP, i, i, INTP],
         "shg_relin_apply": [VP, VP, i, i, VP, VP, i, INTP, VP, VP, i, i, INTP],
+        "shg_relin_apply_noalloc": [VP, VP, i, i, VP, VP, i, INTP, VP, VP, i, i, INTP],
         "shg_galois_apply": [VP, VP, i, VP, VP, INTP, VP, VP, i, i, INTP],
         "shg_gadget_decompose": [VP, VP, INTP, VP, i, i, INTP],
         "shg_lazy_fma": [VP, VP, sz, VP, sz, i, VP, i, INTP],
@@ -291,6 +292,12 @@ class Context:
         _check(lib().shg_relin_apply(self.h, comps.p, int(degree), int(ntt_mask), key_as.p,
                                      key_bs.p, int(num_key_polys), _ints(log_bs), out0.p,
                                      out1.p, int(count), len(primes), _ints(primes)))
+
+    def relin_apply_noalloc(self, comps, degree, ntt_mask, key_as, key_bs, num_key_polys, log_bs,
+                            out0, out1, count, primes):
+        _check(lib().shg_relin_apply_noalloc(self.h, comps.p, int(degree), int(ntt_mask), key_as.p,
+                                             key_bs.p, int(num_key_polys), _ints(log_bs), out0.p,
+                                             out1.p, int(count), len(primes), _ints(primes)))
 
     def galois_apply(self, comps, c1_is_ntt, key_as, key_bs, log_bs, out0, out1, count, primes):
         _check(lib().shg_galois_apply(self.h, comps.p, int(bool(c1_is_ntt)), key_as.p,
