@@ -24,6 +24,7 @@ sys.path.insert(0, os.path.join(ROOT, "shell-encryption_amd", "python"))
 sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 HBM_PEAK_GBS = 8000.0
+FP64_OPS_PEAK_T = 39.3  # 78.6 TFLOP/s FP64 vector, an FMA counted as two flops
 # HBM bytes per launch of the key-switch kernel, measured by rocprofv3
 # FETCH_SIZE + WRITE_SIZE and scaled by the same counters' reading for a
 # streaming kernel of known size (bench.py --copy-probe).
@@ -52,22 +53,79 @@ def percentile(v, p):
     return float(np.percentile(np.asarray(v), p))
 
 
-def cpu_baseline(qs, comps, kas, kbs, log_bs, budget_s=12.0):
-    """The oracle's relinearization (a scalar restatement of the reference's
-    RnsRelinKey::ApplyTo) on one host core, over a bounded sample."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def host_threads():
+    """Worker threads for the CPU baseline: the cores this process may use,
+    at most 16 (the host share of one GPU)."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    return max(1, min(16, avail))
+
+
+def _timed(fn, items, threads, budget_s, cap):
+    """Runs fn(item) over items (cycled) on `threads` host threads until
+    budget_s elapses or `cap` calls are done; returns (calls, seconds)."""
+    import threading
+    lock = threading.Lock()
+    state = {"next": 0}
+    t0 = time.perf_counter()
+
+    def worker():
+        while True:
+            with lock:
+                i = state["next"]
+                if i >= cap or time.perf_counter() - t0 >= budget_s:
+                    return
+                state["next"] = i + 1
+            fn(items[i % len(items)])
+
+    th = [threading.Thread(target=worker) for _ in range(threads)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    return state["next"], time.perf_counter() - t0
+
+
+def cpu_baseline(qs, comps, kas, kbs, log_bs, ntt_inputs):
+    """The oracle (a scalar C restatement of the reference's
+    RnsRelinKey::ApplyTo and ForwardNumberTheoreticTransform; its calls
+    release the interpreter lock) on this host: relinearizations on one
+    thread and on every available core (independent ciphertexts per thread,
+    as the reference has no threading of its own), and forward transforms
+    of RNS polynomials at (2^14, 8) likewise.  Bounded samples of a few
+    seconds each."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     rns = oracle.Rns(qs, LOG_N)
-    done, t0 = 0, time.perf_counter()
-    while True:
-        rns.relin_apply(comps[done % len(comps)], kas, kbs, log_bs)
-        done += 1
-        el = time.perf_counter() - t0
-        if el >= budget_s or done >= 64:
-            break
-    return {"value": done / el, "unit": "key-switches/s", "cores": 1, "kind": "port",
-            "sample": "%d relinearizations of degree-2 ciphertexts at n=2^14, L=8, k=40 by the "
-                      "oracle (single-threaded C restatement), %.1f s" % (done, el)}
+    threads = host_threads()
+    relin = lambda c: rns.relin_apply(c, kas, kbs, log_bs)  # noqa: E731
+    ntt = lambda x: rns.ntt_fwd(x)  # noqa: E731
+    r1, s1 = _timed(relin, comps, 1, 6.0, 48)
+    rt, st = _timed(relin, comps, threads, 6.0, 48 * threads)
+    n1, ns1 = _timed(ntt, ntt_inputs, 1, 3.0, 400)
+    nt, nst = _timed(ntt, ntt_inputs, threads, 3.0, 400 * threads)
+    return {"value": round(rt / st, 2), "unit": "key-switches/s", "cores": threads,
+            "kind": "port", "cpu_model": cpu_model(),
+            "single_thread_key_switches_per_s": round(r1 / s1, 2),
+            "ntts_per_s": {"threads_1": round(n1 / ns1, 1),
+                           "threads_%d" % threads: round(nt / nst, 1)},
+            "sample": "oracle relinearizations of degree-2 ciphertexts at n=2^14, L=8, k=40: "
+                      "%d on 1 thread in %.1f s, %d on %d threads in %.1f s; forward NTTs of RNS "
+                      "polynomials at (2^14, 8): %d in %.1f s on 1 thread, %d in %.1f s on %d "
+                      "threads" % (r1, s1, rt, threads, st, n1, ns1, nt, nst, threads)}
 
 
 def measured_traffic():
@@ -249,8 +307,8 @@ def uneven_batch(ctx, rng):
         ln, L = int(log_ns[p]), int(Ls[p])
         if L < 2:
             continue
-        rns_ql = oracle.Rns(primes[ln][:L], ln).ql_inv()
-        mjobs.append((d.ptr + 8 * int(offs[p]), pids[ln][:L], list(rns_ql), ln))
+        ql = ctx.ql_inv(pids[ln][:L])
+        mjobs.append((d.ptr + 8 * int(offs[p]), pids[ln][:L], list(ql), ln))
     mplan = ctx.mod_reduce_plan(mjobs, 65537, True)
     mplan.run(sg.BATCH_MOD_REDUCE)
     ctx.sync()
@@ -268,6 +326,56 @@ def uneven_batch(ctx, rng):
     return res
 
 
+SERVING_RATES = (25.0, 50.0, 100.0, 200.0, 400.0, 4000.0)
+
+
+def serving_checks(ctx, srv, db_id, db_dev, cols, pids, qs, keys, query, log_n, levels, log_bs):
+    """Bit-exactness of the serving leg: the server's answer to one query
+    equals the same query expanded and multiplied by direct calls; the
+    inner product matches a big-integer sum over all rows at sampled
+    positions; three expansion levels at this shape match the oracle."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    from common import expand_oracle
+    L, n = len(qs), 1 << log_n
+    rows = 1 << levels
+    answer, _ = srv.wait(srv.submit(db_id, query))
+    buf = ctx.empty((rows, 2, L, n))
+    init = np.zeros((rows, 2, L, n), dtype=np.uint64)
+    init[0] = query
+    buf.upload(init)
+    ctx.expand([buf], levels, keys, log_bs, pids)
+    out = ctx.empty((cols, 2, L, n))
+    ctx.pir_inner_product([buf], levels, db_dev, cols, [out], pids)
+    direct = out.download()
+    server_equals_direct = bool(np.array_equal(direct.ravel(), answer))
+    expd = buf.download()
+    recs = db_dev.download()  # exported records: x * 2^-64, so e * x is the Montgomery product
+    prng = np.random.default_rng(7)
+    ok = True
+    for _ in range(16):
+        limb, pos = int(prng.integers(0, L)), int(prng.integers(0, n))
+        q = qs[limb]
+        for comp in range(2):
+            e = expd[:, comp, limb, pos].astype(object)
+            for c in range(cols):
+                x = recs[c::cols, limb, pos][:rows].astype(object)
+                ok &= int(np.dot(e, x)) % q == int(direct[c, comp, limb, pos])
+    rns = oracle.Rns(qs, log_n)
+    host_keys = [(k[0].download(), k[1].download()) for k in keys[:3]]
+    small = ctx.empty((8, 2, L, n))
+    init8 = np.zeros((8, 2, L, n), dtype=np.uint64)
+    init8[0] = query
+    small.upload(init8)
+    ctx.expand([small], 3, keys[:3], log_bs, pids)
+    exp_ok = bool(np.array_equal(small.download(), expand_oracle(rns, query, 3, host_keys, log_bs)))
+    return {"server_equals_direct_calls": server_equals_direct,
+            "inner_product_sampled_bit_exact": bool(ok),
+            "inner_product_sample": "16 (limb, position) pairs x 2 components x %d columns, "
+                                    "sums over all %d rows with big integers" % (cols, rows),
+            "expansion_3_levels_vs_oracle": exp_ok}
+
+
 def serving(ctx, rng):
     """The serving workload: encrypted queries (one ciphertext at n=2^12,
     L=2 primes below 2^50) against three device-resident synthetic databases
@@ -275,8 +383,9 @@ def serving(ctx, rng):
     uniform mod t=65537), each query expanded by 12 levels of oblivious
     expansion (Galois substitutions + key switching, base 2^10) into 4096
     row selectors and then multiplied into the database (4096 rows x 4, 16
-    or 64 columns).  Poisson arrivals at a low and a saturating rate; the
-    server admits by HBM footprint and batches requests of one stage."""
+    or 64 columns).  Poisson arrivals swept over offered rates through the
+    knee; the server admits by HBM footprint, starts newcomers in cohorts
+    of four while others run, and batches requests of one stage."""
     import shell_gpu as sg
     log_n, Lq, levels, t, log_b = 12, 2, 12, 65537, 10
     qs = primes_for(log_n, Lq)
@@ -297,25 +406,35 @@ def serving(ctx, rng):
     ctx.sync()
     queries = uniform(rng, qs, (8, 2), n)
     budget = 16 << 30
-    srv = sg.PirServer(ctx, pids, log_n, [log_b] * Lq, keys, max_active=16, hbm_budget=budget)
+    cohort = 4
+    srv = sg.PirServer(ctx, pids, log_n, [log_b] * Lq, keys, max_active=16, hbm_budget=budget,
+                       cohort=cohort)
     ids = [srv.add_db(d, c) for d, c in dbs]
     srv.start()
     srv.wait(srv.submit(ids[0], queries[0]))  # warm-up: first-use allocations
+    print("[bench] serving: bit-exactness checks", file=sys.stderr, flush=True)
     res = {"workload": "n=2^12, L=2, 12 expansion levels (4096 rows), t=65537, base 2^10",
-           "capacity": srv.stats()["capacity"],
+           "capacity": srv.stats()["capacity"], "cohort": cohort,
            "per_query_hbm_bytes": ((rows + 64) * 2 * Lq * n * 8), "dbs": {}}
+    res["checks"] = serving_checks(ctx, srv, ids[0], dbs[0][0], dbs[0][1], pids, qs, keys,
+                                   queries[0], log_n, levels, [log_b] * Lq)
+    st0 = srv.stats()
     for (d, cols), records, db in zip(dbs, db_sizes, ids):
-        entry = {"records": records, "columns": cols, "db_bytes": records * Lq * n * 8}
-        for name, rate, count in (("low_rate", 20.0, 12), ("high_rate", 4000.0, 64)):
-            print("[bench] serving: %d records, %s" % (records, name), file=sys.stderr, flush=True)
+        entry = {"records": records, "columns": cols, "db_bytes": records * Lq * n * 8,
+                 "rates": []}
+        for rate in SERVING_RATES:
+            count = int(min(96, max(16, rate * 0.4)))
+            print("[bench] serving: %d records, %.0f/s offered" % (records, rate),
+                  file=sys.stderr, flush=True)
             lat, wall = srv.load_test(db, queries, count, rate, seed=records + int(rate))
-            entry[name] = {"offered_qps": rate, "requests": count,
-                           "queries_per_s": round(count / wall, 2),
-                           "p50_ms": round(percentile(lat, 50), 3),
-                           "p99_ms": round(percentile(lat, 99), 3)}
+            entry["rates"].append({"offered_qps": rate, "requests": count,
+                                   "queries_per_s": round(count / wall, 2),
+                                   "p50_ms": round(percentile(lat, 50), 3),
+                                   "p99_ms": round(percentile(lat, 99), 3)})
         res["dbs"][str(records)] = entry
     st = srv.stats()
-    res["mean_batch"] = round(st["batched_items"] / max(st["stage_runs"], 1), 2)
+    res["mean_batch"] = round((st["batched_items"] - st0["batched_items"]) /
+                              max(st["stage_runs"] - st0["stage_runs"], 1), 2)
     srv.close()
     del dbs
     return res
@@ -398,32 +517,36 @@ def main():
         e0, e1 = rns.relin_apply(comps[c], kas, kbs, log_bs)
         bit_exact &= bool(np.array_equal(o0[c], e0) and np.array_equal(o1[c], e1))
 
-    # Dominant kernel roofline: algorithmic bytes per ciphertext = its gadget
-    # digits as 16-bit planes (k*n*2, written by the inverse-transform step)
-    # + c0, c1 (2*L*n*8) + output (2*L*n*8), plus the key (2*k*L*n*8) shared
-    # by the whole launch.
+    # Dominant kernel: keyswitch_kernel, bound by FP64 vector issue.  Work
+    # per key switch (relinearization of one degree-2 ciphertext at n=2^14,
+    # L=8, k=40): 320 forward limb transforms of 114,688 butterflies, each
+    # butterfly one exact FMA-based modular product (6 FP64 operations) and
+    # two additions = 8 operations, plus 2*k*L*n = 10,485,760 key
+    # multiply-accumulate terms of 7 operations (the product and the add).
+    # Peak: MI355X's 78.6 TFLOP/s FP64 vector rate counts an FMA as two
+    # flops, i.e. 39.3 T FP64 operations issued per second.
     ks_avg_ms = ks_ms / max(ks_launches, 1)
-    ks_bytes = BATCH * (k * n * 2 + 4 * poly_words * 8) + 2 * k * poly_words * 8
+    bfly_per_ks = L * k * (n // 2) * LOG_N
+    mac_per_ks = 2 * k * L * n
+    ops_per_ks = bfly_per_ks * 8 + mac_per_ks * 7
+    ops = BATCH * ops_per_ks
+    valu_tops = ops / (ks_avg_ms * 1e-3) / 1e12
+    # HBM, secondary: per key switch c2 (1 MiB) + c0, c1 (2 MiB) + output
+    # (2 MiB), and the 80 MiB key once per launch.
+    ks_bytes = BATCH * 5 * poly_words * 8 + 2 * k * poly_words * 8
     ks_gbs = ks_bytes / (ks_avg_ms * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "achieved": round(ks_gbs, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(ks_gbs / HBM_PEAK_GBS, 4), "traffic": measured_traffic(),
+    roofline = {"bound": "valu", "achieved": round(valu_tops, 2), "peak": FP64_OPS_PEAK_T,
+                "unit": "T FP64 ops/s", "frac": round(valu_tops / FP64_OPS_PEAK_T, 4),
+                "traffic": measured_traffic(),
                 "kernel": "keyswitch_kernel<13,4,1,F64KsArith,planes>",
                 "avg_launch_ms": round(ks_avg_ms, 4),
-                "algorithmic_bytes_per_launch": ks_bytes}
+                "ops_per_launch": ops,
+                "op_costs": {"butterflies_per_key_switch": bfly_per_ks, "ops_per_butterfly": 8,
+                             "macs_per_key_switch": mac_per_ks, "ops_per_mac": 7},
+                "hbm": {"achieved": round(ks_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(ks_gbs / HBM_PEAK_GBS, 4),
+                        "algorithmic_bytes_per_launch": ks_bytes}}
 
-    # The key switch is bound by FP64 vector arithmetic, not HBM: its
-    # algorithmic FP64 operations per launch (a Cooley-Tukey butterfly = one
-    # exact FMA-based product of 6 operations + 2 additions; stage 0 of a
-    # half-limb = product + 1 addition; a key multiply-accumulate term = 7)
-    # against MI355X's FP64 vector issue rate (78.6 TFLOP/s counting an FMA
-    # as two flops = 39.3 T operations/s).
-    half = n // 2
-    bfly = BATCH * L * k * (2 * (half // 2) * (LOG_N - 1) * 8 + 2 * half * 7)
-    mac = BATCH * L * k * n * 2 * 7
-    ops = bfly + mac
-    fp64_valu = {"achieved": round(ops / (ks_avg_ms * 1e-3) / 1e12, 2), "peak": 39.3,
-                 "unit": "T FP64 ops/s", "frac": round(ops / (ks_avg_ms * 1e-3) / 39.3e12, 4),
-                 "ops_per_launch": ops}
     out = {
         "metric": "key-switches/s & NTTs/s at n=2^14,L=8 batched vs plain (bit-exact); p50/p99",
         "value": round(value, 1), "unit": "key-switches/s", "n_gpus": 1,
@@ -436,7 +559,6 @@ def main():
         "latency_ms": {"p50": round(percentile(lat, 50), 4), "p99": round(percentile(lat, 99), 4)},
         "bit_exact": bit_exact,
         "roofline": roofline,
-        "fp64_valu": fp64_valu,
         "keyswitch_intt_ms_per_step": round(intt_ms / max(intt_launches, 1), 4),
     }
 
@@ -515,7 +637,7 @@ def main():
         ctx.sync()
         out["galois_keyswitches_per_s"] = round(3 * BATCH / (time.perf_counter() - t0), 1)
         del gal_in, d_sub
-        ql_inv = rns.ql_inv()
+        ql_inv = ctx.ql_inv(pids)
         ms_in = ctx.array(comps[:, :2])
         ctx.mod_reduce_lsb(ms_in, 65537, ql_inv, True, BATCH * 2, pids)
         ctx.sync()
@@ -552,7 +674,8 @@ def main():
 
     if not args.no_cpu:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(qs, comps[:8], kas, kbs, log_bs)
+        out["cpu_baseline"] = cpu_baseline(qs, comps[:16], kas, kbs, log_bs,
+                                           uniform(rng, qs, (8,), n))
     ctx.close()
     print(json.dumps(out), flush=True)
     return 0

@@ -273,6 +273,8 @@ typedef struct shg_pir_server_config {
   int levels;              /* expansion levels (1..log_n): 2^levels database rows */
   int max_active;          /* slots: most requests on the device at once */
   size_t hbm_budget_bytes; /* per-request buffer budget; 0 = slots only */
+  int cohort;              /* while requests run, admit newcomers only when this
+                              many can start together (0 or 1: at once) */
 } shg_pir_server_config;
 int shg_pir_server_create(shg_context* ctx, const int* primes, int num_limbs,
                           const int* host_log_bs, const uint64_t* const* key_as,

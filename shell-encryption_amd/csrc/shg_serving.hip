@@ -22,6 +22,7 @@
 #include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <algorithm>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -151,7 +152,15 @@ void shg_pir_server::run() {
       if (active.empty())
         cv_work.wait(lock, [&] { return stop || !pending.empty(); });
       if (stop) break;
-      while (!pending.empty() && (int)active.size() < capacity) {
+      // Cohorts: while requests are on the device, newcomers wait until
+      // `cohort` of them can start together, so that they advance through
+      // the stages (and share the batched calls) as one group.  An idle
+      // device admits whatever is queued.
+      const int cohort = std::max(1, std::min(cfg.cohort, capacity));
+      const int free_slots = capacity - (int)active.size();
+      const bool admit = active.empty() ||
+                         (free_slots >= cohort && (int)pending.size() >= cohort);
+      while (admit && !pending.empty() && (int)active.size() < capacity) {
         Slot* free_slot = nullptr;
         for (auto& sl : slots)
           if (!sl.req) {
@@ -233,6 +242,7 @@ int shg_pir_server_create(shg_context* ctx, const int* primes, int num_limbs,
   if (cfg->levels < 1 || cfg->levels > cfg->log_n)
     return set_error(SHG_E_INVALID_ARGUMENT, "levels must be in [1, log_n].");
   if (cfg->max_active < 1) return set_error(SHG_E_INVALID_ARGUMENT, "max_active must be positive.");
+  if (cfg->cohort < 0) return set_error(SHG_E_INVALID_ARGUMENT, "cohort must be non-negative.");
   if (!key_as || !key_bs) return set_error(SHG_E_INVALID_ARGUMENT, "one Galois key per level is required.");
   auto s = std::make_unique<shg_pir_server>();
   s->ctx = ctx;

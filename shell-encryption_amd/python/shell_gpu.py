@@ -37,7 +37,7 @@ class LimbJob(ctypes.Structure):
 
 class PirServerConfig(ctypes.Structure):
     _fields_ = [("log_n", ctypes.c_int), ("levels", ctypes.c_int), ("max_active", ctypes.c_int),
-                ("hbm_budget_bytes", ctypes.c_size_t)]
+                ("hbm_budget_bytes", ctypes.c_size_t), ("cohort", ctypes.c_int)]
 
 
 class PolyJob(ctypes.Structure):
@@ -444,13 +444,14 @@ class Graph:
 class PirServer:
     """Continuous-batching server of encrypted queries (shg_pir_server_*)."""
 
-    def __init__(self, ctx, primes, log_n, log_bs, keys, max_active, hbm_budget=0):
+    def __init__(self, ctx, primes, log_n, log_bs, keys, max_active, hbm_budget=0, cohort=0):
         self.ctx = ctx
         self.levels = len(keys)
         self.keys = keys  # keep the key arrays alive
         ka = Context._ptrs([k[0] for k in keys])
         kb = Context._ptrs([k[1] for k in keys])
-        cfg = PirServerConfig(int(log_n), self.levels, int(max_active), int(hbm_budget))
+        cfg = PirServerConfig(int(log_n), self.levels, int(max_active), int(hbm_budget),
+                              int(cohort))
         self.h = VP()
         _check(lib().shg_pir_server_create(ctx.h, _ints(primes), len(primes), _ints(log_bs), ka, kb,
                                            ctypes.byref(cfg), ctypes.byref(self.h)))

@@ -41,8 +41,8 @@ def setup(ctx):
                 queries=queries)
 
 
-def _server(ctx, s, max_active, budget=0):
-    srv = sg.PirServer(ctx, s["pids"], LOG_N, s["log_bs"], s["dkeys"], max_active, budget)
+def _server(ctx, s, max_active, budget=0, cohort=0):
+    srv = sg.PirServer(ctx, s["pids"], LOG_N, s["log_bs"], s["dkeys"], max_active, budget, cohort)
     ids = [srv.add_db(s["dbx"][0], C), srv.add_db(s["dbx"][1], C + 1)]
     srv.start()
     return srv, ids
@@ -82,4 +82,23 @@ def test_hbm_budget_limits_admission(ctx, setup):
     lat, wall = srv.load_test(dbs[0], np.stack(s["queries"][:2]), 4, rate=2000.0)
     srv.close()
     assert len(outs) == 3 and all(o.size == C * 2 * L * n for o in outs)
+    assert np.all(lat > 0) and wall > 0
+
+
+def test_cohort_admission_same_answers(ctx, setup):
+    """Newcomers admitted in cohorts of three while others run: every answer
+    equals the one computed alone, under a Poisson load as well."""
+    s = setup
+    srv, dbs = _server(ctx, s, max_active=6, cohort=3)
+    alone = [srv.wait(srv.submit(dbs[0], q))[0] for q in s["queries"]]
+    first = [srv.submit(dbs[0], q) for q in s["queries"][:2]]
+    later = [srv.submit(dbs[0], q) for q in s["queries"]]
+    got_first = [srv.wait(t)[0] for t in first]
+    got_later = [srv.wait(t)[0] for t in later]
+    lat, wall = srv.load_test(dbs[0], np.stack(s["queries"]), 12, rate=500.0)
+    srv.close()
+    for g, m in enumerate(got_first):
+        assert np.array_equal(alone[g], m), g
+    for g, m in enumerate(got_later):
+        assert np.array_equal(alone[g], m), g
     assert np.all(lat > 0) and wall > 0
