@@ -219,6 +219,33 @@ int shg_lazy_fma(shg_context* ctx, const uint64_t* as, size_t as_stride,
                  const uint64_t* bs, size_t bs_stride, int num_terms,
                  uint64_t* out, int num_limbs, const int* primes);
 
+/* ---- wire formats --------------------------------------------------------
+ * RnsPolynomial::Serialize / Deserialize (rns/rns_polynomial.h:99-140) store
+ * one byte string per limb, MontgomeryInt::SerializeVector /
+ * DeserializeVector (montgomery.cc:143-219): the n representations of limb i
+ * as consecutive log2(q_i)-bit fields of a little-endian bit stream
+ * (TranscribeBits, transcription.h:60-150), ceil(n * log2(q_i) / 8) bytes.
+ * Here one polynomial's strings lie back to back; *poly_bytes is their sum
+ * and limb_bytes (may be NULL) receives each string's length.  n <= 2^15
+ * (kMaxNumCoeffs). */
+int shg_serialized_bytes(shg_context* ctx, const int* primes, int num_limbs,
+                         size_t* limb_bytes, size_t* poly_bytes);
+/* count polynomials (device words) -> count * poly_bytes device bytes. */
+int shg_serialize_polys(shg_context* ctx, const uint64_t* polys, int count,
+                        int num_limbs, const int* primes, uint8_t* out);
+/* The inverse, straight into device buffers; like the reference, the fields
+ * are taken as representations without a range check. */
+int shg_deserialize_polys(shg_context* ctx, const uint8_t* in, int count,
+                          int num_limbs, const int* primes, uint64_t* polys);
+/* RnsPolynomial::SampleUniform (rns/rns_polynomial.h:640-667) of `count`
+ * polynomials drawn in order from one SingleThreadChaChaPrng seeded with the
+ * 32-byte host_seed (prng/single_thread_chacha_prng.cc:36-62): the random
+ * pad that RnsGaloisKey regenerates from its seed
+ * (rns/rns_galois_key.cc:81-105, 204-236).  Output is limb-major per
+ * polynomial (NTT form, as the reference marks it); moduli of 9..62 bits. */
+int shg_sample_uniform_chacha(shg_context* ctx, const uint8_t* host_seed, size_t seed_len,
+                              int count, int num_limbs, const int* primes, uint64_t* out);
+
 /* ---- encrypted queries against a device-resident database --------------
  * Oblivious expansion (ObliviousExpander::ExpandCiphertext,
  * oblivious_expand.h:173-227, there on the non-RNS stack) on RNS

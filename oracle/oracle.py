@@ -104,8 +104,75 @@ def lib():
         L.so_lazy_fma.argtypes = [PP, ctypes.c_int, ctypes.c_int, U64P, U64P, U64P]
         L.so_fill_uniform.argtypes = [PP, ctypes.c_int, u64, U64P]
         L.so_now_seconds.restype = ctypes.c_double
+        U8P = ctypes.POINTER(ctypes.c_uint8)
+        sz = ctypes.c_size_t
+        L.so_transcribe_bits.argtypes = [U64P, sz, ctypes.c_int, ctypes.c_longlong, ctypes.c_int,
+                                         ctypes.c_int, ctypes.c_int, U64P, sz]
+        L.so_transcribe_bits.restype = ctypes.c_longlong
+        L.so_serialize_vector.argtypes = [MP, U64P, sz, U8P, sz]
+        L.so_serialize_vector.restype = ctypes.c_longlong
+        L.so_deserialize_vector.argtypes = [MP, ctypes.c_int, U8P, sz, U64P]
+        L.so_deserialize_vector.restype = ctypes.c_int
+        L.so_chacha20_block.argtypes = [U8P, ctypes.c_uint32, U8P, U8P]
+        L.so_chacha_prng_init.argtypes = [ctypes.c_void_p, U8P, sz]
+        L.so_chacha_prng_init.restype = ctypes.c_int
+        L.so_chacha_rand8.argtypes = [ctypes.c_void_p]
+        L.so_chacha_rand8.restype = ctypes.c_uint8
+        L.so_chacha_rand64.argtypes = [ctypes.c_void_p]
+        L.so_chacha_rand64.restype = u64
+        L.so_sample_uniform_chacha.argtypes = [PP, ctypes.c_int, ctypes.c_void_p, U64P]
         _lib = L
     return _lib
+
+
+U8P_ = ctypes.POINTER(ctypes.c_uint8)
+CHACHA_OUTPUT_BYTES = 255 * 32
+
+
+def _p8(a):
+    assert a.dtype == np.uint8 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(U8P_)
+
+
+def transcribe_bits(values, in_type_bits, bit_length, in_bits, out_bits, out_type_bits):
+    """TranscribeBits over integers given as a uint64 array."""
+    v = np.ascontiguousarray(values, dtype=np.uint64)
+    cap = max(1, -(-int(bit_length) // int(out_bits)))
+    out = np.zeros(cap, dtype=np.uint64)
+    r = lib().so_transcribe_bits(v.ctypes.data_as(U64P), v.size, int(in_type_bits),
+                                 int(bit_length), int(in_bits), int(out_bits), int(out_type_bits),
+                                 out.ctypes.data_as(U64P), cap)
+    if r < 0:
+        raise ValueError("TranscribeBits error %d" % r)
+    return out[:r]
+
+
+def chacha20_block(key, counter, nonce):
+    k = np.frombuffer(bytes(key), dtype=np.uint8).copy()
+    nn = np.frombuffer(bytes(nonce), dtype=np.uint8).copy()
+    out = np.zeros(64, dtype=np.uint8)
+    lib().so_chacha20_block(_p8(k), int(counter), _p8(nn), _p8(out))
+    return out.tobytes()
+
+
+class ChaChaPrng:
+    """SingleThreadChaChaPrng restated (prng/single_thread_chacha_prng.cc)."""
+
+    def __init__(self, key):
+        self._buf = ctypes.create_string_buffer(32 + 8 + CHACHA_OUTPUT_BYTES + 64)
+        k = np.frombuffer(bytes(key), dtype=np.uint8).copy()
+        if lib().so_chacha_prng_init(ctypes.addressof(self._buf), _p8(k), k.size) != 0:
+            raise ValueError("the ChaCha PRNG takes a 32-byte key")
+
+    @property
+    def h(self):
+        return ctypes.addressof(self._buf)
+
+    def rand8(self):
+        return int(lib().so_chacha_rand8(self.h))
+
+    def rand64(self):
+        return int(lib().so_chacha_rand64(self.h))
 
 
 def _p64(a):
@@ -260,6 +327,41 @@ class Rns:
         out = np.zeros((count, self.L, self.n), dtype=np.uint64)
         for c in range(count):
             lib().so_fill_uniform(self._arr, self.L, int(seed) * 1000003 + c, _p64(out[c]))
+        return out
+
+    def serialize(self, x):
+        """RnsPolynomial::Serialize's coefficient vectors: one byte string per
+        limb (MontgomeryInt::SerializeVector)."""
+        x = self._poly(x).reshape(self.L, self.n)
+        out = []
+        for i, p in enumerate(self.primes):
+            b = p.mont.s.log_modulus
+            buf = np.zeros((self.n * b + 7) // 8, dtype=np.uint8)
+            r = lib().so_serialize_vector(p.mont.ref(), _p64(np.ascontiguousarray(x[i])), self.n,
+                                          _p8(buf), buf.size)
+            if r < 0:
+                raise ValueError("SerializeVector error %d" % r)
+            out.append(buf[:r].tobytes())
+        return out
+
+    def deserialize(self, vectors):
+        """RnsPolynomial::Deserialize's coefficients from per-limb strings."""
+        assert len(vectors) == self.L
+        out = np.zeros((self.L, self.n), dtype=np.uint64)
+        for i, (p, v) in enumerate(zip(self.primes, vectors)):
+            b = np.frombuffer(bytes(v), dtype=np.uint8).copy() if len(v) else np.zeros(1, np.uint8)
+            row = np.zeros(self.n, dtype=np.uint64)
+            r = lib().so_deserialize_vector(p.mont.ref(), self.n, _p8(b), len(v), _p64(row))
+            if r != 0:
+                raise ValueError("DeserializeVector error %d" % r)
+            out[i] = row
+        return out
+
+    def sample_uniform_chacha(self, prng, count):
+        """`count` RnsPolynomial::SampleUniform draws from one ChaCha PRNG."""
+        out = np.zeros((count, self.L, self.n), dtype=np.uint64)
+        for c in range(count):
+            lib().so_sample_uniform_chacha(self._arr, self.L, prng.h, _p64(out[c]))
         return out
 
     def ntt_fwd(self, x):

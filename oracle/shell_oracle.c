@@ -714,3 +714,194 @@ double so_now_seconds(void) {
   clock_gettime(CLOCK_MONOTONIC, &ts);
   return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
 }
+
+/* ------------------------------------------------------------------------ */
+/* L8: wire formats and the ChaCha PRNG                                      */
+/* ------------------------------------------------------------------------ */
+
+static uint64_t so_low_mask(int bits) { return bits >= 64 ? ~0ull : (((uint64_t)1 << bits) - 1); }
+
+/* TranscribeBits<InputInt, OutputInt> (transcription.h:60-150): `in` holds
+ * input integers of in_type_bits (8 or 64) bits, of which the low
+ * input_bits_per_int are used; output integers receive
+ * output_bits_per_int bits each.  Returns the number of output integers, or
+ * a negative error (the reference's InvalidArgument cases). */
+long long so_transcribe_bits(const uint64_t* in, size_t in_len, int in_type_bits,
+                             long long input_bit_length, int input_bits_per_int,
+                             int output_bits_per_int, int out_type_bits, uint64_t* out,
+                             size_t out_cap) {
+  if (input_bits_per_int > in_type_bits) return -1;
+  if (output_bits_per_int > out_type_bits) return -2;
+  if (input_bit_length < 0) return -3;
+  if (input_bit_length == 0) return in_len == 0 ? 0 : -4;
+  const size_t input_chunks =
+      (size_t)((input_bit_length + input_bits_per_int - 1) / input_bits_per_int);
+  if (in_len < input_chunks) return -5;
+  const size_t output_chunks =
+      (size_t)((input_bit_length + output_bits_per_int - 1) / output_bits_per_int);
+  if (out_cap < output_chunks) return -6;
+  memset(out, 0, output_chunks * sizeof(uint64_t));
+  const uint64_t in_mask = so_low_mask(in_type_bits);
+  long long remaining = input_bit_length;
+  size_t op = 0;
+  int size_out = (int)(remaining < output_bits_per_int ? remaining : output_bits_per_int);
+  int needed = size_out;
+  for (size_t i = 0; i < input_chunks; i++) {
+    int nin = (int)(input_bits_per_int < remaining ? input_bits_per_int : remaining);
+    /* The chunk's bits moved to the top of an input-width integer. */
+    const uint64_t input = (in[i] << (in_type_bits - nin)) & in_mask;
+    while (nin > 0) {
+      if (needed == 0) {
+        op++;
+        size_out = (int)(remaining < output_bits_per_int ? remaining : output_bits_per_int);
+        needed = size_out;
+      }
+      const int nproc = nin < needed ? nin : needed;
+      const uint64_t bits_left = (input << (nin - nproc)) & in_mask;
+      const uint64_t mask = bits_left >> (in_type_bits - nproc);
+      out[op] |= (mask << (size_out - needed)) & so_low_mask(out_type_bits);
+      nin -= nproc;
+      needed -= nproc;
+    }
+    remaining -= input_bits_per_int;
+  }
+  return (long long)output_chunks;
+}
+
+/* MontgomeryInt<Uint64>::SerializeVector (montgomery.cc:143-167): the
+ * representations' low log_modulus bits, packed into bytes.  Returns the
+ * byte count, or a negative error. */
+long long so_serialize_vector(const so_mont* p, const uint64_t* coeffs, size_t count,
+                              uint8_t* out, size_t out_cap) {
+  if (count > ((size_t)1 << 15)) return -10;  /* kMaxNumCoeffs (constants.h:75) */
+  if (count == 0) return -11;                 /* "Cannot serialize an empty vector." */
+  const int b = (int)p->log_modulus;
+  const size_t nbytes = (count * (size_t)b + 7) / 8;
+  if (out_cap < nbytes) return -6;
+  uint64_t* tmp = (uint64_t*)malloc(nbytes * sizeof(uint64_t));
+  const long long r = so_transcribe_bits(coeffs, count, 64, (long long)count * b, b, 8, 8, tmp,
+                                         nbytes);
+  for (long long i = 0; i < r; i++) out[i] = (uint8_t)tmp[i];
+  free(tmp);
+  return r;
+}
+
+/* MontgomeryInt<Uint64>::DeserializeVector (montgomery.cc:184-219): the
+ * coefficients are taken as representations without reduction. */
+int so_deserialize_vector(const so_mont* p, int num_coeffs, const uint8_t* in, size_t in_len,
+                          uint64_t* coeffs) {
+  if (num_coeffs < 0) return -12;
+  if (num_coeffs > (1 << 15)) return -10;
+  const int b = (int)p->log_modulus;
+  uint64_t* tmp = (uint64_t*)malloc((in_len ? in_len : 1) * sizeof(uint64_t));
+  for (size_t i = 0; i < in_len; i++) tmp[i] = in[i];
+  const long long r = so_transcribe_bits(tmp, in_len, 8, (long long)b * num_coeffs, 8, b, 64,
+                                         coeffs, (size_t)num_coeffs);
+  free(tmp);
+  if (r < 0) return (int)r;
+  if (r < num_coeffs) return -13; /* "Given serialization is invalid." */
+  return 0;
+}
+
+/* ChaCha20 block function of RFC 7539 Sec. 2.3 (used by the reference through
+ * BoringSSL's CRYPTO_chacha_20, prng/chacha_prng_util.cc:36-60). */
+#define SO_ROTL32(v, c) (((v) << (c)) | ((v) >> (32 - (c))))
+#define SO_QR(a, b, c, d)            \
+  a += b; d ^= a; d = SO_ROTL32(d, 16); \
+  c += d; b ^= c; b = SO_ROTL32(b, 12); \
+  a += b; d ^= a; d = SO_ROTL32(d, 8);  \
+  c += d; b ^= c; b = SO_ROTL32(b, 7);
+static uint32_t so_le32(const uint8_t* p) {
+  return (uint32_t)p[0] | (uint32_t)p[1] << 8 | (uint32_t)p[2] << 16 | (uint32_t)p[3] << 24;
+}
+void so_chacha20_block(const uint8_t key[32], uint32_t counter, const uint8_t nonce[12],
+                       uint8_t out[64]) {
+  uint32_t s[16], x[16];
+  s[0] = 0x61707865u; s[1] = 0x3320646eu; s[2] = 0x79622d32u; s[3] = 0x6b206574u;
+  for (int i = 0; i < 8; i++) s[4 + i] = so_le32(key + 4 * i);
+  s[12] = counter;
+  for (int i = 0; i < 3; i++) s[13 + i] = so_le32(nonce + 4 * i);
+  memcpy(x, s, sizeof(s));
+  for (int r = 0; r < 10; r++) {
+    SO_QR(x[0], x[4], x[8], x[12]) SO_QR(x[1], x[5], x[9], x[13])
+    SO_QR(x[2], x[6], x[10], x[14]) SO_QR(x[3], x[7], x[11], x[15])
+    SO_QR(x[0], x[5], x[10], x[15]) SO_QR(x[1], x[6], x[11], x[12])
+    SO_QR(x[2], x[7], x[8], x[13]) SO_QR(x[3], x[4], x[9], x[14])
+  }
+  for (int i = 0; i < 16; i++) {
+    const uint32_t v = x[i] + s[i];
+    out[4 * i] = (uint8_t)v; out[4 * i + 1] = (uint8_t)(v >> 8);
+    out[4 * i + 2] = (uint8_t)(v >> 16); out[4 * i + 3] = (uint8_t)(v >> 24);
+  }
+}
+
+/* ChaChaPrngResalt (prng/chacha_prng_util.cc:27-60): the buffer is the
+ * ChaCha20 keystream (block counter from 0) under nonce "salt" followed by
+ * the 64-bit little-endian salt counter, kChaChaOutputBytes = 255 * 32 long. */
+static void so_chacha_resalt(so_chacha_prng* g) {
+  uint8_t nonce[12] = {'s', 'a', 'l', 't'};
+  uint64_t c = (uint64_t)g->salt_counter;
+  for (int i = 0; i < 8; i++) { nonce[4 + i] = (uint8_t)(c & 0xff); c >>= 8; }
+  for (int blk = 0; blk * 64 < SO_CHACHA_OUTPUT_BYTES; blk++) {
+    uint8_t ks[64];
+    so_chacha20_block(g->key, (uint32_t)blk, nonce, ks);
+    const int take = SO_CHACHA_OUTPUT_BYTES - blk * 64 < 64 ? SO_CHACHA_OUTPUT_BYTES - blk * 64 : 64;
+    memcpy(g->buffer + blk * 64, ks, (size_t)take);
+  }
+  g->salt_counter++;
+  g->position = 0;
+}
+
+/* SingleThreadChaChaPrng::Create (prng/single_thread_chacha_prng.cc:36-52):
+ * a 32-byte key, the first buffer drawn at once. */
+int so_chacha_prng_init(so_chacha_prng* g, const uint8_t* key, size_t key_len) {
+  if (key_len != 32) return -1;
+  memcpy(g->key, key, 32);
+  g->salt_counter = 0;
+  so_chacha_resalt(g);
+  return 0;
+}
+
+/* ChaChaPrngRand8 / ChaChaPrngRand64 (prng/chacha_prng_util.cc:76-104):
+ * bytes in order, eight of them little-endian for 64 bits. */
+uint8_t so_chacha_rand8(so_chacha_prng* g) {
+  if (g->position >= SO_CHACHA_OUTPUT_BYTES) so_chacha_resalt(g);
+  return g->buffer[g->position++];
+}
+uint64_t so_chacha_rand64(so_chacha_prng* g) {
+  uint64_t r = 0;
+  for (int i = 0; i < 8; i++) r += (uint64_t)so_chacha_rand8(g) << (8 * i);
+  return r;
+}
+
+/* MontgomeryInt::GenerateRandomInt (montgomery.h:631-660) and ImportRandom
+ * (montgomery.h:260-273): log_modulus random bits (64-bit draws, a final
+ * 8-bit draw for the last <= 8 bits), rejected until below the modulus and
+ * stored directly as the representation. */
+uint64_t so_import_random_chacha(const so_mont* p, so_chacha_prng* g) {
+  for (;;) {
+    int bits = (int)p->log_modulus;
+    uint64_t r = 0;
+    while (bits > 0) {
+      if (bits <= 8) {
+        const uint64_t v = so_chacha_rand8(g);
+        r = (r << bits) + (v & so_low_mask(bits));
+        break;
+      }
+      const uint64_t v = so_chacha_rand64(g);
+      const int take = bits < 64 ? bits : 64;
+      r = (take >= 64 ? 0 : (r << take)) + (v & so_low_mask(take));
+      bits -= take;
+    }
+    if (r < p->modulus) return r;
+  }
+}
+
+/* RnsPolynomial::SampleUniform (rns/rns_polynomial.h:640-667): limb by limb,
+ * coefficient by coefficient, from one PRNG. */
+void so_sample_uniform_chacha(so_prime* const* primes, int nl, so_chacha_prng* g, uint64_t* x) {
+  for (int i = 0; i < nl; i++) {
+    const size_t n = (size_t)1 << primes[i]->log_n;
+    for (size_t j = 0; j < n; j++) x[i * n + j] = so_import_random_chacha(&primes[i]->mp, g);
+  }
+}

@@ -14,6 +14,7 @@
 #ifndef SHELL_ORACLE_H_
 #define SHELL_ORACLE_H_
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -131,6 +132,34 @@ void so_fill_uniform(so_prime* const* primes, int nl, uint64_t seed,
 
 /* Timed CPU baseline helpers: run a relinearization of `count` ciphertexts. */
 double so_now_seconds(void);
+
+/* ---- wire formats and the ChaCha PRNG ---------------------------------- */
+/* TranscribeBits (transcription.h:60-150) over integers held in uint64. */
+long long so_transcribe_bits(const uint64_t* in, size_t in_len, int in_type_bits,
+                             long long input_bit_length, int input_bits_per_int,
+                             int output_bits_per_int, int out_type_bits, uint64_t* out,
+                             size_t out_cap);
+/* MontgomeryInt::SerializeVector / DeserializeVector (montgomery.cc:143-219). */
+long long so_serialize_vector(const so_mont* p, const uint64_t* coeffs, size_t count,
+                              uint8_t* out, size_t out_cap);
+int so_deserialize_vector(const so_mont* p, int num_coeffs, const uint8_t* in, size_t in_len,
+                          uint64_t* coeffs);
+/* RFC 7539 ChaCha20 block; the reference's SingleThreadChaChaPrng
+ * (prng/single_thread_chacha_prng.cc, prng/chacha_prng_util.cc). */
+#define SO_CHACHA_OUTPUT_BYTES (255 * 32)
+typedef struct so_chacha_prng {
+  uint8_t key[32];
+  int position;
+  int salt_counter;
+  uint8_t buffer[SO_CHACHA_OUTPUT_BYTES];
+} so_chacha_prng;
+void so_chacha20_block(const uint8_t key[32], uint32_t counter, const uint8_t nonce[12],
+                       uint8_t out[64]);
+int so_chacha_prng_init(so_chacha_prng* g, const uint8_t* key, size_t key_len);
+uint8_t so_chacha_rand8(so_chacha_prng* g);
+uint64_t so_chacha_rand64(so_chacha_prng* g);
+uint64_t so_import_random_chacha(const so_mont* p, so_chacha_prng* g);
+void so_sample_uniform_chacha(so_prime* const* primes, int nl, so_chacha_prng* g, uint64_t* x);
 
 #ifdef __cplusplus
 }

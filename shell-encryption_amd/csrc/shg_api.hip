@@ -1380,6 +1380,101 @@ int shg_lazy_fma(shg_context* ctx, const uint64_t* as, size_t as_stride, const u
   return e == hipSuccess ? SHG_OK : hip_fail(e, "lazy fma launch");
 }
 
+// The serialized layout of one polynomial over `primes`
+// (MontgomeryInt::SerializeVector per limb, montgomery.cc:143-167).
+static int wire_desc(shg_context* ctx, const int* primes, int num_limbs, shg::WireDesc* wd) {
+  LimbPrimes lp{};
+  int log_n = 0;
+  int rc = check_primes(ctx, primes, num_limbs, &log_n, &lp);
+  if (rc) return rc;
+  const long long n = 1LL << log_n;
+  if (n > (1LL << 15))
+    return fail(SHG_E_INVALID_ARGUMENT,
+                "Number of coefficients, %lld, cannot be larger than 32768.", n);
+  wd->num_limbs = num_limbs;
+  wd->log_n = log_n;
+  wd->aligned = 1;
+  unsigned long long off = 0;
+  for (int i = 0; i < num_limbs; ++i) {
+    const int b = (int)ctx->primes[primes[i]]->mp->log_modulus;
+    wd->bits[i] = (unsigned char)b;
+    wd->limb_off[i] = off;
+    const unsigned long long bytes = ((unsigned long long)n * b + 7) / 8;
+    if (bytes % 8) wd->aligned = 0;
+    off += bytes;
+  }
+  wd->limb_off[num_limbs] = off;
+  return SHG_OK;
+}
+
+int shg_serialized_bytes(shg_context* ctx, const int* primes, int num_limbs, size_t* limb_bytes,
+                         size_t* poly_bytes) {
+  shg::WireDesc wd{};
+  int rc = wire_desc(ctx, primes, num_limbs, &wd);
+  if (rc) return rc;
+  if (limb_bytes)
+    for (int i = 0; i < num_limbs; ++i) limb_bytes[i] = wd.limb_off[i + 1] - wd.limb_off[i];
+  if (poly_bytes) *poly_bytes = wd.limb_off[num_limbs];
+  return SHG_OK;
+}
+
+int shg_serialize_polys(shg_context* ctx, const uint64_t* polys, int count, int num_limbs,
+                        const int* primes, uint8_t* out) {
+  shg::WireDesc wd{};
+  int rc = wire_desc(ctx, primes, num_limbs, &wd);
+  if (rc) return rc;
+  if (count < 0) return fail(SHG_E_INVALID_ARGUMENT, "count must be non-negative.");
+  if (count == 0) return SHG_OK;
+  if (!aligned_ptr(polys) || !out) return fail(SHG_E_INVALID_ARGUMENT, "data must not be null.");
+  if ((uintptr_t)out % 8) wd.aligned = 0;
+  hipError_t e = shg::launch_pack(polys, out, wd, count, ctx->stream);
+  return e == hipSuccess ? SHG_OK : hip_fail(e, "serialize launch");
+}
+
+int shg_deserialize_polys(shg_context* ctx, const uint8_t* in, int count, int num_limbs,
+                          const int* primes, uint64_t* polys) {
+  shg::WireDesc wd{};
+  int rc = wire_desc(ctx, primes, num_limbs, &wd);
+  if (rc) return rc;
+  if (count < 0) return fail(SHG_E_INVALID_ARGUMENT, "count must be non-negative.");
+  if (count == 0) return SHG_OK;
+  if (!aligned_ptr(polys) || !in) return fail(SHG_E_INVALID_ARGUMENT, "data must not be null.");
+  if ((uintptr_t)in % 8) wd.aligned = 0;
+  hipError_t e = shg::launch_unpack(in, polys, wd, count, ctx->stream);
+  return e == hipSuccess ? SHG_OK : hip_fail(e, "deserialize launch");
+}
+
+int shg_sample_uniform_chacha(shg_context* ctx, const uint8_t* host_seed, size_t seed_len,
+                              int count, int num_limbs, const int* primes, uint64_t* out) {
+  LimbPrimes lp{};
+  int log_n = 0;
+  int rc = check_primes(ctx, primes, num_limbs, &log_n, &lp);
+  if (rc) return rc;
+  if (!host_seed || seed_len != 32)
+    return fail(SHG_E_INVALID_ARGUMENT,
+                "Cannot create Prng with key of the wrong size. Real key length of %zu instead "
+                "of expected key length of 32.", host_seed ? seed_len : (size_t)0);
+  if (count < 0) return fail(SHG_E_INVALID_ARGUMENT, "count must be non-negative.");
+  if (count == 0) return SHG_OK;
+  if (!aligned_ptr(out)) return fail(SHG_E_INVALID_ARGUMENT, "out must not be null.");
+  shg::PadDesc pd{};
+  for (int i = 0; i < 8; ++i)
+    pd.key[i] = (uint32_t)host_seed[4 * i] | (uint32_t)host_seed[4 * i + 1] << 8 |
+                (uint32_t)host_seed[4 * i + 2] << 16 | (uint32_t)host_seed[4 * i + 3] << 24;
+  pd.count = count;
+  pd.num_limbs = num_limbs;
+  pd.log_n = log_n;
+  for (int i = 0; i < num_limbs; ++i) {
+    const Params* P = ctx->primes[primes[i]]->mp.get();
+    if (P->log_modulus <= 8)
+      return fail(SHG_E_UNSUPPORTED, "moduli of 8 bits or fewer are not supported here.");
+    pd.bits[i] = (unsigned char)P->log_modulus;
+    pd.q[i] = P->modulus;
+  }
+  hipError_t e = shg::launch_chacha_pad(pd, out, ctx->stream);
+  return e == hipSuccess ? SHG_OK : hip_fail(e, "sample launch");
+}
+
 int shg_profile_enable(shg_context* ctx, int enable) {
   if (!ctx) return fail(SHG_E_INVALID_ARGUMENT, "context must not be null.");
   if (capturing(ctx)) return refuse_capture(ctx, "shg_profile_enable");

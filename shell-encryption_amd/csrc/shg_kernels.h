@@ -212,6 +212,36 @@ hipError_t launch_batch_limbs(const LimbJobDev* jobs, int num_jobs, int op, int 
 hipError_t launch_ntt15(bool inverse, uint64_t* data, const PrimeDev* primes,
                         const LimbPrimes& lp, int rows, int num_limbs, hipStream_t s);
 
+// Wire formats (shg_wire.hip).  A polynomial's serialized form is its limbs'
+// byte strings back to back: limb i holds n residues of bits[i] = log2(q_i)
+// bits each, little-endian bit stream, at byte limb_off[i] of the
+// polynomial; limb_off[num_limbs] is the polynomial's size.  `aligned`: every
+// limb string is a multiple of 8 bytes long and the buffer is 8-byte
+// aligned (64-bit loads and stores).
+struct WireDesc {
+  int num_limbs;
+  int log_n;
+  int aligned;
+  unsigned char bits[SHG_MAX_LIMBS];
+  unsigned long long limb_off[SHG_MAX_LIMBS + 1];
+};
+hipError_t launch_pack(const uint64_t* polys, uint8_t* out, const WireDesc& wd, int count,
+                       hipStream_t s);
+hipError_t launch_unpack(const uint8_t* in, uint64_t* polys, const WireDesc& wd, int count,
+                         hipStream_t s);
+// RnsPolynomial::SampleUniform of `count` polynomials from one
+// SingleThreadChaChaPrng stream (key = the 32-byte seed as 8 little-endian
+// words); every modulus has 9..62 bits.
+struct PadDesc {
+  uint32_t key[8];
+  int count;
+  int num_limbs;
+  int log_n;
+  unsigned char bits[SHG_MAX_LIMBS];
+  uint64_t q[SHG_MAX_LIMBS];
+};
+hipError_t launch_chacha_pad(const PadDesc& pd, uint64_t* out, hipStream_t s);
+
 // Oblivious expansion and the inner product against a database (shg_pir.hip).
 hipError_t launch_expand_substitute(const uint64_t* const* cts, int count, int m, uint64_t* S,
                                     int power, int logn, int num_limbs, hipStream_t s);

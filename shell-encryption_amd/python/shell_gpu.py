@@ -106,6 +106,10 @@ def lib():
                                      ctypes.POINTER(ctypes.c_double),
                                      ctypes.POINTER(ctypes.c_double)],
         "shg_pir_server_destroy": [VP],
+        "shg_serialized_bytes": [VP, INTP, i, ctypes.POINTER(sz), ctypes.POINTER(sz)],
+        "shg_serialize_polys": [VP, VP, i, i, INTP, VP],
+        "shg_deserialize_polys": [VP, VP, i, i, INTP, VP],
+        "shg_sample_uniform_chacha": [VP, ctypes.c_char_p, sz, i, i, INTP, VP],
         "shg_graph_begin": [VP],
         "shg_graph_end": [VP, ctypes.POINTER(VP)],
         "shg_graph_launch": [VP, VP],
@@ -354,6 +358,29 @@ class Context:
         _check(lib().shg_pir_inner_product(self.h, self._ptrs(expanded), len(expanded), int(levels),
                                            db_x.p, int(num_columns), self._ptrs(outs), len(primes),
                                            _ints(primes)))
+
+    def serialized_bytes(self, primes):
+        """(per-limb byte lengths, polynomial byte length) of the wire form."""
+        limbs = (ctypes.c_size_t * len(primes))()
+        total = ctypes.c_size_t()
+        _check(lib().shg_serialized_bytes(self.h, _ints(primes), len(primes), limbs,
+                                          ctypes.byref(total)))
+        return list(limbs), total.value
+
+    def serialize(self, polys, count, primes, out, offset_bytes=0):
+        """Packs count polynomials into the device byte buffer `out`
+        (a DeviceArray, used as bytes from offset_bytes)."""
+        _check(lib().shg_serialize_polys(self.h, polys.p, int(count), len(primes), _ints(primes),
+                                         VP(out.ptr + int(offset_bytes))))
+
+    def deserialize(self, src, count, primes, polys, offset_bytes=0):
+        _check(lib().shg_deserialize_polys(self.h, VP(src.ptr + int(offset_bytes)), int(count),
+                                           len(primes), _ints(primes), polys.p))
+
+    def sample_uniform_chacha(self, seed, count, primes, out):
+        seed = bytes(seed)
+        _check(lib().shg_sample_uniform_chacha(self.h, seed, len(seed), int(count), len(primes),
+                                               _ints(primes), out.p))
 
     def graph_begin(self):
         _check(lib().shg_graph_begin(self.h))
