@@ -65,6 +65,32 @@ class RnsRlweCiphertext {
                              error_params_);
   }
 
+  // rns/rns_ciphertext.h:70-98: each component through
+  // RnsPolynomial::Serialize / Deserialize (bit-packed on the device).
+  static StatusOr<RnsRlweCiphertext> Deserialize(const SerializedRnsRlweCiphertext& serialized,
+                                                 ModuliVec moduli,
+                                                 const RnsErrorParams<ModularInt>* error_params) {
+    if (error_params == nullptr) return InvalidArgumentError("`error_params` must not be null.");
+    std::vector<RnsPolynomial<ModularInt>> components;
+    for (int i = 0; i < serialized.components_size(); ++i) {
+      RLWE_ASSIGN_OR_RETURN(auto c, RnsPolynomial<ModularInt>::Deserialize(
+                                        serialized.components(i),
+                                        ModuliSpan<ModularInt>(moduli.data(), moduli.size())));
+      components.push_back(std::move(c));
+    }
+    return RnsRlweCiphertext(std::move(components), std::move(moduli), serialized.power_of_s(),
+                             serialized.error(), error_params);
+  }
+  StatusOr<SerializedRnsRlweCiphertext> Serialize() const {
+    SerializedRnsRlweCiphertext serialized;
+    for (const auto& c : components_) {
+      RLWE_ASSIGN_OR_RETURN(*serialized.add_components(), c.Serialize(Moduli()));
+    }
+    serialized.set_power_of_s(power_of_s_);
+    serialized.set_error(error_);
+    return serialized;
+  }
+
   Status ConvertToNttForm() {
     for (auto& c : components_) RLWE_RETURN_IF_ERROR(c.ConvertToNttForm(moduli_));
     return OkStatus();
@@ -111,12 +137,20 @@ class RnsBgvCiphertext : public RnsRlweCiphertext<ModularInt> {
   using Base::Base;
   explicit RnsBgvCiphertext(Base ct) : Base(std::move(ct)) {}
 
+  static StatusOr<RnsBgvCiphertext> Deserialize(const SerializedRnsRlweCiphertext& serialized,
+                                                typename Base::ModuliVec moduli,
+                                                const RnsErrorParams<ModularInt>* error_params) {
+    RLWE_ASSIGN_OR_RETURN(Base ct, Base::Deserialize(serialized, std::move(moduli), error_params));
+    return RnsBgvCiphertext(std::move(ct));
+  }
+
   StatusOr<RnsBgvCiphertext> operator*(const RnsBgvCiphertext& that) const {
     RLWE_ASSIGN_OR_RETURN(Base product, this->Mul(that));
     return RnsBgvCiphertext(std::move(product));
   }
 
-  // rns/rns_bgv_ciphertext.h:146-157: all components in one launch.
+  // rns/rns_bgv_ciphertext.h:146-157: one fused ModReduceLsb launch per
+  // component.
   Status ModReduce(typename ModularInt::Int t, const RnsInt<ModularInt>& ql_inv) {
     if (this->moduli().size() <= 1)
       return FailedPreconditionError(

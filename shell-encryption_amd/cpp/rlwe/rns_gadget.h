@@ -94,13 +94,17 @@ class RnsGadget {
 template <typename ModularInt>
 class RnsErrorParams {
  public:
+  // rns/rns_error_params.h:43-72.  The auxiliary moduli (BFV basis
+  // extension) only enter bounds this path does not use.
   static StatusOr<RnsErrorParams> Create(int log_n, ModuliSpan<ModularInt> main_moduli,
+                                         ModuliSpan<ModularInt> aux_moduli,
                                          int log_plaintext_modulus, double sigma) {
     if (log_n <= 0) return InvalidArgumentError("`log_n` must be positive.");
     if (main_moduli.empty()) return InvalidArgumentError("`main_moduli` must not be empty.");
     if (log_plaintext_modulus <= 0)
       return InvalidArgumentError("`log_plaintext_modulus` must be positive.");
     if (sigma <= 0) return InvalidArgumentError("`sigma` must be positive.");
+    (void)aux_moduli;
     return RnsErrorParams(log_n, log_plaintext_modulus, sigma);
   }
   double BoundOnGadgetBasedKeySwitching(int num_components, int log_gadget_base,

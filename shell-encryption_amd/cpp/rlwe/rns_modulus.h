@@ -67,6 +67,25 @@ inline std::vector<int> DevicePrimes(ModuliSpan<ModularInt> moduli) {
   return ids;
 }
 
+// Device prime handles of a moduli span for ring degree 2^log_n: the
+// moduli's own handles, or handles registered for that degree.
+template <typename ModularInt>
+inline StatusOr<std::vector<int>> DevicePrimesAt(ModuliSpan<ModularInt> moduli, int log_n) {
+  std::vector<int> ids;
+  ids.reserve(moduli.size());
+  for (size_t i = 0; i < moduli.size(); ++i) {
+    if (moduli[i]->log_n == log_n) {
+      ids.push_back(moduli[i]->device_prime);
+      continue;
+    }
+    int id = -1;
+    RLWE_RETURN_IF_ERROR(
+        device::FromCode(shg_prime_create(device::Context(), moduli[i]->Modulus(), log_n, &id)));
+    ids.push_back(id);
+  }
+  return ids;
+}
+
 }  // namespace rlwe
 
 #endif  // SHG_RLWE_RNS_MODULUS_H_

@@ -11,7 +11,9 @@
 #include <vector>
 
 #include "rlwe/device.h"
+#include "rlwe/prng.h"
 #include "rlwe/rns_modulus.h"
+#include "rlwe/serialization.h"
 
 namespace rlwe {
 
@@ -63,6 +65,147 @@ class RnsPolynomial {
       for (size_t j = 0; j < n; ++j) flat[i * n + j] = moduli[i]->ModParams()->r_mod_modulus;
     RLWE_RETURN_IF_ERROR(p.buf_->Upload(flat.data(), flat.size()));
     return p;
+  }
+
+  // rns/rns_polynomial.h:640-667: ImportRandom (montgomery.h:260-273) for
+  // every coefficient, limb by limb, from any SecurePrng: log2(q) random bits
+  // per draw (a 64-bit draw, or one byte for moduli of 8 bits or fewer),
+  // redrawn while >= q, stored as the representation.  The draws are made on
+  // the host; RnsGaloisKey's seeded pads use shg_sample_uniform_chacha.
+  template <typename Prng = SecurePrng>
+  static StatusOr<RnsPolynomial> SampleUniform(int log_n, Prng* prng, Moduli moduli) {
+    if (log_n <= 0) return InvalidArgumentError("`log_n` must be positive.");
+    if (prng == nullptr) return InvalidArgumentError("`prng` must not be null.");
+    if (moduli.empty()) return InvalidArgumentError("`moduli` must not be empty.");
+    const size_t n = size_t{1} << log_n;
+    RnsPolynomial p(log_n, static_cast<int>(moduli.size()), true);
+    std::vector<uint64_t> flat(p.Words());
+    for (size_t i = 0; i < moduli.size(); ++i) {
+      const auto* mp = moduli[i]->ModParams();
+      const int bits = (int)mp->log_modulus;
+      for (size_t j = 0; j < n; ++j) {
+        uint64_t r;
+        do {
+          r = 0;
+          for (int need = bits; need > 0;) {
+            if (need <= 8) {
+              RLWE_ASSIGN_OR_RETURN(uint8_t v, prng->Rand8());
+              r = (r << need) + (v & ((1u << need) - 1));
+              break;
+            }
+            RLWE_ASSIGN_OR_RETURN(uint64_t v, prng->Rand64());
+            const int take = need < 64 ? need : 64;
+            r = (take == 64 ? 0 : (r << take)) + (take == 64 ? v : (v & ((1ull << take) - 1)));
+            need -= take;
+          }
+        } while (r >= mp->modulus);
+        flat[i * n + j] = r;
+      }
+    }
+    RLWE_RETURN_IF_ERROR(p.buf_->Upload(flat.data(), flat.size()));
+    return p;
+  }
+
+  // rns/rns_polynomial.cc:140-169: coefficients mod q in balanced form,
+  // lifted to every modulus (ConvertModulusBalanced) and transformed.
+  static StatusOr<RnsPolynomial> ConvertBalancedFromPolynomialCoeffs(
+      const std::vector<ModularInt>& coeffs_q, const ModularIntParams* mod_params_q,
+      Moduli moduli) {
+    const size_t num_coeffs = coeffs_q.size();
+    if (num_coeffs == 0 || (num_coeffs & (num_coeffs - 1)) != 0)
+      return InvalidArgumentError("`coeffs_q` must have length a power of 2.");
+    if (mod_params_q == nullptr) return InvalidArgumentError("`mod_params_q` must not be null.");
+    if (moduli.empty()) return InvalidArgumentError("`moduli` must not be empty.");
+    int log_n = 0;
+    while ((size_t{1} << log_n) < num_coeffs) ++log_n;
+    RnsPolynomial p(log_n, static_cast<int>(moduli.size()), false);
+    std::vector<uint64_t> flat(p.Words());
+    const uint64_t q = mod_params_q->modulus;
+    for (size_t i = 0; i < moduli.size(); ++i) {
+      const auto* mp = moduli[i]->ModParams();
+      const uint64_t pi = mp->modulus;
+      const uint64_t diff = q > pi ? q - pi : pi - q;
+      RLWE_ASSIGN_OR_RETURN(ModularInt diff_p, ModularInt::ImportInt(diff, mp));
+      for (size_t j = 0; j < num_coeffs; ++j) {
+        if (pi == q) {
+          flat[i * num_coeffs + j] = coeffs_q[j].GetMontgomeryRepresentation();
+          continue;
+        }
+        const uint64_t x = coeffs_q[j].ExportInt(mod_params_q);
+        RLWE_ASSIGN_OR_RETURN(ModularInt xp, ModularInt::ImportInt(x, mp));
+        if (x > (q >> 1)) {
+          if (pi > q) xp.AddInPlace(diff_p, mp); else xp.SubInPlace(diff_p, mp);
+        }
+        flat[i * num_coeffs + j] = xp.GetMontgomeryRepresentation();
+      }
+    }
+    RLWE_RETURN_IF_ERROR(p.buf_->Upload(flat.data(), flat.size()));
+    RLWE_RETURN_IF_ERROR(p.ConvertToNttForm(moduli));
+    return p;
+  }
+
+  // rns/rns_polynomial.h:99-122: per limb MontgomeryInt::DeserializeVector
+  // (montgomery.cc:184-219), unpacked on the device.
+  static StatusOr<RnsPolynomial> Deserialize(const SerializedRnsPolynomial& serialized,
+                                             Moduli moduli) {
+    const int log_n = serialized.log_n();
+    if (log_n <= 0) return InvalidArgumentError("`log_n` must be positive.");
+    const int L = serialized.coeff_vectors_size();
+    if (L != (int)moduli.size())
+      return InvalidArgumentError("Number of serialized coefficient vectors must be " +
+                                  std::to_string(moduli.size()));
+    if (log_n > 15)
+      return InvalidArgumentError("Number of coefficients, " + std::to_string(1LL << std::min(log_n, 62)) +
+                                  ", cannot be larger than 32768.");
+    RLWE_ASSIGN_OR_RETURN(std::vector<int> ids, DevicePrimesAt(moduli, log_n));
+    std::vector<size_t> limb_bytes(L);
+    size_t poly_bytes = 0;
+    RLWE_RETURN_IF_ERROR(device::FromCode(shg_serialized_bytes(
+        device::Context(), ids.data(), L, limb_bytes.data(), &poly_bytes)));
+    std::vector<uint64_t> staged((poly_bytes + 7) / 8 + 1, 0);
+    char* dst = reinterpret_cast<char*>(staged.data());
+    for (int i = 0; i < L; ++i) {
+      const std::string& v = serialized.coeff_vectors(i);
+      const long long bits = (long long)moduli[i]->ModParams()->log_modulus << log_n;
+      if (v.size() < limb_bytes[i])
+        return InvalidArgumentError("The input vector of size " + std::to_string(v.size()) +
+                                    " is too small to contain " + std::to_string(bits) + " bits.");
+      std::memcpy(dst, v.data(), limb_bytes[i]);
+      dst += limb_bytes[i];
+    }
+    device::Buffer bytes(staged.size());
+    RLWE_RETURN_IF_ERROR(bytes.Upload(staged.data(), staged.size()));
+    RnsPolynomial p(log_n, L, serialized.is_ntt());
+    RLWE_RETURN_IF_ERROR(device::FromCode(shg_deserialize_polys(
+        device::Context(), reinterpret_cast<const uint8_t*>(bytes.data()), 1, L, ids.data(),
+        p.buf_->data())));
+    return p;
+  }
+
+  // rns/rns_polynomial.h:124-140: per limb MontgomeryInt::SerializeVector,
+  // packed on the device.
+  StatusOr<SerializedRnsPolynomial> Serialize(Moduli moduli) const {
+    RLWE_RETURN_IF_ERROR(CheckModuli(moduli));
+    RLWE_ASSIGN_OR_RETURN(std::vector<int> ids, DevicePrimesAt(moduli, log_n_));
+    std::vector<size_t> limb_bytes(num_moduli_);
+    size_t poly_bytes = 0;
+    RLWE_RETURN_IF_ERROR(device::FromCode(shg_serialized_bytes(
+        device::Context(), ids.data(), num_moduli_, limb_bytes.data(), &poly_bytes)));
+    device::Buffer bytes((poly_bytes + 7) / 8);
+    RLWE_RETURN_IF_ERROR(device::FromCode(shg_serialize_polys(
+        device::Context(), buf_->data(), 1, num_moduli_, ids.data(),
+        reinterpret_cast<uint8_t*>(bytes.data()))));
+    std::vector<uint64_t> host(bytes.words());
+    RLWE_RETURN_IF_ERROR(bytes.Download(host.data(), host.size()));
+    const char* src = reinterpret_cast<const char*>(host.data());
+    SerializedRnsPolynomial out;
+    out.set_log_n(log_n_);
+    out.set_is_ntt(is_ntt_);
+    for (int i = 0; i < num_moduli_; ++i) {
+      out.add_coeff_vectors()->assign(src, limb_bytes[i]);
+      src += limb_bytes[i];
+    }
+    return out;
   }
 
   // Wraps an existing device buffer region (used by batched containers).

@@ -12,6 +12,7 @@
 
 #include "rlwe/rns_context.h"
 #include "rlwe/rns_relinearization_key.h"
+#include "rlwe/rns_secret_key.h"
 #include "shell_oracle.h"
 
 using ModularInt = rlwe::MontgomeryInt<uint64_t>;
@@ -271,7 +272,7 @@ static void TestKeySwitching(int log_n, int L, int batch) {
       log_n, log_bs, ctx.MainPrimeModulusComplements(L - 1).value(),
       ctx.MainPrimeModulusCrtFactors(L - 1).value(), moduli).value();
   const int k = gadget.Dimension();
-  auto error_params = rlwe::RnsErrorParams<ModularInt>::Create(log_n, moduli, 17, 4.0).value();
+  auto error_params = rlwe::RnsErrorParams<ModularInt>::Create(log_n, moduli, {}, 17, 4.0).value();
   std::vector<uint64_t> kas((size_t)k * L * n), kbs((size_t)k * L * n);
   std::vector<Poly> pa, pb;
   for (int j = 0; j < k; ++j) {
@@ -342,6 +343,192 @@ static void TestKeySwitching(int log_n, int L, int batch) {
   std::printf("  key switching at n=2^%d, L=%d, k=%d, batch %d: ok\n", log_n, L, k, batch);
 }
 
+// rns/rns_polynomial_test.cc (Serialize/Deserialize), rns/rns_ciphertext_test.cc:
+// the bytes are the oracle's SerializeVector per limb, and the messages
+// survive their binary encoding.
+static void TestSerialization() {
+  const int log_n = 12, L = 3, n = 1 << log_n;
+  auto qs = NttPrimes(log_n, L);
+  auto ctx = rlwe::RnsContext<ModularInt>::CreateForBgv(log_n, qs, 65537).value();
+  auto moduli = ctx.MainPrimeModuli();
+  OracleChain oc(qs, log_n);
+  auto x = Uniform(oc, L, 77);
+  x[5] = qs[0];  // NegateInPlace(0) = q fits the field width
+  auto p = FromFlat(x, L, n, true);
+  ASSIGN_OR_DIE(ser, p.Serialize(moduli));
+  CHECK(ser.log_n() == log_n && ser.is_ntt() && ser.coeff_vectors_size() == L);
+  for (int i = 0; i < L; ++i) {
+    std::vector<uint8_t> ref(((size_t)n * 50 + 7) / 8 + 8);
+    const long long nb = so_serialize_vector(&oc.primes[i]->mp, x.data() + (size_t)i * n, n,
+                                             ref.data(), ref.size());
+    CHECK(nb == (long long)ser.coeff_vectors(i).size());
+    CHECK(std::memcmp(ref.data(), ser.coeff_vectors(i).data(), (size_t)nb) == 0);
+  }
+  rlwe::SerializedRnsPolynomial parsed;
+  CHECK(parsed.ParseFromString(ser.SerializeAsString()));
+  ASSIGN_OR_DIE(back, Poly::Deserialize(parsed, moduli));
+  CHECK(back == p);
+  // Reference error messages.
+  auto bad = Poly::Deserialize(parsed, rlwe::Span<const rlwe::PrimeModulus<ModularInt>* const>(
+                                           moduli.data(), 2));
+  CHECK(!bad.ok() && bad.status().message().find("coefficient vectors must be 2") != std::string::npos);
+  rlwe::SerializedRnsPolynomial shortv;
+  shortv.set_log_n(log_n);
+  for (int i = 0; i < L; ++i) *shortv.add_coeff_vectors() = ser.coeff_vectors(i).substr(0, 10);
+  auto bad2 = Poly::Deserialize(shortv, moduli);
+  CHECK(!bad2.ok() && bad2.status().message().find("too small") != std::string::npos);
+  CHECK(!p.Serialize(rlwe::Span<const rlwe::PrimeModulus<ModularInt>* const>(moduli.data(), 2)).ok());
+  // Ciphertext round trip through its encoding.
+  auto error_params = rlwe::RnsErrorParams<ModularInt>::Create(log_n, moduli, {}, 17, 4.0).value();
+  std::vector<Poly> comps = {p, FromFlat(Uniform(oc, L, 78), L, n, true)};
+  Ct ct(std::move(comps), moduli, 5, 123.5, &error_params);
+  ASSIGN_OR_DIE(sct, ct.Serialize());
+  rlwe::SerializedRnsRlweCiphertext sct2;
+  CHECK(sct2.ParseFromString(sct.SerializeAsString()));
+  ASSIGN_OR_DIE(ct2, Ct::Deserialize(sct2, moduli, &error_params));
+  CHECK(ct2.PowerOfS() == 5 && ct2.Error() == 123.5 && ct2.Len() == 2);
+  CHECK(ct2.Components()[0] == ct.Components()[0] && ct2.Components()[1] == ct.Components()[1]);
+  CHECK(!Ct::Deserialize(sct2, moduli, nullptr).ok());
+  std::printf("  serialization: ok\n");
+}
+
+// The key relation of a fresh key column: b + a*s - target = t*e with e a
+// centered binomial of the given variance, the same small integer on every
+// limb (rns/rns_relinearization_key.cc:160-170).
+static void CheckKeyColumn(const Poly& a, const Poly& b, const Poly& s, const Poly& target,
+                           const std::vector<const rlwe::PrimeModulus<ModularInt>*>& moduli,
+                           uint64_t t, int variance) {
+  Poly r = b;
+  CHECK_OK(r.FusedMulAddInPlace(a, s, moduli));
+  CHECK_OK(r.SubInPlace(target, moduli));
+  CHECK_OK(r.ConvertToCoeffForm(moduli));
+  const auto& cv = r.Coeffs();
+  const int n = r.NumCoeffs();
+  for (int j = 0; j < n; ++j) {
+    long long e0 = 0;
+    for (size_t i = 0; i < moduli.size(); ++i) {
+      const auto* mp = moduli[i]->ModParams();
+      const uint64_t v = cv[i][j].ExportInt(mp);
+      const long long c = v > mp->modulus / 2 ? -(long long)(mp->modulus - v) : (long long)v;
+      if (i == 0) e0 = c;
+      CHECK(c == e0);
+    }
+    CHECK(e0 % (long long)t == 0);
+    const long long e = e0 / (long long)t;
+    CHECK(e >= -2 * variance && e <= 2 * variance);
+  }
+}
+
+// Key generation (rns/rns_relinearization_key_test.cc, rns/rns_galois_key_test.cc):
+// the pads are the oracle's SampleUniform from the key's seed, every column
+// satisfies the key relation, a Galois key survives Serialize/Deserialize
+// (key_a regenerated from the seed on the device), and Compose equals the
+// oracle's Galois key switch of the substituted columns.
+static void TestKeyGeneration(rlwe::PrngType prng_type) {
+  const int log_n = 10, L = 3, n = 1 << log_n, variance = 8;
+  const uint64_t t = 65537;
+  auto qs = NttPrimes(log_n, L);
+  auto ctx = rlwe::RnsContext<ModularInt>::CreateForBgv(log_n, qs, t).value();
+  auto moduli = ctx.MainPrimeModuli();
+  OracleChain oc(qs, log_n);
+  std::vector<size_t> log_bs(L, 10);
+  auto gadget = rlwe::RnsGadget<ModularInt>::Create(
+      log_n, log_bs, ctx.MainPrimeModulusComplements(L - 1).value(),
+      ctx.MainPrimeModulusCrtFactors(L - 1).value(), moduli).value();
+  const int k = gadget.Dimension();
+  auto prng = rlwe::SingleThreadChaChaPrng::Create(std::string(32, '\x07')).value();
+  ASSIGN_OR_DIE(sk, rlwe::RnsRlweSecretKey<ModularInt>::Sample(log_n, variance, moduli, prng.get()));
+  const Poly& s = sk.Key();
+  // The secret itself is a centered binomial: s - 0 = 1 * e.
+  ASSIGN_OR_DIE(zero, Poly::CreateZero(log_n, moduli));
+  CheckKeyColumn(zero, s, zero, zero, moduli, 1, variance);
+
+  ASSIGN_OR_DIE(relin, rlwe::RnsRelinKey<ModularInt>::CreateForBgv(sk, 2, variance, &gadget, t,
+                                                                   prng_type));
+  CHECK((int)relin.GetKeyA().size() == k && relin.GetPrngType() == prng_type);
+  ASSIGN_OR_DIE(s2, s.Mul(s, moduli));
+  for (int j = 0; j < k; ++j) {
+    Poly target = s2;
+    CHECK_OK(target.MulInPlace(gadget.Component(j), moduli));
+    CheckKeyColumn(relin.GetKeyA()[j], relin.GetKeyB()[j], s, target, moduli, t, variance);
+  }
+  if (prng_type == rlwe::PRNG_TYPE_CHACHA) {
+    so_chacha_prng* g = (so_chacha_prng*)std::malloc(sizeof(so_chacha_prng));
+    CHECK(so_chacha_prng_init(g, (const uint8_t*)relin.PrngSeed().data(), 32) == 0);
+    std::vector<uint64_t> a((size_t)L * n);
+    for (int j = 0; j < k; ++j) {
+      so_sample_uniform_chacha(oc.pp(), L, g, a.data());
+      CHECK(Flat(relin.GetKeyA()[j]) == a);
+    }
+    std::free(g);
+  } else {
+    auto hp = rlwe::SingleThreadHkdfPrng::Create(relin.PrngSeed()).value();
+    for (int j = 0; j < k; ++j) {
+      ASSIGN_OR_DIE(a, Poly::SampleUniform(log_n, hp.get(), moduli));
+      CHECK(a == relin.GetKeyA()[j]);
+    }
+  }
+
+  ASSIGN_OR_DIE(g5, rlwe::RnsGaloisKey<ModularInt>::CreateForBgv(sk, 5, variance, &gadget, t,
+                                                                 prng_type));
+  ASSIGN_OR_DIE(s5, s.Substitute(5, moduli));
+  for (int j = 0; j < k; ++j) {
+    Poly target = s5;
+    CHECK_OK(target.MulInPlace(gadget.Component(j), moduli));
+    CheckKeyColumn(g5.GetKeyA()[j], g5.GetKeyB()[j], s, target, moduli, t, variance);
+  }
+  CHECK(g5.Dimension() == k && g5.SubstitutionPower() == 5);
+  ASSIGN_OR_DIE(sg, g5.Serialize());
+  rlwe::SerializedRnsGaloisKey sg2;
+  CHECK(sg2.ParseFromString(sg.SerializeAsString()));
+  ASSIGN_OR_DIE(g5b, rlwe::RnsGaloisKey<ModularInt>::Deserialize(sg2, &gadget, moduli));
+  CHECK(g5b.SubstitutionPower() == 5 && g5b.Dimension() == k);
+  for (int j = 0; j < k; ++j) {
+    CHECK(g5b.GetKeyA()[j] == g5.GetKeyA()[j]);
+    CHECK(g5b.GetKeyB()[j] == g5.GetKeyB()[j]);
+  }
+
+  // Compose: key for s(X^(5*7)) from keys for s(X^5) and s(X^7).
+  ASSIGN_OR_DIE(g7, rlwe::RnsGaloisKey<ModularInt>::CreateForBgv(sk, 7, variance, &gadget, t,
+                                                                 prng_type));
+  ASSIGN_OR_DIE(g35, g5.Compose(g7));
+  CHECK(g35.SubstitutionPower() == 35 && g35.Dimension() == k);
+  std::vector<uint64_t> ka((size_t)k * L * n), kb((size_t)k * L * n);
+  for (int j = 0; j < k; ++j) {
+    auto fa = Flat(g5.GetKeyA()[j]), fb = Flat(g5.GetKeyB()[j]);
+    std::copy(fa.begin(), fa.end(), ka.begin() + (size_t)j * L * n);
+    std::copy(fb.begin(), fb.end(), kb.begin() + (size_t)j * L * n);
+  }
+  int lb[64];
+  for (int i = 0; i < L; ++i) lb[i] = 10;
+  std::vector<uint64_t> sa((size_t)L * n), sb((size_t)L * n), e0((size_t)L * n), e1((size_t)L * n);
+  for (int i = 0; i < k; ++i) {
+    auto fa = Flat(g7.GetKeyA()[i]), fb = Flat(g7.GetKeyB()[i]);
+    so_poly_substitute(oc.pp(), L, 5, fa.data(), sa.data());
+    so_poly_substitute(oc.pp(), L, 5, fb.data(), sb.data());
+    CHECK(so_galois_apply(oc.pp(), L, lb, sb.data(), sa.data(), 1, ka.data(), kb.data(), e0.data(),
+                          e1.data()) == 0);
+    CHECK(Flat(g35.GetKeyB()[i]) == e0 && Flat(g35.GetKeyA()[i]) == e1);
+  }
+  // ComposeWithPads with the digits of sigma(a'_i) equals Compose's b side.
+  std::vector<std::vector<Poly>> digits;
+  for (int i = 0; i < k; ++i) {
+    ASSIGN_OR_DIE(a_sub, g7.GetKeyA()[i].Substitute(5, moduli));
+    CHECK_OK(a_sub.ConvertToCoeffForm(moduli));
+    ASSIGN_OR_DIE(d, gadget.Decompose(a_sub, moduli));
+    for (auto& x : d) CHECK_OK(x.ConvertToNttForm(moduli));
+    digits.push_back(std::move(d));
+  }
+  ASSIGN_OR_DIE(pads, rlwe::RnsGaloisKey<ModularInt>::CreatePadOfComposedKey(g5.GetKeyA(), digits, moduli));
+  ASSIGN_OR_DIE(g35p, g5.ComposeWithPads(g7, digits, pads));
+  for (int i = 0; i < k; ++i) {
+    CHECK(g35p.GetKeyA()[i] == g35.GetKeyA()[i]);
+    CHECK(g35p.GetKeyB()[i] == g35.GetKeyB()[i]);
+  }
+  std::printf("  key generation, Galois serialization and composition (%s): ok\n",
+              prng_type == rlwe::PRNG_TYPE_CHACHA ? "ChaCha" : "HKDF");
+}
+
 int main() {
   TestMontgomery();
   TestNttParameters();
@@ -349,6 +536,9 @@ int main() {
   TestModReduceAndGadget();
   TestKeySwitching(10, 3, 4);
   TestKeySwitching(14, 8, 2);
+  TestSerialization();
+  TestKeyGeneration(rlwe::PRNG_TYPE_CHACHA);
+  TestKeyGeneration(rlwe::PRNG_TYPE_HKDF);
   std::printf("rns_api_test: %d checks passed\n", g_checks);
   return 0;
 }
