@@ -154,6 +154,38 @@ int shg_mod_reduce_lsb(shg_context* ctx, uint64_t* polys, uint64_t t,
                        const uint64_t* host_ql_inv, int is_ntt, int count,
                        int num_limbs, const int* primes);
 
+/* ---- RnsPolynomial::ModReduceMsb (rns/rns_polynomial.cc:349-417) in place,
+ *      the BFV modulus switch: limb i <- (a_i - lift([a]_{q_l})) * q_l^-1
+ *      with the balanced lift of the dropped limb; layout and ql_inv as for
+ *      shg_mod_reduce_lsb. */
+int shg_mod_reduce_msb(shg_context* ctx, uint64_t* polys, const uint64_t* host_ql_inv,
+                       int is_ntt, int count, int num_limbs, const int* primes);
+
+/* ---- RNS basis conversion from the chain Q = prod q_i (num_in limbs) to
+ *      P = prod p_j (num_out limbs), count polynomials, coefficient-form
+ *      output; at most 16 moduli per chain.  All constants are Montgomery
+ *      reps; host_q_hat_inv_mod_qs[i] = (Q/q_i)^-1 mod q_i.
+ *  SHG_BASIS_EXACT:  SwitchRnsBasis (rns/rns_polynomial.cc:419-517), the
+ *     centered representative of a mod P.  host_matrix[i*num_out + j] =
+ *     (Q/q_i) mod p_j, host_vec[j] = Q mod p_j.
+ *  SHG_BASIS_SCALE:  ScaleAndSwitchRnsBasis (rns/rns_polynomial.cc:519-636),
+ *     round(P * a / Q) mod P.  host_matrix[j*num_in + i] = q_i^-1 mod p_j,
+ *     host_vec[i] = P mod q_i.
+ *  SHG_BASIS_APPROX: ApproxSwitchRnsBasis (rns/rns_polynomial.cc:638-701),
+ *     sum_i [a_i (Q/q_i)^-1]_{q_i} (Q/q_i) mod p_j, the bracket lifted
+ *     balanced when `balanced`; input may be in NTT form (in_is_ntt);
+ *     host_matrix as for EXACT, host_vec unused.
+ *  Coefficient-form input is required for EXACT and SCALE, as in the
+ *  reference.  `out` must not overlap `in`. */
+#define SHG_BASIS_EXACT 0
+#define SHG_BASIS_SCALE 1
+#define SHG_BASIS_APPROX 2
+int shg_switch_rns_basis(shg_context* ctx, int mode, const uint64_t* in, int in_is_ntt,
+                         int balanced, int count, int num_in, const int* in_primes,
+                         uint64_t* out, int num_out, const int* out_primes,
+                         const uint64_t* host_q_hat_inv_mod_qs, const uint64_t* host_matrix,
+                         const uint64_t* host_vec);
+
 /* ---- RnsRlweCiphertext::Mul (rns/rns_ciphertext.h:501-524): for each of
  *      `count` pairs, out (len_a+len_b-1 components) = tensor product of a
  *      (len_a components) and b (len_b components), all NTT form. -------- */

@@ -121,6 +121,14 @@ def lib():
         L.so_chacha_rand64.argtypes = [ctypes.c_void_p]
         L.so_chacha_rand64.restype = u64
         L.so_sample_uniform_chacha.argtypes = [PP, ctypes.c_int, ctypes.c_void_p, U64P]
+        L.so_poly_mod_reduce_msb.argtypes = [PP, ctypes.c_int, U64P, ctypes.c_int, U64P]
+        L.so_poly_mod_reduce_msb.restype = ctypes.c_int
+        L.so_switch_rns_basis.argtypes = [PP, ctypes.c_int, PP, ctypes.c_int, U64P, U64P, U64P,
+                                          U64P, U64P]
+        L.so_scale_and_switch_rns_basis.argtypes = [PP, ctypes.c_int, PP, ctypes.c_int, U64P, U64P,
+                                                    U64P, U64P, U64P]
+        L.so_approx_switch_rns_basis.argtypes = [PP, ctypes.c_int, PP, ctypes.c_int, U64P, U64P,
+                                                 ctypes.c_int, ctypes.c_int, U64P, U64P]
         _lib = L
     return _lib
 
@@ -356,6 +364,61 @@ class Rns:
                 raise ValueError("DeserializeVector error %d" % r)
             out[i] = row
         return out
+
+    def mod_reduce_msb(self, x, ql_inv, is_ntt=True):
+        """RnsPolynomial::ModReduceMsb: the first L-1 limbs of the result."""
+        y = self._poly(x).reshape(self.L, self.n)
+        qi = np.ascontiguousarray(ql_inv, dtype=np.uint64)
+        rc = lib().so_poly_mod_reduce_msb(self._arr, self.L, _p64(qi), int(bool(is_ntt)), _p64(y))
+        assert rc == 0
+        return y[: self.L - 1]
+
+    def basis_constants(self, out):
+        """CRT constants for switching from this chain (Q) to `out` (P), as
+        Montgomery representations: q_hat_inv[i] = (Q/q_i)^-1 mod q_i,
+        q_hat_mod_ps[i, j] = (Q/q_i) mod p_j, q_mod_ps[j] = Q mod p_j,
+        q_inv_mod_ps[j, i] = q_i^-1 mod p_j, p_mod_qs[i] = P mod q_i."""
+        Q, P = 1, 1
+        for q in self.qs:
+            Q *= q
+        for p in out.qs:
+            P *= p
+        imp = lambda m, v: m.imp(v % m.s.modulus)  # noqa: E731
+        qm = [p.mont for p in self.primes]
+        pm = [p.mont for p in out.primes]
+        return {
+            "q_hat_inv": np.array([imp(qm[i], pow(Q // q, -1, q)) for i, q in enumerate(self.qs)],
+                                  dtype=np.uint64),
+            "q_hat_mod_ps": np.array([[imp(pm[j], Q // q) for j in range(out.L)] for q in self.qs],
+                                     dtype=np.uint64),
+            "q_mod_ps": np.array([imp(pm[j], Q) for j in range(out.L)], dtype=np.uint64),
+            "q_inv_mod_ps": np.array([[imp(pm[j], pow(q, -1, out.qs[j])) for q in self.qs]
+                                      for j in range(out.L)], dtype=np.uint64),
+            "p_mod_qs": np.array([imp(qm[i], P) for i in range(self.L)], dtype=np.uint64),
+        }
+
+    def switch_rns_basis(self, x, out, c):
+        y = np.zeros((out.L, self.n), dtype=np.uint64)
+        lib().so_switch_rns_basis(self._arr, self.L, out._arr, out.L, _p64(c["q_hat_inv"]),
+                                  _p64(np.ascontiguousarray(c["q_hat_mod_ps"])), _p64(c["q_mod_ps"]),
+                                  _p64(self._poly(x)), _p64(y))
+        return y
+
+    def scale_and_switch_rns_basis(self, x, out, c):
+        y = np.zeros((out.L, self.n), dtype=np.uint64)
+        lib().so_scale_and_switch_rns_basis(self._arr, self.L, out._arr, out.L,
+                                            _p64(c["q_hat_inv"]),
+                                            _p64(np.ascontiguousarray(c["q_inv_mod_ps"])),
+                                            _p64(c["p_mod_qs"]), _p64(self._poly(x)), _p64(y))
+        return y
+
+    def approx_switch_rns_basis(self, x, out, c, is_ntt, balanced):
+        y = np.zeros((out.L, self.n), dtype=np.uint64)
+        lib().so_approx_switch_rns_basis(self._arr, self.L, out._arr, out.L, _p64(c["q_hat_inv"]),
+                                         _p64(np.ascontiguousarray(c["q_hat_mod_ps"])),
+                                         int(bool(is_ntt)), int(bool(balanced)),
+                                         _p64(self._poly(x)), _p64(y))
+        return y
 
     def sample_uniform_chacha(self, prng, count):
         """`count` RnsPolynomial::SampleUniform draws from one ChaCha PRNG."""

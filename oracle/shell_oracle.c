@@ -13,6 +13,7 @@
  */
 #include "shell_oracle.h"
 
+#include <math.h>
 #include <stdlib.h>
 #include <string.h>
 #include <time.h>
@@ -904,4 +905,173 @@ void so_sample_uniform_chacha(so_prime* const* primes, int nl, so_chacha_prng* g
     const size_t n = (size_t)1 << primes[i]->log_n;
     for (size_t j = 0; j < n; j++) x[i * n + j] = so_import_random_chacha(&primes[i]->mp, g);
   }
+}
+
+/* ------------------------------------------------------------------------ */
+/* L9: BFV-side RNS operations (rns/rns_polynomial.cc:349-852)               */
+/* ------------------------------------------------------------------------ */
+
+/* ConvertModulus (modulus_conversion.h:140-157): export mod q, import mod p. */
+static void so_convert_modulus(const so_mont* pq, const so_mont* pp, const uint64_t* in,
+                               uint64_t* out, int n) {
+  if (pq->modulus == pp->modulus) {
+    memmove(out, in, sizeof(uint64_t) * (size_t)n);
+    return;
+  }
+  for (int i = 0; i < n; i++) out[i] = so_import(pp, so_export(pq, in[i]));
+}
+
+/* RnsPolynomial::ModReduceMsb (rns/rns_polynomial.cc:349-417): on return
+ * the first (nl - 1) limbs hold (a - [a]_{q_l}) / q_l. */
+int so_poly_mod_reduce_msb(so_prime* const* primes, int nl, const uint64_t* ql_inv, int is_ntt,
+                           uint64_t* x) {
+  if (nl <= 1) return -1;
+  const size_t n = (size_t)1 << primes[0]->log_n;
+  const int nr = nl - 1;
+  const so_prime* pl = primes[nl - 1];
+  uint64_t* delta_ql = (uint64_t*)malloc(n * 8);
+  memcpy(delta_ql, x + (size_t)nr * n, n * 8);
+  uint64_t* delta = (uint64_t*)malloc(n * 8 * (size_t)nr);
+  if (is_ntt) {
+    so_ntt_inv(pl, delta_ql);
+    for (int i = 0; i < nr; i++)
+      so_convert_modulus_balanced(&pl->mp, &primes[i]->mp, delta_ql, delta + i * n, (int)n);
+    so_poly_ntt_fwd(primes, nr, delta);
+    so_poly_pointwise(primes, nr, SO_SUB, x, delta, NULL);
+    so_poly_mul_scalar(primes, nr, x, ql_inv);
+  } else {
+    for (int i = 0; i < nr; i++) {
+      const so_mont* mi = &primes[i]->mp;
+      uint64_t* di = delta + i * n;
+      so_convert_modulus_balanced(&pl->mp, mi, delta_ql, di, (int)n);
+      for (size_t j = 0; j < n; j++) x[i * n + j] = so_sub(mi, x[i * n + j], di[j]);
+      for (size_t j = 0; j < n; j++) x[i * n + j] = so_mul(mi, x[i * n + j], ql_inv[i]);
+    }
+  }
+  free(delta);
+  free(delta_ql);
+  return 0;
+}
+
+/* y_i = a_i * q_hat_inv_i mod q_i for every limb, and u = round(sum_k
+ * export(y_k) / q_k) per coefficient, in the reference's order of floating
+ * point operations (rns/rns_polynomial.cc:451-483). */
+static void so_scaled_and_u(so_prime* const* in_p, int L, const uint64_t* q_hat_inv,
+                            const uint64_t* pre_scale, const uint64_t* in, uint64_t* y,
+                            double* us) {
+  const size_t n = (size_t)1 << in_p[0]->log_n;
+  for (size_t k = 0; k < n; k++) us[k] = 0.0;
+  for (int i = 0; i < L; i++) {
+    const so_mont* mi = &in_p[i]->mp;
+    for (size_t k = 0; k < n; k++) {
+      uint64_t v = in[i * n + k];
+      if (pre_scale) v = so_mul(mi, v, pre_scale[i]);
+      y[i * n + k] = so_mul(mi, v, q_hat_inv[i]);
+    }
+    const double qd = (double)mi->modulus;
+    for (size_t k = 0; k < n; k++) {
+      double s = (double)so_export(mi, y[i * n + k]);
+      s /= qd;
+      us[k] += s;
+    }
+  }
+}
+
+/* RnsPolynomial::SwitchRnsBasis (rns/rns_polynomial.cc:419-517): exact
+ * conversion of a coefficient-form polynomial mod Q to mod P.
+ * q_hat_mod_ps[i * K + j] = (Q/q_i) mod p_j, q_mod_ps[j] = Q mod p_j. */
+void so_switch_rns_basis(so_prime* const* in_p, int L, so_prime* const* out_p, int K,
+                         const uint64_t* q_hat_inv, const uint64_t* q_hat_mod_ps,
+                         const uint64_t* q_mod_ps, const uint64_t* in, uint64_t* out) {
+  const size_t n = (size_t)1 << in_p[0]->log_n;
+  uint64_t* y = (uint64_t*)malloc(n * 8 * (size_t)L);
+  double* us = (double*)malloc(n * sizeof(double));
+  uint64_t* tmp = (uint64_t*)malloc(n * 8);
+  so_scaled_and_u(in_p, L, q_hat_inv, NULL, in, y, us);
+  memset(out, 0, n * 8 * (size_t)K);
+  for (int i = 0; i < L; i++) {
+    for (int j = 0; j < K; j++) {
+      const so_mont* mj = &out_p[j]->mp;
+      so_convert_modulus(&in_p[i]->mp, mj, y + i * n, tmp, (int)n);
+      for (size_t k = 0; k < n; k++) {
+        tmp[k] = so_mul(mj, tmp[k], q_hat_mod_ps[i * K + j]);
+        out[j * n + k] = so_add(mj, out[j * n + k], tmp[k]);
+      }
+    }
+  }
+  for (int j = 0; j < K; j++) {
+    const so_mont* mj = &out_p[j]->mp;
+    for (size_t k = 0; k < n; k++) {
+      const uint64_t u = (uint64_t)round(us[k]);
+      const uint64_t c = so_mul(mj, so_import(mj, u), q_mod_ps[j]);
+      out[j * n + k] = so_sub(mj, out[j * n + k], c);
+    }
+  }
+  free(tmp);
+  free(us);
+  free(y);
+}
+
+/* RnsPolynomial::ScaleAndSwitchRnsBasis (rns/rns_polynomial.cc:519-636):
+ * round(P * a / Q) mod P for a coefficient-form polynomial mod Q.
+ * q_inv_mod_ps[j * L + i] = q_i^-1 mod p_j, p_mod_qs[i] = P mod q_i. */
+void so_scale_and_switch_rns_basis(so_prime* const* in_p, int L, so_prime* const* out_p, int K,
+                                   const uint64_t* q_hat_inv, const uint64_t* q_inv_mod_ps,
+                                   const uint64_t* p_mod_qs, const uint64_t* in, uint64_t* out) {
+  const size_t n = (size_t)1 << in_p[0]->log_n;
+  uint64_t* y = (uint64_t*)malloc(n * 8 * (size_t)L);
+  double* us = (double*)malloc(n * sizeof(double));
+  uint64_t* tmp = (uint64_t*)malloc(n * 8);
+  so_scaled_and_u(in_p, L, q_hat_inv, p_mod_qs, in, y, us);
+  memset(out, 0, n * 8 * (size_t)K);
+  for (int i = 0; i < L; i++) {
+    for (int j = 0; j < K; j++) {
+      const so_mont* mj = &out_p[j]->mp;
+      so_convert_modulus(&in_p[i]->mp, mj, y + i * n, tmp, (int)n);
+      const uint64_t neg = so_neg(mj, q_inv_mod_ps[j * L + i]);
+      for (size_t k = 0; k < n; k++) {
+        tmp[k] = so_mul(mj, tmp[k], neg);
+        out[j * n + k] = so_add(mj, out[j * n + k], tmp[k]);
+      }
+    }
+  }
+  for (int j = 0; j < K; j++) {
+    const so_mont* mj = &out_p[j]->mp;
+    for (size_t k = 0; k < n; k++) {
+      const uint64_t u = (uint64_t)round(us[k]);
+      out[j * n + k] = so_add(mj, out[j * n + k], so_import(mj, u));
+    }
+  }
+  free(tmp);
+  free(us);
+  free(y);
+}
+
+/* RnsPolynomial::ApproxSwitchRnsBasis (rns/rns_polynomial.cc:638-701): the
+ * fast basis conversion sum_i [a_i (Q/q_i)^-1]_{q_i} (Q/q_i) mod p_j, the
+ * bracket lifted balanced or not; input in either form, output in
+ * coefficient form. */
+void so_approx_switch_rns_basis(so_prime* const* in_p, int L, so_prime* const* out_p, int K,
+                                const uint64_t* q_hat_inv, const uint64_t* q_hat_mod_ps,
+                                int is_ntt, int balanced, const uint64_t* in, uint64_t* out) {
+  const size_t n = (size_t)1 << in_p[0]->log_n;
+  uint64_t* a = (uint64_t*)malloc(n * 8);
+  uint64_t* tmp = (uint64_t*)malloc(n * 8);
+  memset(out, 0, n * 8 * (size_t)K);
+  for (int i = 0; i < L; i++) {
+    const so_mont* mi = &in_p[i]->mp;
+    for (size_t k = 0; k < n; k++) a[k] = so_mul(mi, in[i * n + k], q_hat_inv[i]);
+    if (is_ntt) so_ntt_inv(in_p[i], a);
+    for (int j = 0; j < K; j++) {
+      const so_mont* mj = &out_p[j]->mp;
+      if (balanced) so_convert_modulus_balanced(mi, mj, a, tmp, (int)n);
+      else so_convert_modulus(mi, mj, a, tmp, (int)n);
+      for (size_t k = 0; k < n; k++) {
+        tmp[k] = so_mul(mj, tmp[k], q_hat_mod_ps[i * K + j]);
+        out[j * n + k] = so_add(mj, out[j * n + k], tmp[k]);
+      }
+    }
+  }
+  free(tmp);
+  free(a);
 }

@@ -161,6 +161,19 @@ uint64_t so_chacha_rand64(so_chacha_prng* g);
 uint64_t so_import_random_chacha(const so_mont* p, so_chacha_prng* g);
 void so_sample_uniform_chacha(so_prime* const* primes, int nl, so_chacha_prng* g, uint64_t* x);
 
+/* ---- BFV-side RNS operations (rns/rns_polynomial.cc:349-701) ---------- */
+int so_poly_mod_reduce_msb(so_prime* const* primes, int nl, const uint64_t* ql_inv, int is_ntt,
+                           uint64_t* x);
+void so_switch_rns_basis(so_prime* const* in_p, int L, so_prime* const* out_p, int K,
+                         const uint64_t* q_hat_inv, const uint64_t* q_hat_mod_ps,
+                         const uint64_t* q_mod_ps, const uint64_t* in, uint64_t* out);
+void so_scale_and_switch_rns_basis(so_prime* const* in_p, int L, so_prime* const* out_p, int K,
+                                   const uint64_t* q_hat_inv, const uint64_t* q_inv_mod_ps,
+                                   const uint64_t* p_mod_qs, const uint64_t* in, uint64_t* out);
+void so_approx_switch_rns_basis(so_prime* const* in_p, int L, so_prime* const* out_p, int K,
+                                const uint64_t* q_hat_inv, const uint64_t* q_hat_mod_ps,
+                                int is_ntt, int balanced, const uint64_t* in, uint64_t* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -19,7 +19,7 @@ INCLUDE = os.path.join(ROOT, "include")
 LIB = os.path.join(HERE, "libshellgpu.so")
 
 SOURCES = ["shg_api.hip", "shg_ntt.hip", "shg_ops.hip", "shg_keyswitch.hip", "shg_batch.hip",
-           "shg_pir.hip", "shg_serving.hip", "shg_wire.hip"]
+           "shg_pir.hip", "shg_serving.hip", "shg_wire.hip", "shg_bfv.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("SHG_OFFLOAD_ARCH", "gfx950")

@@ -370,6 +370,137 @@ class RnsPolynomial {
     return OkStatus();
   }
 
+  // rns/rns_polynomial.cc:349-417: (a - lift([a]_{q_l})) / q_l, the BFV
+  // modulus switch, in either form.
+  Status ModReduceMsb(const RnsInt<ModularInt>& ql_inv, Moduli moduli) {
+    RLWE_RETURN_IF_ERROR(CheckModuli(moduli));
+    if (num_moduli_ <= 1)
+      return InvalidArgumentError("ModReduceMsb cannot apply with only one prime modulus.");
+    if (ql_inv.NumModuli() != num_moduli_ - 1)
+      return InvalidArgumentError("`ql_inv` must be defined with respect to " +
+                                  std::to_string(num_moduli_ - 1) + " moduli.");
+    std::vector<uint64_t> qi(num_moduli_ - 1);
+    for (int i = 0; i < num_moduli_ - 1; ++i) qi[i] = ql_inv.zs[i].GetMontgomeryRepresentation();
+    auto ids = DevicePrimes<ModularInt>(moduli);
+    RLWE_RETURN_IF_ERROR(device::FromCode(shg_mod_reduce_msb(
+        device::Context(), buf_->data(), qi.data(), is_ntt_ ? 1 : 0, 1, num_moduli_, ids.data())));
+    num_moduli_ -= 1;
+    Touch();
+    return OkStatus();
+  }
+
+  // rns/rns_polynomial.cc:419-517: exact conversion to `output_moduli`
+  // (the centered representative), coefficient form.
+  StatusOr<RnsPolynomial> SwitchRnsBasis(Moduli this_moduli, Moduli output_moduli,
+                                         Span<const ModularInt> prime_q_hat_inv_mod_qs,
+                                         Span<const RnsInt<ModularInt>> prime_q_hat_mod_ps,
+                                         Span<const ModularInt> q_mod_ps) const {
+    if (is_ntt_) return InvalidArgumentError("Polynomial must be in coefficient form.");
+    RLWE_RETURN_IF_ERROR(CheckBasisArgs(this_moduli, output_moduli, prime_q_hat_inv_mod_qs.size(),
+                                        prime_q_hat_mod_ps.size(), false));
+    if (q_mod_ps.size() != output_moduli.size())
+      return InvalidArgumentError("`q_mod_ps` must contain " + std::to_string(output_moduli.size()) +
+                                  " elements.");
+    std::vector<uint64_t> mat, vec;
+    for (int i = 0; i < num_moduli_; ++i)
+      for (size_t j = 0; j < output_moduli.size(); ++j)
+        mat.push_back(prime_q_hat_mod_ps[i].Component(j).GetMontgomeryRepresentation());
+    for (size_t j = 0; j < output_moduli.size(); ++j) vec.push_back(q_mod_ps[j].GetMontgomeryRepresentation());
+    return Switch(SHG_BASIS_EXACT, false, this_moduli, output_moduli, prime_q_hat_inv_mod_qs, mat,
+                  vec);
+  }
+
+  // rns/rns_polynomial.cc:519-636: round(P * a / Q) mod P, coefficient form.
+  StatusOr<RnsPolynomial> ScaleAndSwitchRnsBasis(Moduli this_moduli, Moduli output_moduli,
+                                                 Span<const ModularInt> prime_q_hat_inv_mod_qs,
+                                                 Span<const RnsInt<ModularInt>> prime_q_inv_mod_ps,
+                                                 Span<const ModularInt> p_mod_qs) const {
+    if (is_ntt_) return InvalidArgumentError("Polynomial must be in coefficient form.");
+    RLWE_RETURN_IF_ERROR(CheckBasisArgs(this_moduli, output_moduli, prime_q_hat_inv_mod_qs.size(),
+                                        num_moduli_, false));
+    if (prime_q_inv_mod_ps.size() != output_moduli.size())
+      return InvalidArgumentError("`prime_q_inv_mod_ps` must contain " +
+                                  std::to_string(output_moduli.size()) + " elements.");
+    if ((int)p_mod_qs.size() < num_moduli_)
+      return InvalidArgumentError("`p_mod_qs` must contain at least " +
+                                  std::to_string(num_moduli_) + " elements.");
+    std::vector<uint64_t> mat, vec;
+    for (size_t j = 0; j < output_moduli.size(); ++j)
+      for (int i = 0; i < num_moduli_; ++i)
+        mat.push_back(prime_q_inv_mod_ps[j].Component(i).GetMontgomeryRepresentation());
+    for (int i = 0; i < num_moduli_; ++i) vec.push_back(p_mod_qs[i].GetMontgomeryRepresentation());
+    return Switch(SHG_BASIS_SCALE, false, this_moduli, output_moduli, prime_q_hat_inv_mod_qs, mat,
+                  vec);
+  }
+
+  // rns/rns_polynomial.cc:638-701: fast approximate conversion, input in
+  // either form, coefficient-form output.
+  StatusOr<RnsPolynomial> ApproxSwitchRnsBasis(Moduli this_moduli, Moduli output_moduli,
+                                               Span<const ModularInt> prime_q_hat_inv_mod_qs,
+                                               Span<const RnsInt<ModularInt>> prime_q_hat_mod_ps,
+                                               bool is_balanced_rep = false) const {
+    RLWE_RETURN_IF_ERROR(CheckBasisArgs(this_moduli, output_moduli, prime_q_hat_inv_mod_qs.size(),
+                                        prime_q_hat_mod_ps.size(), true));
+    std::vector<uint64_t> mat;
+    for (int i = 0; i < num_moduli_; ++i)
+      for (size_t j = 0; j < output_moduli.size(); ++j)
+        mat.push_back(prime_q_hat_mod_ps[i].Component(j).GetMontgomeryRepresentation());
+    return Switch(SHG_BASIS_APPROX, is_balanced_rep, this_moduli, output_moduli,
+                  prime_q_hat_inv_mod_qs, mat, {});
+  }
+
+  // rns/rns_polynomial.cc:703-797: P^-1 (p_mod_t * a + t [k * aux]_P) mod Q
+  // with k = -p_mod_t * t^-1 mod P, delta switched to Q approximately with
+  // the balanced lift.
+  Status ApproxModReduceLsb(RnsPolynomial polynomial_aux, Moduli this_moduli, Moduli aux_moduli,
+                            Span<const ModularInt> p_inv_mod_qs,
+                            Span<const ModularInt> prime_p_hat_inv_mod_ps,
+                            Span<const RnsInt<ModularInt>> prime_p_hat_mod_qs,
+                            Span<const ModularInt> t_inv_mod_ps, typename ModularInt::Int p_mod_t,
+                            typename ModularInt::Int t) {
+    RLWE_RETURN_IF_ERROR(CheckApproxModReduce(polynomial_aux, this_moduli, aux_moduli, p_inv_mod_qs,
+                                              prime_p_hat_inv_mod_ps, prime_p_hat_mod_qs));
+    if ((int)t_inv_mod_ps.size() != polynomial_aux.num_moduli_)
+      return InvalidArgumentError("`t_inv_mod_ps` must contain " +
+                                  std::to_string(polynomial_aux.num_moduli_) + " elements.");
+    // aux <- aux * p_mod_t * (-t^-1) per aux limb.
+    std::vector<ModularInt> scal;
+    for (size_t j = 0; j < aux_moduli.size(); ++j) {
+      const auto* mp = aux_moduli[j]->ModParams();
+      RLWE_ASSIGN_OR_RETURN(ModularInt r, ModularInt::ImportInt(p_mod_t, mp));
+      ModularInt neg = ModularInt(mp->modulus - t_inv_mod_ps[j].GetMontgomeryRepresentation());
+      scal.push_back(r.Mul(neg, mp));
+    }
+    RLWE_RETURN_IF_ERROR(polynomial_aux.MulInPlace(Span<const ModularInt>(scal), aux_moduli));
+    RLWE_ASSIGN_OR_RETURN(RnsPolynomial delta,
+                          polynomial_aux.ApproxSwitchRnsBasis(aux_moduli, this_moduli,
+                                                              prime_p_hat_inv_mod_ps,
+                                                              prime_p_hat_mod_qs, true));
+    RLWE_RETURN_IF_ERROR(delta.ConvertToNttForm(this_moduli));
+    RLWE_RETURN_IF_ERROR(delta.MulInPlace(t, this_moduli));
+    if (!is_ntt_) RLWE_RETURN_IF_ERROR(ConvertToNttForm(this_moduli));
+    RLWE_RETURN_IF_ERROR(MulInPlace(p_mod_t, this_moduli));
+    RLWE_RETURN_IF_ERROR(AddInPlace(delta, this_moduli));
+    return MulInPlace(p_inv_mod_qs, this_moduli);
+  }
+
+  // rns/rns_polynomial.cc:799-852: P^-1 (a - [aux]_P) mod Q.
+  Status ApproxModReduceMsb(RnsPolynomial polynomial_aux, Moduli this_moduli, Moduli aux_moduli,
+                            Span<const ModularInt> p_inv_mod_qs,
+                            Span<const ModularInt> prime_p_hat_inv_mod_ps,
+                            Span<const RnsInt<ModularInt>> prime_p_hat_mod_qs) {
+    RLWE_RETURN_IF_ERROR(CheckApproxModReduce(polynomial_aux, this_moduli, aux_moduli, p_inv_mod_qs,
+                                              prime_p_hat_inv_mod_ps, prime_p_hat_mod_qs));
+    RLWE_ASSIGN_OR_RETURN(RnsPolynomial delta,
+                          polynomial_aux.ApproxSwitchRnsBasis(aux_moduli, this_moduli,
+                                                              prime_p_hat_inv_mod_ps,
+                                                              prime_p_hat_mod_qs, false));
+    RLWE_RETURN_IF_ERROR(delta.ConvertToNttForm(this_moduli));
+    if (!is_ntt_) RLWE_RETURN_IF_ERROR(ConvertToNttForm(this_moduli));
+    RLWE_RETURN_IF_ERROR(SubInPlace(delta, this_moduli));
+    return MulInPlace(p_inv_mod_qs, this_moduli);
+  }
+
   void AppendCoeffVector(std::vector<ModularInt> coeffs) {
     auto host = Coeffs();
     host.push_back(std::move(coeffs));
@@ -429,6 +560,56 @@ class RnsPolynomial {
       return InvalidArgumentError("`moduli` must contain " + std::to_string(num_moduli_) +
                                   " RNS moduli.");
     return OkStatus();
+  }
+  Status CheckBasisArgs(Moduli this_moduli, Moduli output_moduli, size_t num_q_hat_inv,
+                        size_t num_q_hat, bool exact_counts) const {
+    if ((int)this_moduli.size() != num_moduli_)
+      return InvalidArgumentError("`this_moduli` must contain " + std::to_string(num_moduli_) +
+                                  " RNS moduli.");
+    if (output_moduli.empty()) return InvalidArgumentError("`output_moduli` must not be empty.");
+    const std::string qualifier = exact_counts ? "" : "at least ";
+    if (exact_counts ? num_q_hat_inv != (size_t)num_moduli_ : num_q_hat_inv < (size_t)num_moduli_)
+      return InvalidArgumentError("`prime_q_hat_inv_mod_qs` must contain " + qualifier +
+                                  std::to_string(num_moduli_) + " elements.");
+    if (exact_counts ? num_q_hat != (size_t)num_moduli_ : num_q_hat < (size_t)num_moduli_)
+      return InvalidArgumentError("`prime_q_hat_mod_ps` must contain " + qualifier +
+                                  std::to_string(num_moduli_) + " elements.");
+    return OkStatus();
+  }
+  Status CheckApproxModReduce(const RnsPolynomial& aux, Moduli this_moduli, Moduli aux_moduli,
+                              Span<const ModularInt> p_inv_mod_qs,
+                              Span<const ModularInt> prime_p_hat_inv_mod_ps,
+                              Span<const RnsInt<ModularInt>> prime_p_hat_mod_qs) const {
+    if ((int)this_moduli.size() != num_moduli_)
+      return InvalidArgumentError("`this_moduli` must contain " + std::to_string(num_moduli_) +
+                                  " RNS moduli.");
+    if ((int)aux_moduli.size() != aux.num_moduli_)
+      return InvalidArgumentError("`aux_moduli` must contain " + std::to_string(aux.num_moduli_) +
+                                  " RNS moduli.");
+    if ((int)p_inv_mod_qs.size() != num_moduli_)
+      return InvalidArgumentError("`p_inv_mod_qs` must contain " + std::to_string(num_moduli_) +
+                                  " elements.");
+    if ((int)prime_p_hat_inv_mod_ps.size() != aux.num_moduli_)
+      return InvalidArgumentError("`prime_p_hat_inv_mod_ps` must contain " +
+                                  std::to_string(aux.num_moduli_) + " elements.");
+    if ((int)prime_p_hat_mod_qs.size() != aux.num_moduli_)
+      return InvalidArgumentError("`prime_p_hat_mod_qs` must contain " +
+                                  std::to_string(aux.num_moduli_) + " elements.");
+    return OkStatus();
+  }
+  StatusOr<RnsPolynomial> Switch(int mode, bool balanced, Moduli this_moduli, Moduli output_moduli,
+                                 Span<const ModularInt> q_hat_inv, const std::vector<uint64_t>& mat,
+                                 const std::vector<uint64_t>& vec) const {
+    std::vector<uint64_t> qh;
+    for (int i = 0; i < num_moduli_; ++i) qh.push_back(q_hat_inv[i].GetMontgomeryRepresentation());
+    auto in_ids = DevicePrimes<ModularInt>(this_moduli);
+    auto out_ids = DevicePrimes<ModularInt>(output_moduli);
+    RnsPolynomial out(log_n_, (int)output_moduli.size(), false);
+    RLWE_RETURN_IF_ERROR(device::FromCode(shg_switch_rns_basis(
+        device::Context(), mode, buf_->data(), is_ntt_ ? 1 : 0, balanced ? 1 : 0, 1, num_moduli_,
+        in_ids.data(), out.buf_->data(), (int)output_moduli.size(), out_ids.data(), qh.data(),
+        mat.data(), vec.empty() ? nullptr : vec.data())));
+    return out;
   }
   Status CheckThat(const RnsPolynomial& that, const char* name) const {
     if (that.num_moduli_ != num_moduli_)

@@ -529,6 +529,153 @@ static void TestKeyGeneration(rlwe::PrngType prng_type) {
               prng_type == rlwe::PRNG_TYPE_CHACHA ? "ChaCha" : "HKDF");
 }
 
+static uint64_t MulMod(uint64_t a, uint64_t b, uint64_t m) {
+  return (uint64_t)((unsigned __int128)a * b % m);
+}
+static uint64_t PowMod(uint64_t a, uint64_t e, uint64_t m) {
+  uint64_t r = 1 % m;
+  for (a %= m; e; e >>= 1, a = MulMod(a, a, m))
+    if (e & 1) r = MulMod(r, a, m);
+  return r;
+}
+
+// rns/rns_polynomial.cc:349-852 through the C++ API against the oracle:
+// ModReduceMsb, the three basis conversions and both approximate modulus
+// reductions (BFV with an auxiliary chain P).
+static void TestBfvOps() {
+  const int log_n = 11, L = 4, K = 3, n = 1 << log_n;
+  auto all = NttPrimes(log_n, L + K);
+  std::vector<uint64_t> qs(all.begin(), all.begin() + L), ps(all.begin() + L, all.end());
+  auto ctx = rlwe::RnsContext<ModularInt>::CreateForBgv(log_n, qs, 65537).value();
+  auto pctx = rlwe::RnsContext<ModularInt>::CreateForBgv(log_n, ps, 65537).value();
+  auto qm = ctx.MainPrimeModuli(), pm = pctx.MainPrimeModuli();
+  OracleChain oq(qs, log_n), op(ps, log_n);
+  auto imp = [](uint64_t v, const rlwe::PrimeModulus<ModularInt>* m) {
+    return ModularInt::ImportInt(v % m->Modulus(), m->ModParams()).value();
+  };
+  // Constants of the conversion from `from` to `to`.
+  struct Consts {
+    std::vector<ModularInt> hat_inv, mod_to, inv_from_mod_to_flat;
+    std::vector<rlwe::RnsInt<ModularInt>> hat_mod_to, inv_mod_to;
+    std::vector<ModularInt> to_mod_from;
+  };
+  auto consts = [&](const std::vector<uint64_t>& from, const std::vector<const rlwe::PrimeModulus<ModularInt>*>& fm,
+                    const std::vector<uint64_t>& to, const std::vector<const rlwe::PrimeModulus<ModularInt>*>& tm) {
+    Consts c;
+    for (size_t i = 0; i < from.size(); ++i) {
+      uint64_t hat = 1;
+      for (size_t k = 0; k < from.size(); ++k) if (k != i) hat = MulMod(hat, from[k] % from[i], from[i]);
+      c.hat_inv.push_back(imp(PowMod(hat, from[i] - 2, from[i]), fm[i]));
+      rlwe::RnsInt<ModularInt> r;
+      for (size_t j = 0; j < to.size(); ++j) {
+        uint64_t h = 1;
+        for (size_t k = 0; k < from.size(); ++k) if (k != i) h = MulMod(h, from[k] % to[j], to[j]);
+        r.zs.push_back(imp(h, tm[j]));
+      }
+      c.hat_mod_to.push_back(r);
+      uint64_t t = 1;
+      for (uint64_t p : to) t = MulMod(t, p % from[i], from[i]);
+      c.to_mod_from.push_back(imp(t, fm[i]));
+    }
+    for (size_t j = 0; j < to.size(); ++j) {
+      uint64_t prod = 1;
+      rlwe::RnsInt<ModularInt> r;
+      for (size_t i = 0; i < from.size(); ++i) {
+        prod = MulMod(prod, from[i] % to[j], to[j]);
+        r.zs.push_back(imp(PowMod(from[i] % to[j], to[j] - 2, to[j]), tm[j]));
+      }
+      c.mod_to.push_back(imp(prod, tm[j]));
+      c.inv_mod_to.push_back(r);
+    }
+    return c;
+  };
+  auto reps = [](const std::vector<ModularInt>& v) {
+    std::vector<uint64_t> r;
+    for (const auto& x : v) r.push_back(x.GetMontgomeryRepresentation());
+    return r;
+  };
+  Consts cq = consts(qs, qm, ps, pm), cp = consts(ps, pm, qs, qm);
+  std::vector<uint64_t> hat_q_p, inv_q_p;
+  for (int i = 0; i < L; ++i) for (int j = 0; j < K; ++j) hat_q_p.push_back(cq.hat_mod_to[i].zs[j].GetMontgomeryRepresentation());
+  for (int j = 0; j < K; ++j) for (int i = 0; i < L; ++i) inv_q_p.push_back(cq.inv_mod_to[j].zs[i].GetMontgomeryRepresentation());
+  std::vector<uint64_t> hat_p_q;
+  for (int j = 0; j < K; ++j) for (int i = 0; i < L; ++i) hat_p_q.push_back(cp.hat_mod_to[j].zs[i].GetMontgomeryRepresentation());
+
+  // ModReduceMsb in both forms.
+  auto ql_inv = ctx.MainPrimeModulusInverseResidues()[L - 1].Prefix(L - 1);
+  for (bool ntt : {true, false}) {
+    auto x = Uniform(oq, L, ntt ? 501 : 502);
+    auto p = FromFlat(x, L, n, ntt);
+    CHECK_OK(p.ModReduceMsb(ql_inv, qm));
+    std::vector<uint64_t> qi;
+    for (int i = 0; i < L - 1; ++i) qi.push_back(ql_inv.zs[i].GetMontgomeryRepresentation());
+    CHECK(so_poly_mod_reduce_msb(oq.pp(), L, qi.data(), ntt ? 1 : 0, x.data()) == 0);
+    x.resize((size_t)(L - 1) * n);
+    CHECK(p.NumModuli() == L - 1 && Flat(p) == x);
+  }
+  // Basis conversions.
+  auto x = Uniform(oq, L, 503);
+  auto a = FromFlat(x, L, n, false);
+  std::vector<uint64_t> e((size_t)K * n);
+  ASSIGN_OR_DIE(sw, a.SwitchRnsBasis(qm, pm, cq.hat_inv, cq.hat_mod_to, cq.mod_to));
+  so_switch_rns_basis(oq.pp(), L, op.pp(), K, reps(cq.hat_inv).data(), hat_q_p.data(),
+                      reps(cq.mod_to).data(), x.data(), e.data());
+  CHECK(Flat(sw) == e && !sw.IsNttForm());
+  ASSIGN_OR_DIE(sc, a.ScaleAndSwitchRnsBasis(qm, pm, cq.hat_inv, cq.inv_mod_to, cq.to_mod_from));
+  so_scale_and_switch_rns_basis(oq.pp(), L, op.pp(), K, reps(cq.hat_inv).data(), inv_q_p.data(),
+                                reps(cq.to_mod_from).data(), x.data(), e.data());
+  CHECK(Flat(sc) == e);
+  auto an = FromFlat(x, L, n, true);
+  for (bool bal : {false, true}) {
+    ASSIGN_OR_DIE(ap, an.ApproxSwitchRnsBasis(qm, pm, cq.hat_inv, cq.hat_mod_to, bal));
+    so_approx_switch_rns_basis(oq.pp(), L, op.pp(), K, reps(cq.hat_inv).data(), hat_q_p.data(), 1,
+                               bal ? 1 : 0, x.data(), e.data());
+    CHECK(Flat(ap) == e);
+  }
+  CHECK(!an.SwitchRnsBasis(qm, pm, cq.hat_inv, cq.hat_mod_to, cq.mod_to).ok());  // NTT form
+  // Approximate modulus reductions by P (aux chain), a in NTT form mod Q.
+  const uint64_t t = 65537;
+  uint64_t P_mod_t = 1;
+  for (uint64_t p : ps) P_mod_t = MulMod(P_mod_t, p % t, t);
+  std::vector<ModularInt> p_inv, t_inv;
+  for (int i = 0; i < L; ++i) {
+    uint64_t P = 1;
+    for (uint64_t p : ps) P = MulMod(P, p % qs[i], qs[i]);
+    p_inv.push_back(imp(PowMod(P, qs[i] - 2, qs[i]), qm[i]));
+  }
+  for (int j = 0; j < K; ++j) t_inv.push_back(imp(PowMod(t, ps[j] - 2, ps[j]), pm[j]));
+  for (int lsb = 0; lsb < 2; ++lsb) {
+    auto av = Uniform(oq, L, 600 + lsb), auxv = Uniform(op, K, 610 + lsb);
+    auto pa = FromFlat(av, L, n, true);
+    auto paux = FromFlat(auxv, K, n, false);
+    std::vector<uint64_t> delta((size_t)L * n), aux2 = auxv;
+    if (lsb) {
+      CHECK_OK(pa.ApproxModReduceLsb(paux, qm, pm, p_inv, cp.hat_inv, cp.hat_mod_to, t_inv, P_mod_t, t));
+      std::vector<uint64_t> sc_aux;
+      for (int j = 0; j < K; ++j) {
+        const so_mont* mj = &op.primes[j]->mp;
+        sc_aux.push_back(so_mul(mj, so_import(mj, P_mod_t), so_neg(mj, t_inv[j].GetMontgomeryRepresentation())));
+      }
+      so_poly_mul_scalar(op.pp(), K, aux2.data(), sc_aux.data());
+      so_approx_switch_rns_basis(op.pp(), K, oq.pp(), L, reps(cp.hat_inv).data(), hat_p_q.data(), 0, 1,
+                                 aux2.data(), delta.data());
+      so_poly_ntt_fwd(oq.pp(), L, delta.data());
+      so_poly_mul_int(oq.pp(), L, delta.data(), t);
+      so_poly_mul_int(oq.pp(), L, av.data(), P_mod_t);
+      so_poly_pointwise(oq.pp(), L, SO_ADD, av.data(), delta.data(), nullptr);
+    } else {
+      CHECK_OK(pa.ApproxModReduceMsb(paux, qm, pm, p_inv, cp.hat_inv, cp.hat_mod_to));
+      so_approx_switch_rns_basis(op.pp(), K, oq.pp(), L, reps(cp.hat_inv).data(), hat_p_q.data(), 0, 0,
+                                 aux2.data(), delta.data());
+      so_poly_ntt_fwd(oq.pp(), L, delta.data());
+      so_poly_pointwise(oq.pp(), L, SO_SUB, av.data(), delta.data(), nullptr);
+    }
+    so_poly_mul_scalar(oq.pp(), L, av.data(), reps(p_inv).data());
+    CHECK(Flat(pa) == av);
+  }
+  std::printf("  BFV: ModReduceMsb, basis conversions, approximate modulus reductions: ok\n");
+}
+
 int main() {
   TestMontgomery();
   TestNttParameters();
@@ -539,6 +686,7 @@ int main() {
   TestSerialization();
   TestKeyGeneration(rlwe::PRNG_TYPE_CHACHA);
   TestKeyGeneration(rlwe::PRNG_TYPE_HKDF);
+  TestBfvOps();
   std::printf("rns_api_test: %d checks passed\n", g_checks);
   return 0;
 }

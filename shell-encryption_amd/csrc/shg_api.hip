@@ -733,32 +733,25 @@ int shg_substitute(shg_context* ctx, const uint64_t* in, uint64_t* out, int powe
   return e == hipSuccess ? SHG_OK : hip_fail(e, "substitute launch");
 }
 
-int shg_mod_reduce_lsb(shg_context* ctx, uint64_t* polys, uint64_t t, const uint64_t* host_ql_inv,
-                       int is_ntt, int count, int num_limbs, const int* primes) {
-  LimbPrimes lp{};
-  int log_n = 0;
-  int rc = check_primes(ctx, primes, num_limbs, &log_n, &lp);
-  if (rc) return rc;
-  if (num_limbs <= 1)
-    return fail(SHG_E_INVALID_ARGUMENT, "ModReduceLsb cannot apply with only one prime modulus.");
-  if (!host_ql_inv)
-    return fail(SHG_E_INVALID_ARGUMENT, "`ql_inv` must be defined with respect to %d moduli.",
-                num_limbs - 1);
-  if (t == 0) return fail(SHG_E_INVALID_ARGUMENT, "plaintext modulus must be positive.");
-  if (count <= 0) return count == 0 ? SHG_OK : fail(SHG_E_INVALID_ARGUMENT, "count must be non-negative.");
-  if (!aligned_ptr(polys)) return fail(SHG_E_INVALID_ARGUMENT, "polynomial data must not be null.");
+}  // extern "C"
+
+// ModReduceLsb and ModReduceMsb (rns/rns_polynomial.cc:262-417) share one
+// form: limb i <- (r * a_i + s * lift([k * a_l])) * q_l^-1 mod q_i, with
+// Lsb: k = (q_l - r) / t, r = q_l mod t, s = t;  Msb: k = 1, r = 1, s = -1.
+// r_mod_qi and s_mod_qi are Montgomery reps per kept limb.
+static int mod_reduce_impl(shg_context* ctx, uint64_t* polys, uint64_t k,
+                           const uint64_t* r_mod_qi, const uint64_t* s_mod_qi,
+                           const uint64_t* host_ql_inv, int is_ntt, int count, int num_limbs,
+                           const int* primes, const LimbPrimes& lp, int log_n) {
+  int rc = SHG_OK;
   const int nr = num_limbs - 1;
   const int last = primes[nr];
   const Params* Pl = ctx->primes[last]->mp.get();
-  const uint64_t r = Pl->modulus % t;
-  const uint64_t k = (Pl->modulus - r) / t;
   shg::ModReduceArgs args{};
-  args.t = t;
   args.k_mod_ql = ModInt::ImportInt(k, Pl).value().GetMontgomeryRepresentation();
   for (int i = 0; i < nr; ++i) {
-    const Params* Pi = ctx->primes[primes[i]]->mp.get();
-    args.r_mod_qi[i] = ModInt::ImportInt(r, Pi).value().GetMontgomeryRepresentation();
-    args.t_mod_qi[i] = ModInt::ImportInt(t, Pi).value().GetMontgomeryRepresentation();
+    args.r_mod_qi[i] = r_mod_qi[i];
+    args.t_mod_qi[i] = s_mod_qi[i];
     args.ql_inv[i] = host_ql_inv[i];
   }
   const size_t n = (size_t)1 << log_n;
@@ -815,6 +808,143 @@ int shg_mod_reduce_lsb(shg_context* ctx, uint64_t* polys, uint64_t t, const uint
     e = shg::launch_modreduce_combine(polys, delta, args, ctx->d_primes, lpr, count, num_limbs,
                                       log_n, s);
   return e == hipSuccess ? SHG_OK : hip_fail(e, "mod_reduce launch");
+}
+
+static int mod_reduce_checks(shg_context* ctx, uint64_t* polys, const uint64_t* host_ql_inv,
+                             int count, int num_limbs, const int* primes, const char* name,
+                             LimbPrimes* lp, int* log_n, bool* done) {
+  *done = true;
+  int rc = check_primes(ctx, primes, num_limbs, log_n, lp);
+  if (rc) return rc;
+  if (num_limbs <= 1)
+    return fail(SHG_E_INVALID_ARGUMENT, "%s cannot apply with only one prime modulus.", name);
+  if (!host_ql_inv)
+    return fail(SHG_E_INVALID_ARGUMENT, "`ql_inv` must be defined with respect to %d moduli.",
+                num_limbs - 1);
+  if (count <= 0) return count == 0 ? SHG_OK : fail(SHG_E_INVALID_ARGUMENT, "count must be non-negative.");
+  if (!aligned_ptr(polys)) return fail(SHG_E_INVALID_ARGUMENT, "polynomial data must not be null.");
+  *done = false;
+  return SHG_OK;
+}
+
+extern "C" {
+
+int shg_mod_reduce_lsb(shg_context* ctx, uint64_t* polys, uint64_t t, const uint64_t* host_ql_inv,
+                       int is_ntt, int count, int num_limbs, const int* primes) {
+  LimbPrimes lp{};
+  int log_n = 0;
+  bool done = true;
+  if (t == 0 && ctx) return fail(SHG_E_INVALID_ARGUMENT, "plaintext modulus must be positive.");
+  int rc = mod_reduce_checks(ctx, polys, host_ql_inv, count, num_limbs, primes, "ModReduceLsb",
+                             &lp, &log_n, &done);
+  if (rc || done) return rc;
+  // rns/rns_polynomial.cc:286-300: q_l = k * t + r.
+  const int nr = num_limbs - 1;
+  const uint64_t ql = ctx->primes[primes[nr]]->q;
+  const uint64_t r = ql % t;
+  const uint64_t k = (ql - r) / t;
+  uint64_t rq[SHG_MAX_LIMBS], tq[SHG_MAX_LIMBS];
+  for (int i = 0; i < nr; ++i) {
+    const Params* Pi = ctx->primes[primes[i]]->mp.get();
+    rq[i] = ModInt::ImportInt(r, Pi).value().GetMontgomeryRepresentation();
+    tq[i] = ModInt::ImportInt(t, Pi).value().GetMontgomeryRepresentation();
+  }
+  return mod_reduce_impl(ctx, polys, k, rq, tq, host_ql_inv, is_ntt, count, num_limbs, primes,
+                         lp, log_n);
+}
+
+int shg_mod_reduce_msb(shg_context* ctx, uint64_t* polys, const uint64_t* host_ql_inv, int is_ntt,
+                       int count, int num_limbs, const int* primes) {
+  LimbPrimes lp{};
+  int log_n = 0;
+  bool done = true;
+  int rc = mod_reduce_checks(ctx, polys, host_ql_inv, count, num_limbs, primes, "ModReduceMsb",
+                             &lp, &log_n, &done);
+  if (rc || done) return rc;
+  // (a - lift([a]_{q_l})) * q_l^-1: k = 1, r = 1, s = -1.
+  const int nr = num_limbs - 1;
+  uint64_t one[SHG_MAX_LIMBS], minus_one[SHG_MAX_LIMBS];
+  for (int i = 0; i < nr; ++i) {
+    const Params* Pi = ctx->primes[primes[i]]->mp.get();
+    one[i] = Pi->r_mod_modulus;
+    minus_one[i] = Pi->modulus - Pi->r_mod_modulus;
+  }
+  return mod_reduce_impl(ctx, polys, 1, one, minus_one, host_ql_inv, is_ntt, count, num_limbs,
+                         primes, lp, log_n);
+}
+
+int shg_switch_rns_basis(shg_context* ctx, int mode, const uint64_t* in, int in_is_ntt,
+                         int balanced, int count, int num_in, const int* in_primes, uint64_t* out,
+                         int num_out, const int* out_primes, const uint64_t* host_q_hat_inv_mod_qs,
+                         const uint64_t* host_matrix, const uint64_t* host_vec) {
+  LimbPrimes lin{}, lout{};
+  int log_n = 0, log_n_out = 0;
+  int rc = check_primes(ctx, in_primes, num_in, &log_n, &lin);
+  if (rc) return rc;
+  rc = check_primes(ctx, out_primes, num_out, &log_n_out, &lout);
+  if (rc) return rc;
+  if (log_n != log_n_out)
+    return fail(SHG_E_INVALID_ARGUMENT, "both RNS chains must share one ring degree.");
+  if (mode != SHG_BASIS_EXACT && mode != SHG_BASIS_SCALE && mode != SHG_BASIS_APPROX)
+    return fail(SHG_E_INVALID_ARGUMENT, "unknown basis conversion %d.", mode);
+  if (num_in > shg::kBasisMax || num_out > shg::kBasisMax)
+    return fail(SHG_E_UNSUPPORTED, "basis conversions take at most %d moduli per chain.",
+                shg::kBasisMax);
+  if (in_is_ntt && mode != SHG_BASIS_APPROX)
+    return fail(SHG_E_INVALID_ARGUMENT, "Polynomial must be in coefficient form.");
+  if (!host_q_hat_inv_mod_qs || !host_matrix || (mode != SHG_BASIS_APPROX && !host_vec))
+    return fail(SHG_E_INVALID_ARGUMENT, "the CRT constants must not be null.");
+  if (count <= 0) return count == 0 ? SHG_OK : fail(SHG_E_INVALID_ARGUMENT, "count must be non-negative.");
+  if (!aligned_ptr(in) || !aligned_ptr(out))
+    return fail(SHG_E_INVALID_ARGUMENT, "polynomial data must not be null.");
+  const size_t n = (size_t)1 << log_n;
+  if ((const void*)out < (const void*)(in + (size_t)count * num_in * n) &&
+      (const void*)in < (const void*)(out + (size_t)count * num_out * n))
+    return fail(SHG_E_INVALID_ARGUMENT, "`out` must not overlap the input.");
+  shg::BasisArgs ba{};
+  ba.mode = mode;
+  ba.nin = num_in;
+  ba.nout = num_out;
+  ba.balanced = mode == SHG_BASIS_APPROX && balanced ? 1 : 0;
+  for (int i = 0; i < num_in; ++i) {
+    ba.in_id[i] = in_primes[i];
+    ba.q_hat_inv[i] = host_q_hat_inv_mod_qs[i];
+    if (mode == SHG_BASIS_SCALE) ba.pre[i] = host_vec[i];
+  }
+  for (int j = 0; j < num_out; ++j) {
+    ba.out_id[j] = out_primes[j];
+    if (mode == SHG_BASIS_EXACT) ba.vec[j] = host_vec[j];
+  }
+  for (int i = 0; i < num_in; ++i)
+    for (int j = 0; j < num_out; ++j) {
+      if (mode == SHG_BASIS_SCALE) {
+        // -(q_i^-1 mod p_j): NegateInPlace of the reference (montgomery.h:390-399).
+        const uint64_t v = host_matrix[(size_t)j * num_in + i];
+        ba.mat[i * num_out + j] = ctx->primes[out_primes[j]]->q - v;
+      } else {
+        ba.mat[i * num_out + j] = host_matrix[(size_t)i * num_out + j];
+      }
+    }
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  const uint64_t* src = in;
+  hipError_t e = hipSuccess;
+  if (in_is_ntt) {
+    // The inverse transform commutes with the per-limb scaling by
+    // (Q/q_i)^-1, so the kernel reads coefficient-form input.
+    rc = ensure_scratch(ctx, (size_t)count * num_in * n * 8);
+    if (rc) return rc;
+    e = hipMemcpyAsync(ctx->scratch, in, (size_t)count * num_in * n * 8, hipMemcpyDeviceToDevice,
+                       ctx->stream);
+    if (e == hipSuccess)
+      e = shg::launch_ntt(true, resolve_variant(ctx, SHG_NTT_AUTO, in_primes, num_in, log_n,
+                                                (long long)count * num_in),
+                          (uint64_t*)ctx->scratch, ctx->d_primes, lin, count * num_in, num_in,
+                          log_n, shg::kMaxLogM, ctx->stream);
+    src = (const uint64_t*)ctx->scratch;
+  }
+  if (e == hipSuccess)
+    e = shg::launch_basis_switch(src, out, ba, ctx->d_primes, count, log_n, ctx->stream);
+  return e == hipSuccess ? SHG_OK : hip_fail(e, "basis conversion launch");
 }
 
 int shg_tensor_product(shg_context* ctx, const uint64_t* a, int len_a, const uint64_t* b, int len_b,

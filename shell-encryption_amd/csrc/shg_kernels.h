@@ -242,6 +242,26 @@ struct PadDesc {
 };
 hipError_t launch_chacha_pad(const PadDesc& pd, uint64_t* out, hipStream_t s);
 
+// RNS basis conversions (shg_bfv.hip): modes as SHG_BASIS_* of shell_gpu.h.
+constexpr int kBasisMax = 16;
+constexpr int kBasisExact = 0;
+constexpr int kBasisScale = 1;
+constexpr int kBasisApprox = 2;
+struct BasisArgs {
+  int mode;
+  int nin, nout;
+  int balanced;  // approximate conversion with the balanced lift
+  int in_id[kBasisMax], out_id[kBasisMax];
+  uint64_t q_hat_inv[kBasisMax];        // (Q/q_i)^-1 mod q_i
+  uint64_t pre[kBasisMax];              // scale: P mod q_i
+  uint64_t mat[kBasisMax * kBasisMax];  // [i * nout + j]: (Q/q_i) mod p_j, or -(q_i^-1) mod p_j
+  uint64_t vec[kBasisMax];              // exact: Q mod p_j
+};
+// Coefficient-form input (count polynomials of nin limbs) -> count
+// polynomials of nout limbs, coefficient form.
+hipError_t launch_basis_switch(const uint64_t* in, uint64_t* out, const BasisArgs& ba,
+                               const PrimeDev* primes, int count, int logn, hipStream_t s);
+
 // Oblivious expansion and the inner product against a database (shg_pir.hip).
 hipError_t launch_expand_substitute(const uint64_t* const* cts, int count, int m, uint64_t* S,
                                     int power, int logn, int num_limbs, hipStream_t s);

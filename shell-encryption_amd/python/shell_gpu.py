@@ -20,6 +20,7 @@ OP_ADD, OP_SUB, OP_MUL, OP_NEG, OP_FMA = 0, 1, 2, 3, 4
 BATCH_NTT_FWD, BATCH_NTT_INV, BATCH_MUL, BATCH_MOD_REDUCE = 0, 1, 2, 3
 NTT_AUTO, NTT_INT, NTT_F64 = 0, 1, 2
 NTT_TILING_ALT, NTT_PREFETCH, NTT_SPLIT_LIMB, NTT_WAVE_EXCHANGE = 4, 8, 16, 32
+BASIS_EXACT, BASIS_SCALE, BASIS_APPROX = 0, 1, 2
 
 _lib = None
 
@@ -75,6 +76,8 @@ def lib():
         "shg_substitute": [VP, VP, VP, i, i, i, INTP],
         "shg_mod_reduce_lsb": [VP, VP, ctypes.c_uint64, U64P, i, i, i, INTP],
         "shg_tensor_product": [VP, VP, i, VP, i, VP, i, i, INTP],
+        "shg_mod_reduce_msb": [VP, VP, U64P, i, i, i, INTP],
+        "shg_switch_rns_basis": [VP, i, VP, i, i, i, i, INTP, VP, i, INTP, U64P, U64P, U64P],
         "shg_relin_apply": [VP, VP, i, i, VP, VP, i, INTP, VP, VP, i, i, INTP],
         "shg_relin_apply_noalloc": [VP, VP, i, i, VP, VP, i, INTP, VP, VP, i, i, INTP],
         "shg_galois_apply": [VP, VP, i, VP, VP, INTP, VP, VP, i, i, INTP],
@@ -286,6 +289,22 @@ class Context:
         _check(lib().shg_mod_reduce_lsb(self.h, x.p, int(t), qi.ctypes.data_as(U64P),
                                         int(bool(is_ntt)), int(count), len(primes),
                                         _ints(primes)))
+
+    def mod_reduce_msb(self, x, ql_inv, is_ntt, count, primes):
+        qi = np.ascontiguousarray(ql_inv, dtype=np.uint64)
+        _check(lib().shg_mod_reduce_msb(self.h, x.p, qi.ctypes.data_as(U64P), int(bool(is_ntt)),
+                                        int(count), len(primes), _ints(primes)))
+
+    def switch_rns_basis(self, mode, src, in_is_ntt, balanced, count, in_primes, out, out_primes,
+                         q_hat_inv, matrix, vec=None):
+        qh = np.ascontiguousarray(q_hat_inv, dtype=np.uint64)
+        m = np.ascontiguousarray(matrix, dtype=np.uint64)
+        v = np.ascontiguousarray(vec if vec is not None else np.zeros(1), dtype=np.uint64)
+        _check(lib().shg_switch_rns_basis(self.h, int(mode), src.p, int(bool(in_is_ntt)),
+                                          int(bool(balanced)), int(count), len(in_primes),
+                                          _ints(in_primes), out.p, len(out_primes),
+                                          _ints(out_primes), qh.ctypes.data_as(U64P),
+                                          m.ctypes.data_as(U64P), v.ctypes.data_as(U64P)))
 
     def tensor_product(self, a, len_a, b, len_b, out, count, primes):
         _check(lib().shg_tensor_product(self.h, a.p, int(len_a), b.p, int(len_b), out.p,
