@@ -251,6 +251,17 @@ int shg_lazy_fma(shg_context* ctx, const uint64_t* as, size_t as_stride,
                  const uint64_t* bs, size_t bs_stride, int num_terms,
                  uint64_t* out, int num_limbs, const int* primes);
 
+/* LazyRnsPolynomial::FusedMulSumAddInPlace / FusedMulDifferenceAddInPlace
+ * (rns/lazy_rns_polynomial.cc:170-254): x <- x + (a + b) * c, or with
+ * subtract != 0 x <- x + (a - b) * c (Montgomery products, NTT form), count
+ * polynomials of num_limbs limbs laid out back to back.  x stays canonical
+ * after every call, which leaves the residue the reference's 128-bit lazy
+ * sum exports.  Moduli of 2^60 and above are refused with the reference's
+ * message (:73-78). */
+int shg_fused_mul_sum_add(shg_context* ctx, int subtract, uint64_t* x, const uint64_t* a,
+                          const uint64_t* b, const uint64_t* c, int count, int num_limbs,
+                          const int* primes);
+
 /* ---- wire formats --------------------------------------------------------
  * RnsPolynomial::Serialize / Deserialize (rns/rns_polynomial.h:99-140) store
  * one byte string per limb, MontgomeryInt::SerializeVector /

@@ -102,6 +102,8 @@ def lib():
         L.so_lazy_log_max_level.argtypes = [MP]
         L.so_lazy_log_max_level.restype = ctypes.c_int
         L.so_lazy_fma.argtypes = [PP, ctypes.c_int, ctypes.c_int, U64P, U64P, U64P]
+        L.so_lazy_mixed.argtypes = [PP, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                    U64P, U64P, U64P, U64P]
         L.so_fill_uniform.argtypes = [PP, ctypes.c_int, u64, U64P]
         L.so_now_seconds.restype = ctypes.c_double
         U8P = ctypes.POINTER(ctypes.c_uint8)
@@ -535,6 +537,23 @@ class Rns:
         bs = np.ascontiguousarray(bs, dtype=np.uint64)
         out = np.zeros((self.L, self.n), dtype=np.uint64)
         lib().so_lazy_fma(self._arr, self.L, as_.shape[0], _p64(as_), _p64(bs), _p64(out))
+        return out
+
+    def lazy_mixed(self, terms):
+        """LazyRnsPolynomial from CreateZero through a list of updates, then
+        Export (rns/lazy_rns_polynomial.h:44-115, .cc:89-254): each term is
+        ("fma", a, b), ("sum", a, b, c) or ("diff", a, b, c) with (L, n)
+        Montgomery arrays."""
+        kinds = {"fma": 0, "sum": 1, "diff": 2}
+        T = len(terms)
+        zero = np.zeros((self.L, self.n), dtype=np.uint64)
+        ks = (ctypes.c_int * max(T, 1))(*[kinds[t[0]] for t in terms])
+        pick = lambda k: np.ascontiguousarray(
+            [np.asarray(t[k], dtype=np.uint64) if len(t) > k else zero for t in terms],
+            dtype=np.uint64).reshape(T, self.L, self.n)
+        as_, bs, cs = pick(1), pick(2), pick(3)
+        out = np.zeros((self.L, self.n), dtype=np.uint64)
+        lib().so_lazy_mixed(self._arr, self.L, T, ks, _p64(as_), _p64(bs), _p64(cs), _p64(out))
         return out
 
     # ---- RnsContext constants (rns/rns_context.cc:169-282) ----

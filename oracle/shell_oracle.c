@@ -680,6 +680,62 @@ void so_lazy_fma(so_prime* const* primes, int nl, int num_terms,
   free(acc);
 }
 
+/* LazyRnsPolynomial over mixed updates (rns/lazy_rns_polynomial.h:44-115,
+ * rns/lazy_rns_polynomial.cc:89-254): starting from CreateZero, term t adds
+ * the integer product a b (kind 0, FusedMulAddInPlace), (a + b) c (kind 1,
+ * FusedMulSumAddInPlace) or (a + q_i - b) c (kind 2,
+ * FusedMulDifferenceAddInPlace) of the Montgomery representations to a
+ * 128-bit sum, refreshing (BarrettReduceBigInt) when the level reaches
+ * 2^log_max_level; Export reduces and applies ExportInt. */
+void so_lazy_mixed(so_prime* const* primes, int nl, int num_terms, const int* kinds,
+                   const uint64_t* as, const uint64_t* bs, const uint64_t* cs, uint64_t* out) {
+  const size_t n = (size_t)1 << primes[0]->log_n;
+  const size_t poly = (size_t)nl * n;
+  int log_max = 62;
+  for (int i = 0; i < nl; i++) {
+    int l = so_lazy_log_max_level(&primes[i]->mp);
+    if (l < log_max) log_max = l;
+  }
+  const uint64_t max_level = (uint64_t)1 << log_max;
+  u128* acc = (u128*)calloc(poly, sizeof(u128));
+  uint64_t level = 0;
+  for (int t = 0; t < num_terms; t++) {
+    if (level == max_level) {
+      for (int i = 0; i < nl; i++)
+        for (size_t j = 0; j < n; j++) {
+          u128 v = acc[i * n + j];
+          acc[i * n + j] = so_barrett_reduce_big(&primes[i]->mp, (uint64_t)v,
+                                                 (uint64_t)(v >> 64));
+        }
+      level = 1;
+    }
+    const uint64_t* a = as + (size_t)t * poly;
+    const uint64_t* b = bs + (size_t)t * poly;
+    const uint64_t* c = cs + (size_t)t * poly;
+    for (int i = 0; i < nl; i++) {
+      const uint64_t q = primes[i]->mp.modulus;
+      for (size_t j = i * n; j < (i + 1) * n; j++) {
+        if (kinds[t] == 0)
+          acc[j] += (u128)a[j] * b[j];
+        else if (kinds[t] == 1)
+          acc[j] += (u128)(a[j] + b[j]) * c[j];
+        else
+          acc[j] += (u128)(a[j] + q - b[j]) * c[j];
+      }
+    }
+    level++;
+  }
+  for (int i = 0; i < nl; i++) {
+    const so_mont* p = &primes[i]->mp;
+    for (size_t j = 0; j < n; j++) {
+      u128 v = acc[i * n + j];
+      out[i * n + j] = so_export(p, so_barrett_reduce_big(p, (uint64_t)v,
+                                                          (uint64_t)(v >> 64)));
+    }
+  }
+  free(acc);
+}
+
 /* ------------------------------------------------------------------------ */
 /* Deterministic inputs                                                      */
 /* ------------------------------------------------------------------------ */

@@ -124,6 +124,8 @@ void so_tensor_product(so_prime* const* primes, int nl, int len_a,
 int so_lazy_log_max_level(const so_mont* p);
 void so_lazy_fma(so_prime* const* primes, int nl, int num_terms,
                  const uint64_t* as, const uint64_t* bs, uint64_t* out);
+void so_lazy_mixed(so_prime* const* primes, int nl, int num_terms, const int* kinds,
+                   const uint64_t* as, const uint64_t* bs, const uint64_t* cs, uint64_t* out);
 
 /* Deterministic generator used by the tests and the bench (SplitMix64). */
 uint64_t so_splitmix64(uint64_t* state);

@@ -1,13 +1,15 @@
 // RnsRlweCiphertext (rns/rns_ciphertext.h:44-631) and RnsBgvCiphertext
 // (rns/rns_bgv_ciphertext.h:35-175): the ciphertext operations on the hot
-// path (tensor product, modulus switch, substitution), each a batched
-// libshellgpu launch over the components.
+// path (tensor product, modulus switch, substitution, the absorb updates of
+// the "b" part, eager and lazy), each a batched libshellgpu launch over the
+// components.
 #ifndef SHG_RLWE_RNS_CIPHERTEXT_H_
 #define SHG_RLWE_RNS_CIPHERTEXT_H_
 
 #include <string>
 #include <vector>
 
+#include "rlwe/lazy_rns_polynomial.h"
 #include "rlwe/rns_gadget.h"
 #include "rlwe/rns_polynomial.h"
 
@@ -63,6 +65,110 @@ class RnsRlweCiphertext {
       result.push_back(RnsPolynomial<ModularInt>::FromDevice(LogN(), L, true, out.data() + m * words));
     return RnsRlweCiphertext(std::move(result), moduli_, power_of_s_, error_ * that.error_,
                              error_params_);
+  }
+
+  // rns/rns_ciphertext.h:309-330: this[0] += ctxt[0] * ptxt, the "a" part
+  // left alone (set later with SetPadComponent).
+  Status FusedAbsorbAddInPlaceWithoutPad(const RnsRlweCiphertext& ctxt,
+                                         const RnsPolynomial<ModularInt>& ptxt) {
+    if (Degree() != 1 || Degree() != ctxt.Degree())
+      return InvalidArgumentError(
+          "`FusedAbsorbAddInPlaceWithoutPad only applies to degree 1 ciphertext.");
+    RLWE_RETURN_IF_ERROR(CheckAbsorbOperand(ctxt));
+    RLWE_RETURN_IF_ERROR(components_[0].FusedMulAddInPlace(ctxt.components_[0], ptxt, moduli_));
+    error_ += ctxt.error_ * error_params_->B_plaintext();
+    return OkStatus();
+  }
+
+  // rns/rns_ciphertext.h:338-368: the same into a lazy "b" part, which
+  // MergeLazyOperations folds back into the components.
+  Status FusedAbsorbAddInPlaceWithoutPadLazily(const RnsRlweCiphertext& ctxt,
+                                               const RnsPolynomial<ModularInt>& ptxt) {
+    if (ctxt.Degree() != 1)
+      return InvalidArgumentError(
+          "`FusedAbsorbAddInPlaceWithoutPad only applies to degree 1 ciphertext.");
+    RLWE_RETURN_IF_ERROR(CheckAbsorbOperand(ctxt));
+    if (lazy_components_.empty()) {
+      RLWE_ASSIGN_OR_RETURN(auto lazy, LazyRnsPolynomial<ModularInt>::Create(
+                                           ctxt.components_[0], ptxt, Moduli()));
+      lazy_components_.push_back(std::move(lazy));
+    } else {
+      RLWE_RETURN_IF_ERROR(
+          lazy_components_[0].FusedMulAddInPlace(ctxt.components_[0], ptxt, Moduli()));
+    }
+    error_ += ctxt.error_ * error_params_->B_plaintext();
+    return OkStatus();
+  }
+
+  // rns/rns_ciphertext.h:370-406: (ctxt1[0] + ctxt2[0]) * ptxt, lazily.
+  Status FusedAbsorbSumAddInPlaceWithoutPadLazily(const RnsRlweCiphertext& ctxt1,
+                                                  const RnsRlweCiphertext& ctxt2,
+                                                  const RnsPolynomial<ModularInt>& ptxt) {
+    if (ctxt1.Degree() != 1 || ctxt2.Degree() != 1)
+      return InvalidArgumentError(
+          "`FusedAbsorbSumAddInPlaceWithoutPad only applies to degree 1 ciphertext.");
+    RLWE_RETURN_IF_ERROR(CheckAbsorbOperands(ctxt1, ctxt2));
+    if (lazy_components_.empty()) {
+      RLWE_ASSIGN_OR_RETURN(auto sum, ctxt1.components_[0].Add(ctxt2.components_[0], Moduli()));
+      RLWE_ASSIGN_OR_RETURN(auto lazy, LazyRnsPolynomial<ModularInt>::Create(sum, ptxt, Moduli()));
+      lazy_components_.push_back(std::move(lazy));
+    } else {
+      RLWE_RETURN_IF_ERROR(lazy_components_[0].FusedMulSumAddInPlace(
+          ctxt1.components_[0], ctxt2.components_[0], ptxt, Moduli()));
+    }
+    error_ += (ctxt1.error_ + ctxt2.error_) * error_params_->B_plaintext();
+    return OkStatus();
+  }
+
+  // rns/rns_ciphertext.h:408-443: (ctxt1[0] - ctxt2[0]) * ptxt, lazily.  As
+  // in the reference, the call that creates the lazy part leaves the error
+  // unchanged.
+  Status FusedAbsorbDiffAddInPlaceWithoutPadLazily(const RnsRlweCiphertext& ctxt1,
+                                                   const RnsRlweCiphertext& ctxt2,
+                                                   const RnsPolynomial<ModularInt>& ptxt) {
+    if (ctxt1.Degree() != 1 || ctxt2.Degree() != 1)
+      return InvalidArgumentError(
+          "`FusedAbsorbDiffAddInPlaceWithoutPad only applies to degree 1 ciphertext.");
+    RLWE_RETURN_IF_ERROR(CheckAbsorbOperands(ctxt1, ctxt2));
+    if (lazy_components_.empty()) {
+      RLWE_ASSIGN_OR_RETURN(auto diff, ctxt1.components_[0].Sub(ctxt2.components_[0], Moduli()));
+      RLWE_ASSIGN_OR_RETURN(auto lazy,
+                            LazyRnsPolynomial<ModularInt>::Create(diff, ptxt, Moduli()));
+      lazy_components_.push_back(std::move(lazy));
+      return OkStatus();
+    }
+    RLWE_RETURN_IF_ERROR(lazy_components_[0].FusedMulDifferenceAddInPlace(
+        ctxt1.components_[0], ctxt2.components_[0], ptxt, Moduli()));
+    error_ += (ctxt1.error_ + ctxt2.error_) * error_params_->B_plaintext();
+    return OkStatus();
+  }
+
+  // rns/rns_ciphertext.h:446-479.
+  Status MergeLazyOperations() {
+    if (lazy_components_.empty())
+      return FailedPreconditionError("The ciphertext was not in lazy mode.");
+    for (size_t i = 0; i < lazy_components_.size() && i < components_.size(); ++i) {
+      RLWE_ASSIGN_OR_RETURN(RnsPolynomial<ModularInt> ci, lazy_components_[i].Export(Moduli()));
+      RLWE_RETURN_IF_ERROR(components_[i].AddInPlace(ci, moduli_));
+    }
+    for (size_t i = components_.size(); i < lazy_components_.size(); ++i) {
+      RLWE_ASSIGN_OR_RETURN(RnsPolynomial<ModularInt> ci, lazy_components_[i].Export(Moduli()));
+      components_.push_back(std::move(ci));
+    }
+    lazy_components_.clear();
+    return OkStatus();
+  }
+
+  // rns/rns_ciphertext.h:483-495: the "a" part; a ciphertext of degree 0
+  // (lazy absorbs into an empty ciphertext) gains it as a new component.
+  Status SetPadComponent(RnsPolynomial<ModularInt> new_a) {
+    if (Degree() > 1)
+      return InvalidArgumentError("`SetPadComponent` only applies to degree 1 ciphertext.");
+    if (Degree() == 1)
+      components_[1] = std::move(new_a);
+    else
+      components_.push_back(std::move(new_a));
+    return OkStatus();
   }
 
   // rns/rns_ciphertext.h:70-98: each component through
@@ -123,7 +229,22 @@ class RnsRlweCiphertext {
   ModuliVec& moduli() { return moduli_; }
 
  private:
+  Status CheckAbsorbOperand(const RnsRlweCiphertext& ctxt) const {
+    if (Level() != ctxt.Level()) return InvalidArgumentError("`ctxt` has a mismatched level.");
+    if (PowerOfS() != ctxt.PowerOfS())
+      return InvalidArgumentError("`ctxt` has a mismatched key power.");
+    return OkStatus();
+  }
+  Status CheckAbsorbOperands(const RnsRlweCiphertext& c1, const RnsRlweCiphertext& c2) const {
+    if (Level() != c1.Level() || Level() != c2.Level())
+      return InvalidArgumentError("`ctxt` has a mismatched level.");
+    if (PowerOfS() != c1.PowerOfS() || PowerOfS() != c2.PowerOfS())
+      return InvalidArgumentError("`ctxt` has a mismatched key power.");
+    return OkStatus();
+  }
+
   std::vector<RnsPolynomial<ModularInt>> components_;
+  std::vector<LazyRnsPolynomial<ModularInt>> lazy_components_;
   ModuliVec moduli_;
   const RnsErrorParams<ModularInt>* error_params_;
   int power_of_s_;

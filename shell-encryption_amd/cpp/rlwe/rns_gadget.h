@@ -117,6 +117,20 @@ class RnsErrorParams {
   double B_secretkey_encryption() const {
     return ExportDoubleT() * std::sqrt(1 << log_n_) * (std::sqrt(3.0) + 6.0 * sigma_);
   }
+  // rns/rns_error_params.h:84-87, 129-172: the size of an NTT coefficient
+  // of a random plaintext (the factor an absorb multiplies the error by), of
+  // a public-key encryption, and of the rounding term of a modulus switch.
+  double B_plaintext() const { return ExportDoubleT() * std::sqrt(3.0 * (1 << log_n_)); }
+  double B_publickey_encryption() const {
+    const double dimension = 1 << log_n_;
+    return ExportDoubleT() * (std::sqrt(dimension) * (6 * sigma_ + std::sqrt(3.0)) +
+                              72 * dimension * sigma_ * sigma_);
+  }
+  double B_scale() const {
+    const double dimension = 1 << log_n_;
+    return ExportDoubleT() *
+           (std::sqrt(3.0 * dimension) + 8.0 * dimension * std::sqrt(1 / 3.0));
+  }
   // As in the reference: the square of ceil(log2 t).
   double ExportDoubleT() const { return std::pow(log_plaintext_modulus_, 2); }
 

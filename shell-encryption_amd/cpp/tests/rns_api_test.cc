@@ -3,6 +3,7 @@
 // the CPU oracle (oracle/shell_oracle.c, linked into this test binary only).
 //
 // Run: rns_api_test  (needs an MI355X; exits non-zero on the first failure)
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -64,10 +65,10 @@ static bool IsPrime(uint64_t n) {
   return true;
 }
 
-// The `count` largest primes below 2^50 with q = 1 mod 2^(log_n+1).
-static std::vector<uint64_t> NttPrimes(int log_n, int count) {
+// The `count` largest primes below 2^log_bound with q = 1 mod 2^(log_n+1).
+static std::vector<uint64_t> NttPrimes(int log_n, int count, int log_bound = 50) {
   const uint64_t step = 1ull << (log_n + 1);
-  uint64_t q = ((1ull << 50) - 1) / step * step + 1;
+  uint64_t q = ((1ull << log_bound) - 1) / step * step + 1;
   std::vector<uint64_t> out;
   while ((int)out.size() < count) {
     if (IsPrime(q)) out.push_back(q);
@@ -676,6 +677,131 @@ static void TestBfvOps() {
   std::printf("  BFV: ModReduceMsb, basis conversions, approximate modulus reductions: ok\n");
 }
 
+// rns/rns_ciphertext.h:309-495 (FusedAbsorb*WithoutPad*, MergeLazyOperations,
+// SetPadComponent) over rns/lazy_rns_polynomial.cc:89-254: lazy sums,
+// differences and products of "b" parts against the oracle's 128-bit lazy
+// sum, the eager absorb against the lazy one, and the error bookkeeping.
+static void TestLazyAbsorb(int log_n, int L) {
+  using Rlwe = rlwe::RnsRlweCiphertext<ModularInt>;
+  const int n = 1 << log_n;
+  auto qs = NttPrimes(log_n, L);
+  auto ctx = rlwe::RnsContext<ModularInt>::CreateForBgv(log_n, qs, 65537).value();
+  auto moduli = ctx.MainPrimeModuli();
+  OracleChain oc(qs, log_n);
+  auto error_params = rlwe::RnsErrorParams<ModularInt>::Create(log_n, moduli, {}, 17, 4.0).value();
+  const double bp = error_params.B_plaintext();
+  CHECK(bp == 17.0 * 17.0 * std::sqrt(3.0 * n));
+  auto make = [&](uint64_t seed, double err, int len) {
+    std::vector<Poly> comps;
+    for (int i = 0; i < len; ++i) comps.push_back(FromFlat(Uniform(oc, L, seed + i), L, n, true));
+    return Rlwe(std::move(comps), moduli, 1, err, &error_params);
+  };
+  const int T = 7;
+  const size_t words = (size_t)L * n;
+  std::vector<Rlwe> c1s, c2s;
+  std::vector<Poly> ptxts;
+  std::vector<uint64_t> as(T * words), bs(T * words), cs(T * words);
+  std::vector<int> kinds(T);
+  for (int t = 0; t < T; ++t) {
+    c1s.push_back(make(3000 + 10 * t, 1.0 + t, 2));
+    c2s.push_back(make(3005 + 10 * t, 0.5 + t, 2));
+    auto pt = Uniform(oc, L, 4000 + t);
+    ptxts.push_back(FromFlat(pt, L, n, true));
+    auto b1 = Flat(c1s[t].Components()[0]), b2 = Flat(c2s[t].Components()[0]);
+    kinds[t] = t % 3;
+    std::copy(b1.begin(), b1.end(), as.begin() + t * words);
+    std::copy((kinds[t] ? b2 : pt).begin(), (kinds[t] ? b2 : pt).end(), bs.begin() + t * words);
+    std::copy(pt.begin(), pt.end(), cs.begin() + t * words);
+  }
+  Rlwe acc = make(5000, 2.0, 2);
+  const auto b0 = Flat(acc.Components()[0]), a0 = Flat(acc.Components()[1]);
+  auto st = acc.MergeLazyOperations();
+  CHECK(!st.ok() && st.message() == "The ciphertext was not in lazy mode.");
+  double err = 2.0;
+  for (int t = 0; t < T; ++t) {
+    if (kinds[t] == 0) {
+      CHECK_OK(acc.FusedAbsorbAddInPlaceWithoutPadLazily(c1s[t], ptxts[t]));
+      err += c1s[t].Error() * bp;
+    } else if (kinds[t] == 1) {
+      CHECK_OK(acc.FusedAbsorbSumAddInPlaceWithoutPadLazily(c1s[t], c2s[t], ptxts[t]));
+      err += (c1s[t].Error() + c2s[t].Error()) * bp;
+    } else {
+      CHECK_OK(acc.FusedAbsorbDiffAddInPlaceWithoutPadLazily(c1s[t], c2s[t], ptxts[t]));
+      err += (c1s[t].Error() + c2s[t].Error()) * bp;
+    }
+  }
+  CHECK(Flat(acc.Components()[0]) == b0);  // the lazy part is not merged yet
+  CHECK_OK(acc.MergeLazyOperations());
+  std::vector<uint64_t> lazy(words);
+  so_lazy_mixed(oc.pp(), L, T, kinds.data(), as.data(), bs.data(), cs.data(), lazy.data());
+  std::vector<uint64_t> expect(words);
+  for (size_t j = 0; j < words; ++j) expect[j] = so_add(&oc.primes[j / n]->mp, b0[j], lazy[j]);
+  CHECK(Flat(acc.Components()[0]) == expect);
+  CHECK(Flat(acc.Components()[1]) == a0);
+  CHECK(acc.Error() == err);
+  CHECK(!acc.MergeLazyOperations().ok());
+
+  // A difference first: the call that creates the lazy part leaves the error.
+  Rlwe d = make(5100, 3.0, 2);
+  CHECK_OK(d.FusedAbsorbDiffAddInPlaceWithoutPadLazily(c1s[0], c2s[0], ptxts[0]));
+  CHECK(d.Error() == 3.0);
+
+  // Eager and lazy products agree.
+  Rlwe eager = make(6000, 1.0, 2), lz = make(6000, 1.0, 2);
+  for (int t = 0; t < 3; ++t) {
+    CHECK_OK(eager.FusedAbsorbAddInPlaceWithoutPad(c1s[t], ptxts[t]));
+    CHECK_OK(lz.FusedAbsorbAddInPlaceWithoutPadLazily(c1s[t], ptxts[t]));
+  }
+  CHECK_OK(lz.MergeLazyOperations());
+  CHECK(eager.Components()[0] == lz.Components()[0] && eager.Components()[1] == lz.Components()[1]);
+  CHECK(eager.Error() == lz.Error());
+
+  // Lazy absorbs into a ciphertext with no components, then the pad.
+  Rlwe z(std::vector<Poly>{}, moduli, 1, 0.0, &error_params);
+  CHECK_OK(z.FusedAbsorbAddInPlaceWithoutPadLazily(c1s[0], ptxts[0]));
+  CHECK_OK(z.MergeLazyOperations());
+  CHECK(z.Degree() == 0);
+  std::vector<uint64_t> e0(words);
+  so_lazy_fma(oc.pp(), L, 1, as.data(), bs.data(), e0.data());
+  CHECK(Flat(z.Components()[0]) == e0);
+  CHECK_OK(z.SetPadComponent(c2s[0].Components()[1]));
+  CHECK(z.Degree() == 1 && z.Components()[1] == c2s[0].Components()[1]);
+  CHECK_OK(z.SetPadComponent(c2s[1].Components()[1]));
+  CHECK(z.Degree() == 1 && z.Components()[1] == c2s[1].Components()[1]);
+
+  // Reference error messages.
+  Rlwe deg2 = make(7000, 1.0, 3);
+  st = deg2.FusedAbsorbAddInPlaceWithoutPad(c1s[0], ptxts[0]);
+  CHECK(!st.ok() && st.message() == "`FusedAbsorbAddInPlaceWithoutPad only applies to degree 1 ciphertext.");
+  st = acc.FusedAbsorbAddInPlaceWithoutPadLazily(deg2, ptxts[0]);
+  CHECK(!st.ok() && st.message() == "`FusedAbsorbAddInPlaceWithoutPad only applies to degree 1 ciphertext.");
+  st = acc.FusedAbsorbSumAddInPlaceWithoutPadLazily(c1s[0], deg2, ptxts[0]);
+  CHECK(!st.ok() && st.message() == "`FusedAbsorbSumAddInPlaceWithoutPad only applies to degree 1 ciphertext.");
+  CHECK(!deg2.SetPadComponent(ptxts[0]).ok());
+  Rlwe other_power(std::vector<Poly>{c1s[0].Components()[0], c1s[0].Components()[1]}, moduli, 3,
+                   1.0, &error_params);
+  st = acc.FusedAbsorbAddInPlaceWithoutPad(other_power, ptxts[0]);
+  CHECK(!st.ok() && st.message() == "`ctxt` has a mismatched key power.");
+  auto coeff = ptxts[0];
+  CHECK_OK(coeff.ConvertToCoeffForm(moduli));
+  auto lz_bad = rlwe::LazyRnsPolynomial<ModularInt>::Create(coeff, ptxts[1], moduli);
+  CHECK(!lz_bad.ok() && lz_bad.status().message() == "Polynomials `a` and `b` must be in the NTT form.");
+  auto lzero = rlwe::LazyRnsPolynomial<ModularInt>::CreateZero(log_n, moduli).value();
+  CHECK(Flat(lzero.Export(moduli).value()) == std::vector<uint64_t>(words, 0));
+  CHECK(!lzero.Export(rlwe::ModuliSpan<ModularInt>(moduli.data(), L - 1)).ok());
+
+  // Moduli of 2^60 and above: no fused multiply-sum-add.
+  auto wide = NttPrimes(log_n, 1, 62);
+  auto wctx = rlwe::RnsContext<ModularInt>::CreateForBgv(log_n, wide, 65537).value();
+  auto wm = wctx.MainPrimeModuli();
+  OracleChain woc(wide, log_n);
+  auto wp = FromFlat(Uniform(woc, 1, 9), 1, n, true);
+  auto wl = rlwe::LazyRnsPolynomial<ModularInt>::Create(wp, wp, wm).value();
+  st = wl.FusedMulSumAddInPlace(wp, wp, wp, wm);
+  CHECK(!st.ok() && st.message() == "Modulus is too large to perform fused multiply-sum-add.");
+  CHECK_OK(wl.FusedMulAddInPlace(wp, wp, wm));
+}
+
 int main() {
   TestMontgomery();
   TestNttParameters();
@@ -687,6 +813,8 @@ int main() {
   TestKeyGeneration(rlwe::PRNG_TYPE_CHACHA);
   TestKeyGeneration(rlwe::PRNG_TYPE_HKDF);
   TestBfvOps();
+  TestLazyAbsorb(10, 3);
+  TestLazyAbsorb(14, 8);
   std::printf("rns_api_test: %d checks passed\n", g_checks);
   return 0;
 }

@@ -683,6 +683,28 @@ int shg_pointwise(shg_context* ctx, int op, uint64_t* x, const uint64_t* a, size
   return e == hipSuccess ? SHG_OK : hip_fail(e, "pointwise launch");
 }
 
+int shg_fused_mul_sum_add(shg_context* ctx, int subtract, uint64_t* x, const uint64_t* a,
+                          const uint64_t* b, const uint64_t* c, int count, int num_limbs,
+                          const int* primes) {
+  LimbPrimes lp{};
+  int log_n = 0;
+  int rc = check_primes(ctx, primes, num_limbs, &log_n, &lp);
+  if (rc) return rc;
+  if (count <= 0) return count == 0 ? SHG_OK : fail(SHG_E_INVALID_ARGUMENT, "count must be non-negative.");
+  if (!aligned_ptr(x)) return fail(SHG_E_INVALID_ARGUMENT, "`this` must not be null.");
+  if (!aligned_ptr(a) || !aligned_ptr(b) || !aligned_ptr(c))
+    return fail(SHG_E_INVALID_ARGUMENT, "Polynomials `a`, `b`, and `c` must not be null.");
+  // rns/lazy_rns_polynomial.cc:73-78: a + b (or a + q - b) and its product
+  // with c must fit the reference's 128-bit lazy sum.
+  for (int i = 0; i < num_limbs; ++i)
+    if (ctx->primes[primes[i]]->q >> 60)
+      return fail(SHG_E_INVALID_ARGUMENT, "Modulus is too large to perform fused multiply-sum-add.");
+  ProfScope prof(ctx, "pointwise");
+  hipError_t e = shg::launch_fused_mul_sum_add(subtract != 0, x, a, b, c, ctx->d_primes, lp, count,
+                                               num_limbs, log_n, ctx->stream);
+  return e == hipSuccess ? SHG_OK : hip_fail(e, "fused multiply-sum-add launch");
+}
+
 int shg_mul_scalar(shg_context* ctx, uint64_t* x, const uint64_t* host_scalars, int count,
                    int num_limbs, const int* primes) {
   LimbPrimes lp{};

@@ -41,6 +41,10 @@ hipError_t launch_pointwise(int op, uint64_t* x, const uint64_t* a, size_t a_str
                             const uint64_t* b, size_t b_stride, const PrimeDev* primes,
                             const LimbPrimes& lp, int count, int num_limbs, int logn,
                             hipStream_t s);
+hipError_t launch_fused_mul_sum_add(bool subtract, uint64_t* x, const uint64_t* a,
+                                    const uint64_t* b, const uint64_t* c, const PrimeDev* primes,
+                                    const LimbPrimes& lp, int count, int num_limbs, int logn,
+                                    hipStream_t s);
 // x[limb] *= scalars[limb] (Montgomery reps; REDC product).
 hipError_t launch_mul_scalar(uint64_t* x, const LimbPrimes& scalars_lo,
                              const LimbPrimes& scalars_hi, const PrimeDev* primes,

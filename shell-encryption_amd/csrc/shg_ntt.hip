@@ -329,9 +329,10 @@ __global__ void __launch_bounds__(64 << LW)
 // consecutive words read with two 16-byte loads), two exchanges inside each
 // wave lead to pass 2's layout and one workgroup-wide exchange to pass 1's,
 // whose words are scaled and stored coalesced.  The next limb is loaded
-// while the current one is transformed.  Barriers: one before the first
-// in-wave write of a limb (other waves may still read the previous limb's
-// workgroup-wide exchange there) and one inside that exchange.
+// while the current one is transformed (from the second pass on).
+// Barriers: one before the first in-wave write of a limb (other waves may
+// still read the previous limb's workgroup-wide exchange there) and one
+// inside that exchange.
 template <int LW, class A>
 __global__ void __launch_bounds__(64 << LW)
     ntt_inv_wave_kernel(uint64_t* __restrict__ data, const PrimeDev* __restrict__ primes,
@@ -374,13 +375,18 @@ __global__ void __launch_bounds__(64 << LW)
     double x[16];
 #pragma unroll
     for (int e = 0; e < 16; ++e) x[e] = ar.load(pf[e]);
-    const int next = item + gridDim.x;
-    if (next < items) load4(data + ((size_t)next << LOGM), tp4, pf);
     wave_pass_inv<LW, 4, 1, 0>(x, tw1, ar);
     __builtin_amdgcn_sched_barrier(0);
     const int tp3 = Lay::thread_part(3, lane_o, wave);
     double tw2[T2::N];
     wave_pass_tw_inv<LW, 3, 3, 0, false>(tw2, tp3, 0, ar);
+    // The next limb's loads go out after the second pass's twiddle loads:
+    // vector memory operations complete in issue order, so twiddles issued
+    // behind a limb's worth of loads would wait for all of them.
+    __builtin_amdgcn_sched_barrier(0);
+    const int next = item + gridDim.x;
+    if (next < items) load4(data + ((size_t)next << LOGM), tp4, pf);
+    __builtin_amdgcn_sched_barrier(0);
     // Every wave has finished reading the previous limb's exchange.
     __syncthreads();
     wave_to_lds<LW, 4>(x, lds, tp4);

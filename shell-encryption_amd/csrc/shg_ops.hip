@@ -129,6 +129,41 @@ hipError_t launch_pointwise(int op, uint64_t* x, const uint64_t* a, size_t a_str
   return hipGetLastError();
 }
 
+// x <- x + (a + b) * c or x + (a - b) * c (Montgomery product), the
+// per-term update of LazyRnsPolynomial::FusedMulSumAddInPlace /
+// FusedMulDifferenceAddInPlace (rns/lazy_rns_polynomial.cc:170-254).  The
+// reference adds the integer products (a + b) c and (a + q - b) c into a
+// 128-bit sum that Export reduces and takes out of Montgomery form; keeping
+// x reduced after every term gives the same residue, canonical.
+template <bool SUB>
+__global__ void fused_mul_sum_add_kernel(uint64_t* __restrict__ x, const uint64_t* __restrict__ a,
+                                         const uint64_t* __restrict__ b,
+                                         const uint64_t* __restrict__ c,
+                                         const PrimeDev* __restrict__ primes, LimbPrimes lp,
+                                         int num_limbs, int logn, long long total) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  const int limb = (int)((i >> logn) % num_limbs);
+  const PrimeDev& P = primes[lp.id[limb]];
+  const uint64_t s = SUB ? mod_sub(a[i], b[i], P.q) : mod_add(a[i], b[i], P.q);
+  x[i] = mont_fma(x[i], s, c[i], P);
+}
+
+hipError_t launch_fused_mul_sum_add(bool subtract, uint64_t* x, const uint64_t* a,
+                                    const uint64_t* b, const uint64_t* c, const PrimeDev* primes,
+                                    const LimbPrimes& lp, int count, int num_limbs, int logn,
+                                    hipStream_t s) {
+  const long long total = ((long long)count * num_limbs) << logn;
+  const dim3 g(grid_for(total, 256)), blk(256);
+  if (subtract)
+    hipLaunchKernelGGL(fused_mul_sum_add_kernel<true>, g, blk, 0, s, x, a, b, c, primes, lp,
+                       num_limbs, logn, total);
+  else
+    hipLaunchKernelGGL(fused_mul_sum_add_kernel<false>, g, blk, 0, s, x, a, b, c, primes, lp,
+                       num_limbs, logn, total);
+  return hipGetLastError();
+}
+
 __global__ void mul_scalar_kernel(uint64_t* __restrict__ x, LimbPrimes slo, LimbPrimes shi,
                                   const PrimeDev* __restrict__ primes, LimbPrimes lp,
                                   int num_limbs, int logn, long long total) {

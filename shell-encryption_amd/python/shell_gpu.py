@@ -83,6 +83,7 @@ def lib():
         "shg_galois_apply": [VP, VP, i, VP, VP, INTP, VP, VP, i, i, INTP],
         "shg_gadget_decompose": [VP, VP, INTP, VP, i, i, INTP],
         "shg_lazy_fma": [VP, VP, sz, VP, sz, i, VP, i, INTP],
+        "shg_fused_mul_sum_add": [VP, i, VP, VP, VP, VP, i, i, INTP],
         "shg_batch_limbs": [VP, ctypes.POINTER(LimbJob), i, i],
         "shg_batch_plan_create": [VP, ctypes.POINTER(LimbJob), i, i, ctypes.POINTER(VP)],
         "shg_batch_plan_destroy": [VP, VP],
@@ -334,6 +335,11 @@ class Context:
     def lazy_fma(self, as_, as_stride, bs, bs_stride, num_terms, out, primes):
         _check(lib().shg_lazy_fma(self.h, as_.p, int(as_stride), bs.p, int(bs_stride),
                                   int(num_terms), out.p, len(primes), _ints(primes)))
+
+    def fused_mul_sum_add(self, x, a, b, c, subtract, count, primes):
+        """x <- x + (a +/- b) * c (LazyRnsPolynomial::FusedMulSum/DifferenceAddInPlace)."""
+        _check(lib().shg_fused_mul_sum_add(self.h, int(bool(subtract)), x.p, a.p, b.p, c.p,
+                                           int(count), len(primes), _ints(primes)))
 
     def batch_limbs(self, jobs, op):
         """jobs: (data_ptr, other_ptr or None, prime, log_n) tuples."""
