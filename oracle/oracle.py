@@ -94,6 +94,9 @@ def lib():
         L.so_relin_apply.argtypes = [PP, ctypes.c_int, INTP, ctypes.c_int, U64P,
                                      ctypes.c_int, U64P, U64P, U64P, U64P]
         L.so_relin_apply.restype = ctypes.c_int
+        L.so_hwy_relin_apply.argtypes = L.so_relin_apply.argtypes
+        L.so_hwy_relin_apply.restype = ctypes.c_int
+        L.so_hwy_poly_ntt_fwd.argtypes = [PP, ctypes.c_int, U64P]
         L.so_galois_apply.argtypes = [PP, ctypes.c_int, INTP, U64P, U64P, ctypes.c_int,
                                       U64P, U64P, U64P, U64P]
         L.so_galois_apply.restype = ctypes.c_int
@@ -498,16 +501,23 @@ class Rns:
         lib().so_gadget_decompose(self._arr, self.L, _ints(log_bs), _p64(aa), _p64(out))
         return out
 
-    def relin_apply(self, comps, key_as, key_bs, log_bs, ntt_mask=None):
+    def hwy_ntt_fwd(self, x):
+        """The vectorised host transform (Highway algorithm, shell_cpu_vec.c)."""
+        y = self._poly(x)
+        lib().so_hwy_poly_ntt_fwd(self._arr, self.L, _p64(y))
+        return y.reshape(np.shape(x))
+
+    def relin_apply(self, comps, key_as, key_bs, log_bs, ntt_mask=None, vectorised=False):
         comps = np.ascontiguousarray(comps, dtype=np.uint64)
         degree = comps.shape[0] - 1
         if ntt_mask is None:
             ntt_mask = (1 << (degree + 1)) - 1
         c0 = np.zeros((self.L, self.n), dtype=np.uint64)
         c1 = np.zeros_like(c0)
-        rc = lib().so_relin_apply(self._arr, self.L, _ints(log_bs), degree, _p64(comps),
-                                  int(ntt_mask), _p64(np.ascontiguousarray(key_as)),
-                                  _p64(np.ascontiguousarray(key_bs)), _p64(c0), _p64(c1))
+        fn = lib().so_hwy_relin_apply if vectorised else lib().so_relin_apply
+        rc = fn(self._arr, self.L, _ints(log_bs), degree, _p64(comps),
+                int(ntt_mask), _p64(np.ascontiguousarray(key_as)),
+                _p64(np.ascontiguousarray(key_bs)), _p64(c0), _p64(c1))
         if rc != 0:
             raise ValueError("relin_apply failed: %d" % rc)
         return c0, c1

@@ -176,6 +176,13 @@ void so_approx_switch_rns_basis(so_prime* const* in_p, int L, so_prime* const* o
                                 const uint64_t* q_hat_inv, const uint64_t* q_hat_mod_ps,
                                 int is_ntt, int balanced, const uint64_t* in, uint64_t* out);
 
+/* ---- vectorised host baseline (shell_cpu_vec.c) ---- */
+void so_hwy_ntt_fwd(const so_prime* pr, uint64_t* x);
+void so_hwy_poly_ntt_fwd(so_prime* const* primes, int nl, uint64_t* x);
+int so_hwy_relin_apply(so_prime* const* primes, int nl, const int* log_bs, int degree,
+                       const uint64_t* comps, int comp_is_ntt_mask, const uint64_t* key_as,
+                       const uint64_t* key_bs, uint64_t* out_c0, uint64_t* out_c1);
+
 #ifdef __cplusplus
 }
 #endif
