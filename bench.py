@@ -672,26 +672,37 @@ def main():
         out["modswitch_ciphertexts_per_s"] = round(3 * BATCH / (time.perf_counter() - t0), 1)
         del ms_in
 
-        # Autotuned transform configuration for the NTT workload's shape.
-        chosen = ctx.autotune_ntt(LOG_N, NTT_BATCH * L)
+        # Autotuned transform configuration for the NTT workload's shape
+        # (its own moduli and batch), timed alternately with the explicit
+        # FP64 default on the same buffer so that clock drift and buffer
+        # placement affect both alike.
+        chosen = ctx.autotune_ntt_moduli(pids, NTT_BATCH)
         d_x = ctx.array(uniform(rng, qs, (NTT_BATCH,), n))
         ctx.ntt_fwd(d_x, NTT_BATCH, pids)
         ctx.sync()
+        acc = {"auto": [0.0, 0, 0.0, 0], "f64": [0.0, 0, 0.0, 0]}
         ctx.profile(True)
-        ctx.profile_reset()
-        for _ in range(10):
-            ctx.ntt_fwd(d_x, NTT_BATCH, pids)
-        for _ in range(10):
-            ctx.ntt_inv(d_x, NTT_BATCH, pids)
-        ctx.sync()
-        f_ms, f_n = ctx.profile_query("ntt_fwd")
-        i_ms, i_n = ctx.profile_query("ntt_inv")
+        for _ in range(5):
+            for name, v in (("auto", sg.NTT_AUTO), ("f64", sg.NTT_F64)):
+                ctx.profile_reset()
+                for _ in range(2):
+                    ctx.ntt_fwd(d_x, NTT_BATCH, pids, v)
+                for _ in range(2):
+                    ctx.ntt_inv(d_x, NTT_BATCH, pids, v)
+                ctx.sync()
+                f_ms, f_n = ctx.profile_query("ntt_fwd")
+                i_ms, i_n = ctx.profile_query("ntt_inv")
+                a = acc[name]
+                a[0] += f_ms; a[1] += f_n; a[2] += i_ms; a[3] += i_n
         ctx.profile(False)
         by = NTT_BATCH * 2 * poly_words * 8
-        out["ntt"]["autotuned"] = {"config": chosen, "fwd_avg_launch_ms": round(f_ms / f_n, 4),
-                                   "inv_avg_launch_ms": round(i_ms / i_n, 4),
-                                   "ntts_per_s": round(NTT_BATCH / (f_ms / f_n * 1e-3), 1),
-                                   "hbm_frac": round(by / (f_ms / f_n * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+        fa = acc["auto"][0] / acc["auto"][1]
+        out["ntt"]["autotuned"] = {"config": chosen, "fwd_avg_launch_ms": round(fa, 4),
+                                   "inv_avg_launch_ms": round(acc["auto"][2] / acc["auto"][3], 4),
+                                   "ntts_per_s": round(NTT_BATCH / (fa * 1e-3), 1),
+                                   "hbm_frac": round(by / (fa * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                                   "f64_same_buffer_fwd_ms": round(acc["f64"][0] / acc["f64"][1], 4),
+                                   "f64_same_buffer_inv_ms": round(acc["f64"][2] / acc["f64"][3], 4)}
         del d_x
         out["uneven_batch"] = uneven_batch(ctx, rng)
         out["serving"] = serving(ctx, rng)

@@ -82,6 +82,11 @@ typedef enum shg_ntt_variant {
  * flags the FP64 forward transform at 2^14 and the inverse at 2^13 and 2^14
  * already are these. */
 #define SHG_NTT_WAVE_EXCHANGE 32
+/* Stage split: 2^13..2^15-coefficient limbs as one radix-2 stage streamed
+ * over HBM (first for the forward transform, last for the inverse) plus
+ * 2^(log n - 1)-word slices, two per limb: twice the workgroups for batches
+ * too small to fill the chip, at the cost of one more pass over the data. */
+#define SHG_NTT_STAGE_SPLIT 64
 
 /* ---- context, memory, streams ------------------------------------------ */
 int shg_create(int device, shg_context** out);
@@ -454,16 +459,29 @@ int shg_profile_query(shg_context* ctx, const char* kernel, double* total_ms,
                       long long* launches);
 
 /* ---- autotuner ----------------------------------------------------------
- * No counterpart in the reference.  Times every NTT launch configuration
- * for `limbs` limbs of 2^log_n words (forward + inverse pairs, best of
- * three interleaved rounds after a warm-up) on a modulus below 2^50, and records
- * the fastest for SHG_NTT_AUTO calls of that ring degree and limb count
- * (rounded down to a power of two) and for batch plans of that ring degree.
- * Configurations: arithmetic (FP64 or 64-bit integer Shoup), the register
- * tiling of 2^12..2^14-word slices (16 words per thread, or 8 / 32), and
- * next-item prefetch.  *chosen receives the configuration code: the variant
- * in bits 0-1, 4 = alternate tiling, 8 = prefetch. */
+ * No counterpart in the reference.  Times the NTT launch configurations
+ * that apply to a shape (forward + inverse pairs, best of three interleaved
+ * rounds after a warm-up) and records the fastest for SHG_NTT_AUTO calls:
+ *   arithmetic      FP64 (moduli below 2^50) or 64-bit integer Shoup;
+ *   kernel family   register/LDS passes, wave exchange (SHG_NTT_WAVE_EXCHANGE),
+ *                   split forward limbs (SHG_NTT_SPLIT_LIMB, 2^14);
+ *   register radix  16 words per thread or the alternate 8 / 32
+ *                   (SHG_NTT_TILING_ALT, 2^12..2^14 slices);
+ *   stage split     whole-limb slices or one streamed stage plus half-limb
+ *                   slices (SHG_NTT_STAGE_SPLIT, 2^13..2^15);
+ *   prefetch        loading the next limb during the current (FP64).
+ * The table is keyed by (log n, limbs per polynomial, total limbs rounded
+ * down to a power of two, modulus width: below 2^50 or not); a call looks up
+ * its own key, then the same key for any limbs per polynomial.
+ * shg_autotune_ntt tunes `limbs` limbs on one synthetic modulus below 2^50
+ * (one limb per polynomial; recorded for every limb count of that total, and
+ * for batch plans of that ring degree).  *chosen receives the configuration
+ * code: the variant in bits 0-1 or-ed with the flags above. */
 int shg_autotune_ntt(shg_context* ctx, int log_n, int limbs, int* chosen);
+/* As shg_autotune_ntt for `count` polynomials over the caller's moduli
+ * (their width selects the candidates; uniform residues of each modulus). */
+int shg_autotune_ntt_moduli(shg_context* ctx, const int* primes, int num_limbs, int count,
+                            int* chosen);
 
 #ifdef __cplusplus
 }

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -71,7 +72,9 @@ struct shg_context {
   int d_capacity = 0;
   void* scratch = nullptr;
   size_t scratch_size = 0;
-  std::map<std::pair<int, int>, int> tuned;  // (log_n, bucket) -> variant
+  // Autotuned NTT configurations: (log_n, limbs per polynomial or -1,
+  // total-limb bucket or -1, modulus width class) -> configuration code.
+  std::map<std::array<int, 4>, int> tuned;
   // Oblivious expansion / inner product work areas and the NTT-form
   // monomials -X^(n - 2^j) = X^(-2^j), keyed by (prime handles, j).
   struct DevBuf {
@@ -288,19 +291,36 @@ int rows_bucket(long long rows) {
 // variant, or for SHG_NTT_AUTO the autotuner's choice for (log_n, rows) if
 // there is one, else FP64 when every modulus allows it.  -1: FP64 asked for
 // with a modulus of 2^50 or more.
+// Width class of a modulus chain for the autotuner's table: 0 when every
+// modulus is below 2^50 (the FP64 variants apply), 1 otherwise.
+int width_class(shg_context* ctx, const int* primes, int num_limbs) {
+  for (int i = 0; i < num_limbs; ++i)
+    if (ctx->primes[primes[i]]->q >= (1ull << 50)) return 1;
+  return 0;
+}
+
+// The autotuned configuration for a shape, or -1: the exact key, then the
+// same key for any limbs per polynomial.
+int tuned_config(shg_context* ctx, int log_n, int num_limbs, long long rows, int wide) {
+  const int bucket = rows_bucket(rows);
+  for (const auto& key : {std::array<int, 4>{log_n, num_limbs, bucket, wide},
+                          std::array<int, 4>{log_n, -1, bucket, wide}}) {
+    auto it = ctx->tuned.find(key);
+    if (it != ctx->tuned.end()) return it->second;
+  }
+  return -1;
+}
+
 int resolve_variant(shg_context* ctx, int variant, const int* primes, int num_limbs, int log_n,
                     long long rows) {
-  bool f64_ok = true;
-  for (int i = 0; i < num_limbs; ++i)
-    if (ctx->primes[primes[i]]->q >= (1ull << 50)) f64_ok = false;
+  const int wide = width_class(ctx, primes, num_limbs);
   const int base = variant & 3;
-  if (base == SHG_NTT_F64 && !f64_ok) return -1;
+  if (base == SHG_NTT_F64 && wide) return -1;
   if (base == SHG_NTT_INT_SHOUP) return variant & ~SHG_NTT_PREFETCH;
   if (base == SHG_NTT_F64) return variant;
-  if (!f64_ok) return SHG_NTT_INT_SHOUP;
-  auto it = ctx->tuned.find({log_n, rows_bucket(rows)});
-  if (it != ctx->tuned.end()) return it->second;
-  return SHG_NTT_F64;
+  const int t = tuned_config(ctx, log_n, num_limbs, rows, wide);
+  if (t >= 0) return t;
+  return wide ? SHG_NTT_INT_SHOUP : SHG_NTT_F64;
 }
 
 // Deterministic Miller-Rabin for 64-bit integers (these bases decide every
@@ -643,7 +663,7 @@ static int ntt_common(shg_context* ctx, bool inverse, uint64_t* polys, int count
   if (count == 0) return SHG_OK;
   if (!aligned16(polys))
     return fail(SHG_E_INVALID_ARGUMENT, "polynomial data must be a non-null 16-byte aligned pointer.");
-  if (variant < 0 || (variant & 3) == 3 || (variant & ~63) != 0)
+  if (variant < 0 || (variant & 3) == 3 || (variant & ~127) != 0)
     return fail(SHG_E_INVALID_ARGUMENT, "unknown transform variant %d.", variant);
   const long long rows = (long long)count * num_limbs;
   if ((rows << (log_n > shg::kMaxLogM ? log_n - shg::kMaxLogM : 0)) > 0x7fffffffLL)
@@ -1712,7 +1732,7 @@ int shg_batch_plan_create(shg_context* ctx, const shg_limb_job* host_jobs, int n
   int v = variant;
   if (v == SHG_NTT_AUTO) {
     v = f64_ok ? SHG_NTT_F64 : SHG_NTT_INT_SHOUP;
-    auto it = ctx->tuned.find({max_log_n, -1});
+    auto it = ctx->tuned.find({max_log_n, -1, -1, 0});
     if (f64_ok && it != ctx->tuned.end()) v = it->second & 3;
   }
   // Largest rings first so the static stride balances the workgroups;
@@ -1952,6 +1972,85 @@ int shg_graph_destroy(shg_context* ctx, shg_graph* graph) {
   return SHG_OK;
 }
 
+namespace {
+
+// Candidate configurations for (log n, width class): every kernel family,
+// register radix, stage split and prefetch setting that applies at that
+// ring degree (flags that a degree ignores are left out).
+std::vector<int> ntt_candidates(int log_n, int wide) {
+  std::vector<int> bases = {SHG_NTT_INT_SHOUP};
+  if (!wide) bases.push_back(SHG_NTT_F64);
+  std::vector<int> out;
+  for (int b : bases) {
+    const bool f64 = b == SHG_NTT_F64;
+    out.push_back(b);
+    if (log_n >= 12 && log_n <= 14) out.push_back(b | shg::kNttCfgAlt);
+    if (f64 && log_n <= 14) {
+      out.push_back(b | shg::kNttCfgPrefetch);
+      if (log_n >= 12) out.push_back(b | shg::kNttCfgAlt | shg::kNttCfgPrefetch);
+    }
+    if (f64 && (log_n == 12 || log_n == 13)) out.push_back(b | shg::kNttCfgWave);
+    if (log_n == 14) out.push_back(b | shg::kNttCfgSplit);
+    if (log_n >= 13 && log_n <= 15) out.push_back(b | shg::kNttCfgHalf);
+  }
+  return out;
+}
+
+// Times `cands` on `limbs` limbs at `buf` (forward + inverse pairs: one
+// warm-up, then three interleaved rounds of four pairs; a candidate's time
+// is its best round, which discounts clock ramps and other transient noise)
+// and returns the fastest.
+int time_candidates(shg_context* ctx, const std::vector<int>& cands, uint64_t* buf,
+                    const LimbPrimes& lp, int num_limbs, int rows, int log_n, hipError_t* err) {
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipError_t e = hipEventCreate(&ev0);
+  if (e == hipSuccess) e = hipEventCreate(&ev1);
+  hipStream_t s = ctx->stream;
+  auto run = [&](int cfg) {
+    hipError_t r = shg::launch_ntt(false, cfg, buf, ctx->d_primes, lp, rows, num_limbs, log_n,
+                                   shg::kMaxLogM, s);
+    if (r == hipSuccess)
+      r = shg::launch_ntt(true, cfg, buf, ctx->d_primes, lp, rows, num_limbs, log_n,
+                          shg::kMaxLogM, s);
+    return r;
+  };
+  std::vector<float> best_round(cands.size(), 1e30f);
+  for (size_t c = 0; c < cands.size() && e == hipSuccess; ++c) e = run(cands[c]);
+  for (int round = 0; round < 3 && e == hipSuccess; ++round) {
+    for (size_t c = 0; c < cands.size() && e == hipSuccess; ++c) {
+      e = hipEventRecord(ev0, s);
+      for (int r = 0; r < 4 && e == hipSuccess; ++r) e = run(cands[c]);
+      if (e == hipSuccess) e = hipEventRecord(ev1, s);
+      if (e == hipSuccess) e = hipEventSynchronize(ev1);
+      float ms = 0.f;
+      if (e == hipSuccess) e = hipEventElapsedTime(&ms, ev0, ev1);
+      if (e == hipSuccess && ms < best_round[c]) best_round[c] = ms;
+    }
+  }
+  int best = cands[0];
+  float best_ms = 1e30f;
+  for (size_t c = 0; c < cands.size(); ++c) {
+    if (best_round[c] < best_ms) {
+      best_ms = best_round[c];
+      best = cands[c];
+    }
+  }
+  if (ev0) (void)hipEventDestroy(ev0);
+  if (ev1) (void)hipEventDestroy(ev1);
+  *err = e;
+  return best;
+}
+
+uint64_t splitmix(uint64_t& st) {
+  st += 0x9e3779b97f4a7c15ull;
+  uint64_t z = st;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+
+}  // namespace
+
 int shg_autotune_ntt(shg_context* ctx, int log_n, int limbs, int* chosen) {
   if (!ctx || !chosen) return fail(SHG_E_INVALID_ARGUMENT, "null argument.");
   if (log_n < 1 || log_n > 15) return fail(SHG_E_INVALID_ARGUMENT, "log_n must be in [1, 15].");
@@ -1969,72 +2068,60 @@ int shg_autotune_ntt(shg_context* ctx, int log_n, int limbs, int* chosen) {
   const size_t words = (size_t)limbs * n;
   std::vector<uint64_t> host(words);
   uint64_t st = 0x9e3779b97f4a7c15ull;
-  for (auto& w : host) {
-    st += 0x9e3779b97f4a7c15ull;
-    uint64_t z = st;
-    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
-    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
-    w = (z ^ (z >> 31)) % q;
-  }
+  for (auto& w : host) w = splitmix(st) % q;
   void* buf = nullptr;
   if (hipMalloc(&buf, words * 8) != hipSuccess)
     return fail(SHG_E_NO_MEMORY, "autotune buffer of %zu bytes failed.", words * 8);
   hipError_t e = hipMemcpy(buf, host.data(), words * 8, hipMemcpyHostToDevice);
   LimbPrimes lp{};
   lp.id[0] = pid;
-  const int logm = log_n < shg::kMaxLogM ? log_n : shg::kMaxLogM;
-  std::vector<int> cands = {SHG_NTT_F64, SHG_NTT_F64 | shg::kNttCfgPrefetch, SHG_NTT_INT_SHOUP};
-  if (logm >= 12) {
-    cands.push_back(SHG_NTT_F64 | shg::kNttCfgAlt);
-    cands.push_back(SHG_NTT_F64 | shg::kNttCfgAlt | shg::kNttCfgPrefetch);
-    cands.push_back(SHG_NTT_INT_SHOUP | shg::kNttCfgAlt);
-  }
-  if (log_n == 14) cands.push_back(SHG_NTT_F64 | shg::kNttCfgSplit);
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  if (e == hipSuccess) e = hipEventCreate(&ev0);
-  if (e == hipSuccess) e = hipEventCreate(&ev1);
-  hipStream_t s = ctx->stream;
-  auto run = [&](int cfg) {
-    hipError_t r = shg::launch_ntt(false, cfg, (uint64_t*)buf, ctx->d_primes, lp, limbs, 1, log_n,
-                                   shg::kMaxLogM, s);
-    if (r == hipSuccess)
-      r = shg::launch_ntt(true, cfg, (uint64_t*)buf, ctx->d_primes, lp, limbs, 1, log_n,
-                          shg::kMaxLogM, s);
-    return r;
-  };
-  // Every candidate is warmed up once, then timed in three interleaved
-  // rounds of four forward+inverse pairs; a candidate's time is its best
-  // round, which discounts clock ramps and other transient noise.
-  std::vector<float> best_round(cands.size(), 1e30f);
-  for (size_t c = 0; c < cands.size() && e == hipSuccess; ++c) e = run(cands[c]);
-  for (int round = 0; round < 3 && e == hipSuccess; ++round) {
-    for (size_t c = 0; c < cands.size() && e == hipSuccess; ++c) {
-      e = hipEventRecord(ev0, s);
-      for (int r = 0; r < 4 && e == hipSuccess; ++r) e = run(cands[c]);
-      if (e == hipSuccess) e = hipEventRecord(ev1, s);
-      if (e == hipSuccess) e = hipEventSynchronize(ev1);
-      float ms = 0.f;
-      if (e == hipSuccess) e = hipEventElapsedTime(&ms, ev0, ev1);
-      if (e == hipSuccess && ms < best_round[c]) best_round[c] = ms;
-    }
-  }
   int best = SHG_NTT_F64;
-  float best_ms = 1e30f;
-  for (size_t c = 0; c < cands.size(); ++c) {
-    if (best_round[c] < best_ms) {
-      best_ms = best_round[c];
-      best = cands[c];
-    }
-  }
-  if (ev0) (void)hipEventDestroy(ev0);
-  if (ev1) (void)hipEventDestroy(ev1);
-  (void)hipStreamSynchronize(s);
+  if (e == hipSuccess)
+    best = time_candidates(ctx, ntt_candidates(log_n, 0), (uint64_t*)buf, lp, 1, limbs, log_n, &e);
+  (void)hipStreamSynchronize(ctx->stream);
   (void)hipFree(buf);
   if (e != hipSuccess) return hip_fail(e, "autotune");
   {
     std::lock_guard<std::mutex> lock(ctx->mu);
-    ctx->tuned[{log_n, rows_bucket(limbs)}] = best;
-    ctx->tuned[{log_n, -1}] = best;
+    const int bucket = rows_bucket(limbs);
+    ctx->tuned[{log_n, -1, bucket, 0}] = best;
+    ctx->tuned[{log_n, -1, -1, 0}] = best;  // batch plans of this ring degree
+  }
+  *chosen = best;
+  return SHG_OK;
+}
+
+int shg_autotune_ntt_moduli(shg_context* ctx, const int* primes, int num_limbs, int count,
+                            int* chosen) {
+  if (!ctx || !chosen) return fail(SHG_E_INVALID_ARGUMENT, "null argument.");
+  LimbPrimes lp{};
+  int log_n = 0;
+  int rc = check_primes(ctx, primes, num_limbs, &log_n, &lp);
+  if (rc) return rc;
+  if (count <= 0) return fail(SHG_E_INVALID_ARGUMENT, "count must be positive.");
+  if (capturing(ctx)) return refuse_capture(ctx, "shg_autotune_ntt_moduli");
+  const int wide = width_class(ctx, primes, num_limbs);
+  const size_t n = (size_t)1 << log_n;
+  const size_t words = (size_t)count * num_limbs * n;
+  std::vector<uint64_t> host(words);
+  uint64_t st = 0x2545f4914f6cdd1dull;
+  for (size_t w = 0; w < words; ++w)
+    host[w] = splitmix(st) % ctx->primes[primes[(w / n) % num_limbs]]->q;
+  void* buf = nullptr;
+  if (hipMalloc(&buf, words * 8) != hipSuccess)
+    return fail(SHG_E_NO_MEMORY, "autotune buffer of %zu bytes failed.", words * 8);
+  hipError_t e = hipMemcpy(buf, host.data(), words * 8, hipMemcpyHostToDevice);
+  const int rows = count * num_limbs;
+  int best = wide ? SHG_NTT_INT_SHOUP : SHG_NTT_F64;
+  if (e == hipSuccess)
+    best = time_candidates(ctx, ntt_candidates(log_n, wide), (uint64_t*)buf, lp, num_limbs, rows,
+                           log_n, &e);
+  (void)hipStreamSynchronize(ctx->stream);
+  (void)hipFree(buf);
+  if (e != hipSuccess) return hip_fail(e, "autotune");
+  {
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    ctx->tuned[{log_n, num_limbs, rows_bucket(rows), wide}] = best;
   }
   *chosen = best;
   return SHG_OK;

@@ -782,7 +782,9 @@ static hipError_t launch_fused(bool inverse, uint64_t* data, const PrimeDev* pri
   if (logn == LOGM)
     return launch_fused_hb<LOGM, LOGE, 0, A, PF>(inverse, data, primes, lp, rows, num_limbs,
                                                  scale, stream);
-  if constexpr (LOGM == kMaxLogM) {
+  // Half-limb slices: 2^15 limbs on 2^14 slices, and the stage split of
+  // 2^13 and 2^14 limbs (without prefetch).
+  if constexpr (LOGM == kMaxLogM || (LOGM >= 12 && !PF)) {
     if (logn == LOGM + 1)
       return launch_fused_hb<LOGM, LOGE, 1, A, PF>(inverse, data, primes, lp, rows, num_limbs,
                                                    scale, stream);
@@ -861,6 +863,15 @@ hipError_t launch_ntt(bool inverse, int cfg, uint64_t* data, const PrimeDev* pri
                       const LimbPrimes& lp, int rows, int num_limbs, int logn, int max_logm,
                       hipStream_t s, int inv_mode) {
   const bool alt = (cfg & kNttCfgAlt) != 0;
+  if (cfg & kNttCfgHalf) {
+    // One streamed stage and half-limb slices (no prefetching variant).
+    if (logn < 13 || logn > 15 || logn - 1 > max_logm) return hipErrorInvalidValue;
+    if ((cfg & 3) == SHG_NTT_F64)
+      return run_ntt<F64Arith, false>(inverse, alt, data, primes, lp, rows, num_limbs, logn,
+                                      logn - 1, inv_mode, s);
+    return run_ntt<IntArith, false>(inverse, alt, data, primes, lp, rows, num_limbs, logn,
+                                    logn - 1, inv_mode, s);
+  }
   // FP64 forward transforms of 2^14 limbs default to the wave-exchange
   // kernel (the prefetching and alternate tilings keep the others).
   const bool wave_default = logn == 14 && (cfg & (kNttCfgAlt | kNttCfgPrefetch | kNttCfgSplit)) == 0;

@@ -20,6 +20,7 @@ OP_ADD, OP_SUB, OP_MUL, OP_NEG, OP_FMA = 0, 1, 2, 3, 4
 BATCH_NTT_FWD, BATCH_NTT_INV, BATCH_MUL, BATCH_MOD_REDUCE = 0, 1, 2, 3
 NTT_AUTO, NTT_INT, NTT_F64 = 0, 1, 2
 NTT_TILING_ALT, NTT_PREFETCH, NTT_SPLIT_LIMB, NTT_WAVE_EXCHANGE = 4, 8, 16, 32
+NTT_STAGE_SPLIT = 64
 BASIS_EXACT, BASIS_SCALE, BASIS_APPROX = 0, 1, 2
 
 _lib = None
@@ -119,6 +120,7 @@ def lib():
         "shg_graph_launch": [VP, VP],
         "shg_graph_destroy": [VP, VP],
         "shg_autotune_ntt": [VP, i, i, INTP],
+        "shg_autotune_ntt_moduli": [VP, INTP, i, i, INTP],
         "shg_profile_enable": [VP, i],
         "shg_profile_reset": [VP],
         "shg_profile_query": [VP, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
@@ -426,6 +428,12 @@ class Context:
         n = ctypes.c_longlong()
         _check(lib().shg_profile_query(self.h, kernel.encode(), ctypes.byref(ms), ctypes.byref(n)))
         return ms.value, n.value
+
+    def autotune_ntt_moduli(self, primes, count):
+        c = ctypes.c_int()
+        _check(lib().shg_autotune_ntt_moduli(self.h, _ints(primes), len(primes), int(count),
+                                             ctypes.byref(c)))
+        return c.value
 
     def autotune_ntt(self, log_n, limbs):
         c = ctypes.c_int()

@@ -449,7 +449,9 @@ def test_ntt_reference_moduli(ctx, entry):
 CONFIGS = [sg.NTT_F64 | sg.NTT_TILING_ALT, sg.NTT_F64 | sg.NTT_PREFETCH,
            sg.NTT_F64 | sg.NTT_TILING_ALT | sg.NTT_PREFETCH, sg.NTT_INT | sg.NTT_TILING_ALT,
            sg.NTT_F64 | sg.NTT_SPLIT_LIMB, sg.NTT_INT | sg.NTT_SPLIT_LIMB,
-           sg.NTT_F64 | sg.NTT_WAVE_EXCHANGE, sg.NTT_INT | sg.NTT_WAVE_EXCHANGE]
+           sg.NTT_F64 | sg.NTT_WAVE_EXCHANGE, sg.NTT_INT | sg.NTT_WAVE_EXCHANGE,
+           sg.NTT_F64 | sg.NTT_STAGE_SPLIT, sg.NTT_INT | sg.NTT_STAGE_SPLIT,
+           sg.NTT_F64 | sg.NTT_STAGE_SPLIT | sg.NTT_TILING_ALT]
 
 
 @pytest.mark.parametrize("cfg", CONFIGS)
@@ -467,6 +469,10 @@ def test_ntt_launch_configs(ctx, cfg, log_n, L, count):
     pids = ctx.primes(qs, log_n)
     x = rns.uniform(500 + log_n + cfg, count)
     d = ctx.array(x)
+    if cfg & sg.NTT_STAGE_SPLIT and log_n < 13:  # half-limb slices start at 2^13
+        with pytest.raises(sg.ShgError):
+            ctx.ntt_fwd(d, count, pids, cfg)
+        return
     ctx.ntt_fwd(d, count, pids, cfg)
     got = d.download()
     for c in sorted({0, 1, count - 1}):
@@ -486,7 +492,7 @@ def test_autotune_then_auto(ctx):
     qs, rns = _rns(13, 4)
     pids = ctx.primes(qs, 13)
     chosen = ctx.autotune_ntt(13, 256)
-    assert chosen & 3 in (sg.NTT_INT, sg.NTT_F64) and chosen & ~63 == 0
+    assert chosen & 3 in (sg.NTT_INT, sg.NTT_F64) and chosen & ~127 == 0
     x = rns.uniform(77, 64)  # 256 limbs: the tuned shape
     d = ctx.array(x)
     ctx.ntt_fwd(d, 64, pids)
@@ -497,3 +503,26 @@ def test_autotune_then_auto(ctx):
     assert np.array_equal(d.download(), x)
     with pytest.raises(sg.ShgError):
         ctx.ntt_fwd(d, 64, pids, 3)
+
+
+@pytest.mark.parametrize("log_n,L,count,log_bound", [(14, 8, 4, 50), (14, 3, 2, 60), (13, 2, 1, 50),
+                                                     (15, 2, 3, 62)])
+def test_autotune_moduli_then_auto(ctx, log_n, L, count, log_bound):
+    """The autotuner over the caller's moduli: wide moduli (2^50 and above)
+    get an integer configuration, and the AUTO calls of that shape that
+    follow use the table's pick, bit-exact against the oracle."""
+    qs = ntt_primes(log_n, L, log_bound)
+    rns = oracle.Rns(qs, log_n)
+    pids = ctx.primes(qs, log_n)
+    chosen = ctx.autotune_ntt_moduli(pids, count)
+    assert chosen & ~127 == 0
+    if log_bound > 50:
+        assert chosen & 3 == sg.NTT_INT
+    x = rns.uniform(31 + log_n, count)
+    d = ctx.array(x)
+    ctx.ntt_fwd(d, count, pids)
+    got = d.download()
+    for c in range(count):
+        assert np.array_equal(got[c], rns.ntt_fwd(x[c])), c
+    ctx.ntt_inv(d, count, pids)
+    assert np.array_equal(d.download(), x)
