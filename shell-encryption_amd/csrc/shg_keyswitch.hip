@@ -40,6 +40,14 @@
 #ifndef SHG_KS_KEY_BUF
 #define SHG_KS_KEY_BUF 1
 #endif
+// Stage 0 of a split limb from a table of its products: with digits of at
+// most kKsTabLogB bits, the product w * v that the first Cooley-Tukey stage
+// forms for each digit v of the other half takes one of 2^b values, held in
+// LDS beside the exchange buffers (2^11 words at most, 16 KiB).
+#ifndef SHG_KS_STAGE0_TABLE
+#define SHG_KS_STAGE0_TABLE 1
+#endif
+constexpr int kKsTabLogB = 11;
 // Ring degrees (bit log n) that take the wave-exchange kernel.
 #ifndef SHG_KS_WAVE_LOGN
 #define SHG_KS_WAVE_LOGN (1 << 12)
@@ -207,6 +215,20 @@ __global__ void __launch_bounds__((1 << LOGM) >> LOGE)
     // contiguous (VC / 8 16-byte loads); the next digit's are loaded while
     // this one is transformed.
     constexpr int VC = ks_planes_vc(LOGN);
+    // HB = 1 with digits of at most kKsTabLogB bits: the stage-0 products
+    // w * v for every digit value v (the very values ct_side forms, so the
+    // transform sees the same words), built once per workgroup and ordered
+    // before the first digit by a barrier of their own.
+    constexpr bool kTab = HB == 1 && SHG_KS_STAGE0_TABLE && std::is_same<V, double>::value;
+    const bool use_tab = kTab && gd.max_log_b <= kKsTabLogB;
+    V* tab = reinterpret_cast<V*>(smem) + lds_words<LOGM>() * (SHG_KS_DB ? 2 : 1);
+    if constexpr (kTab) {
+      if (use_tab) {
+        const auto w = ar.fwd_tw(1);
+        for (int v = t; v < (1 << gd.max_log_b); v += T) tab[v] = ar.ct_product((double)v, w);
+        __syncthreads();
+      }
+    }
     constexpr int NW = VC / 8;
     static_assert(VC == E << HB, "plane layout must match the slice tiling");
     const uint4* pl = reinterpret_cast<const uint4*>(ka.planes + ((size_t)ct * gd.k << LOGN)) +
@@ -242,8 +264,22 @@ __global__ void __launch_bounds__((1 << LOGM) >> LOGE)
         const auto w = ar.fwd_tw(1);
         const double sgn = h ? -1.0 : 1.0;
         static_assert(!A::fwd_reduce_at(0), "stage 0 is not followed by a reduction here");
+        bool done = false;
+        if constexpr (kTab) {
+          if (use_tab) {
 #pragma unroll
-        for (int e = 0; e < E; ++e) x[e] = ar.ct_side(digit(e), digit(E + e), w, sgn);
+            for (int e = 0; e < E; ++e) {
+              const int i = E + e;
+              const uint32_t v = (i & 1) ? (wd[i / 2] >> 16) : (wd[i / 2] & 0xffffu);
+              x[e] = fma(sgn, tab[v], digit(e));
+            }
+            done = true;
+          }
+        }
+        if (!done) {
+#pragma unroll
+          for (int e = 0; e < E; ++e) x[e] = ar.ct_side(digit(e), digit(E + e), w, sgn);
+        }
       }
       transform_and_mac(j, x);
       if (j + 1 < gd.k) {
@@ -545,7 +581,11 @@ template <int LOGM, int LOGE, class A, int HB, bool PL>
 static hipError_t launch_ks(const KeySwitchArgs& ka, const GadgetDesc& gd, const PrimeDev* primes,
                             const LimbPrimes& lp, int count, int num_limbs, hipStream_t s) {
   constexpr int T = (1 << LOGM) >> LOGE;
-  const size_t shmem = (size_t)lds_words<LOGM>() * 8 * (SHG_KS_DB ? 2 : 1);
+  // Exchange buffers, then (split limbs with small digits) the stage-0 table.
+  const bool tab = PL && HB == 1 && SHG_KS_STAGE0_TABLE && gd.max_log_b <= kKsTabLogB &&
+                   std::is_same<typename A::V, double>::value;
+  const size_t shmem = (size_t)lds_words<LOGM>() * 8 * (SHG_KS_DB ? 2 : 1) +
+                       (tab ? ((size_t)8 << gd.max_log_b) : 0);
   auto k = keyswitch_kernel<LOGM, LOGE, HB, A, PL>;
   if (shmem > 65536) set_max_dynamic_shmem((const void*)k, shmem);
   hipLaunchKernelGGL(k, dim3((unsigned)(count * num_limbs) << HB), dim3(T), shmem, s, ka, gd,

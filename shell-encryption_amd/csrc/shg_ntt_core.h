@@ -215,6 +215,9 @@ struct F64Arith {
     v = u - t;
     u = u + t;
   }
+  // The product a Cooley-Tukey butterfly forms before its sum and
+  // difference: ct_side(u, v, w, s) == fma(s, ct_product(v, w), u).
+  __device__ __forceinline__ V ct_product(V v, const Tw& w) const { return SHG_F64_MULMOD(v, w); }
   __device__ __forceinline__ V ct_half(V u, V v, const Tw& w, bool low) const {
     const double t = SHG_F64_MULMOD(v, w);
     return u + (low ? -t : t);

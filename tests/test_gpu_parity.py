@@ -305,9 +305,12 @@ def test_gadget_decompose(ctx):
 
 # Bases above 2^16 take the path without digit planes: an inverse transform
 # with the ExportInt scaling fused (n^-1 2^-64), then the key switch.
+# At 2^14 (split limbs) bases up to 2^11 take stage 0 from a table of
+# products (7, 10, 11 bits), 12 and 16 bits the products themselves.
 @pytest.mark.parametrize("log_n,L,log_b,count", [(6, 2, 10, 3), (12, 3, 10, 2), (12, 2, 10, 64),
                                                  (13, 4, 10, 2), (14, 8, 10, 2), (15, 2, 25, 1),
-                                                 (13, 2, 20, 2), (14, 3, 20, 1)])
+                                                 (13, 2, 20, 2), (14, 3, 20, 1), (14, 3, 7, 2),
+                                                 (14, 2, 11, 2), (14, 2, 12, 1), (14, 2, 16, 1)])
 def test_relin_apply(ctx, log_n, L, log_b, count):
     qs, rns = _rns(log_n, L)
     pids = ctx.primes(qs, log_n)
@@ -526,3 +529,23 @@ def test_autotune_moduli_then_auto(ctx, log_n, L, count, log_bound):
         assert np.array_equal(got[c], rns.ntt_fwd(x[c])), c
     ctx.ntt_inv(d, count, pids)
     assert np.array_equal(d.download(), x)
+
+
+def test_relin_apply_mixed_bases_n16384(ctx):
+    """Different gadget bases per limb (rns/rns_gadget.cc:42-106 takes one
+    per modulus): the stage-0 table covers the largest, 2^11, and digits of
+    the 2^5- and 2^9-base limbs index its low entries."""
+    qs, rns = _rns(14, 3)
+    pids = ctx.primes(qs, 14)
+    log_bs = [11, 5, 9]
+    k, _ = rns.gadget_dims(log_bs)
+    comps = np.stack([rns.uniform(700 + c, 3) for c in range(2)])
+    kas, kbs = rns.uniform(21, k), rns.uniform(22, k)
+    dc = ctx.array(comps)
+    dka, dkb = ctx.array(kas), ctx.array(kbs)
+    o0, o1 = ctx.empty((2, 3, rns.n)), ctx.empty((2, 3, rns.n))
+    ctx.relin_apply(dc, 2, 0b111, dka, dkb, k, log_bs, o0, o1, 2, pids)
+    g0, g1 = o0.download(), o1.download()
+    for c in range(2):
+        e0, e1 = rns.relin_apply(comps[c], kas, kbs, log_bs)
+        assert np.array_equal(g0[c], e0) and np.array_equal(g1[c], e1), c
