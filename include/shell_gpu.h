@@ -242,6 +242,34 @@ int shg_galois_apply(shg_context* ctx, const uint64_t* comps, int c1_is_ntt,
                      uint64_t* out_c1, int count, int num_limbs,
                      const int* primes);
 
+/* A key-switching key prepared once for repeated use.  No counterpart in the
+ * reference (its keys are RnsPolynomials kept by RnsRelinKey/RnsGaloisKey).
+ * shg_ks_key_create keeps the caller's key_as/key_bs (num_key_polys
+ * polynomials of num_limbs limbs; they must stay valid and unchanged while
+ * the key is used) and, when every modulus is below 2^50, device copies of
+ * them with each residue held as an FP64 value (exact), which the FP64
+ * digit-plane key switch at n = 2^13 and 2^14 multiplies without converting
+ * (two operations fewer per key word); other shapes read the caller's
+ * words.  The results equal shg_relin_apply's / shg_galois_apply's word for
+ * word.  Creation synchronises the context's stream and refuses an active
+ * capture; the apply calls may be captured like the plain ones. */
+typedef struct shg_ks_key shg_ks_key;
+int shg_ks_key_create(shg_context* ctx, const uint64_t* key_as,
+                      const uint64_t* key_bs, int num_key_polys,
+                      int num_limbs, const int* primes, shg_ks_key** out);
+int shg_ks_key_destroy(shg_context* ctx, shg_ks_key* key);
+/* shg_relin_apply / shg_galois_apply with a prepared key (its moduli must be
+ * `primes`). */
+int shg_relin_apply_key(shg_context* ctx, const uint64_t* comps, int degree,
+                        int ntt_mask, const shg_ks_key* key,
+                        const int* host_log_bs, uint64_t* out_c0,
+                        uint64_t* out_c1, int count, int num_limbs,
+                        const int* primes);
+int shg_galois_apply_key(shg_context* ctx, const uint64_t* comps, int c1_is_ntt,
+                         const shg_ks_key* key, const int* host_log_bs,
+                         uint64_t* out_c0, uint64_t* out_c1, int count,
+                         int num_limbs, const int* primes);
+
 /* RnsGadget::Decompose (rns/rns_gadget.cc:108-153): `a` (coefficient form)
  * -> k digit polynomials (coefficient form), count inputs. */
 int shg_gadget_decompose(shg_context* ctx, const uint64_t* a,

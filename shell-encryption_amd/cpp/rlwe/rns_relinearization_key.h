@@ -42,6 +42,22 @@ inline StatusOr<std::shared_ptr<device::Buffer>> PackKeys(
   return buf;
 }
 
+// The packed key columns prepared for the key-switch kernels
+// (shg_ks_key_create: FP64-form copies when every modulus is below 2^50);
+// shared by copies of the key object, released with the last.
+template <typename ModularInt>
+inline StatusOr<std::shared_ptr<shg_ks_key>> PrepareKey(
+    const device::Buffer& as, const device::Buffer& bs, int num_key_polys,
+    ModuliSpan<ModularInt> moduli) {
+  auto ids = DevicePrimes<ModularInt>(moduli);
+  shg_ks_key* k = nullptr;
+  RLWE_RETURN_IF_ERROR(device::FromCode(shg_ks_key_create(
+      device::Context(), as.data(), bs.data(), num_key_polys, (int)ids.size(), ids.data(), &k)));
+  return std::shared_ptr<shg_ks_key>(k, [](shg_ks_key* p) {
+    (void)shg_ks_key_destroy(device::Context(), p);
+  });
+}
+
 // Packs the components of `cts` (each of degree `degree`) back to back.
 template <typename ModularInt>
 inline StatusOr<std::shared_ptr<device::Buffer>> PackCiphertexts(
@@ -260,6 +276,9 @@ class RnsRelinKey {
     RnsRelinKey key;
     RLWE_ASSIGN_OR_RETURN(key.d_as_, internal::PackKeys(key_as));
     RLWE_ASSIGN_OR_RETURN(key.d_bs_, internal::PackKeys(key_bs));
+    RLWE_ASSIGN_OR_RETURN(key.ks_, internal::PrepareKey<ModularInt>(
+                                       *key.d_as_, *key.d_bs_, (int)key_as.size(),
+                                       ModuliSpan<ModularInt>(moduli.data(), moduli.size())));
     key.key_as_ = std::move(key_as);
     key.key_bs_ = std::move(key_bs);
     key.gadget_ = gadget;
@@ -297,9 +316,9 @@ class RnsRelinKey {
     device::Buffer o0(words * cts.size()), o1(words * cts.size());
     auto ids = DevicePrimes<ModularInt>(ModuliSpan<ModularInt>(moduli_.data(), moduli_.size()));
     std::vector<int> lbs(gadget_->LogBases().begin(), gadget_->LogBases().end());
-    RLWE_RETURN_IF_ERROR(device::FromCode(shg_relin_apply(
-        device::Context(), comps->data(), cts[0]->Degree(), mask, d_as_->data(), d_bs_->data(),
-        (int)key_as_.size(), lbs.data(), o0.data(), o1.data(), (int)cts.size(), L, ids.data())));
+    RLWE_RETURN_IF_ERROR(device::FromCode(shg_relin_apply_key(
+        device::Context(), comps->data(), cts[0]->Degree(), mask, ks_.get(), lbs.data(),
+        o0.data(), o1.data(), (int)cts.size(), L, ids.data())));
     std::vector<RnsBgvCiphertext<ModularInt>> out;
     for (size_t c = 0; c < cts.size(); ++c) {
       std::vector<RnsPolynomial<ModularInt>> comps_new;
@@ -325,6 +344,7 @@ class RnsRelinKey {
   RnsRelinKey() = default;
   std::vector<RnsPolynomial<ModularInt>> key_as_, key_bs_;
   std::shared_ptr<device::Buffer> d_as_, d_bs_;
+  std::shared_ptr<shg_ks_key> ks_;  // over d_as_, d_bs_ (declared after them)
   const RnsGadget<ModularInt>* gadget_ = nullptr;
   int degree_ = 0;
   std::vector<const PrimeModulus<ModularInt>*> moduli_;
@@ -430,6 +450,9 @@ class RnsGaloisKey {
     RnsGaloisKey key;
     RLWE_ASSIGN_OR_RETURN(key.d_as_, internal::PackKeys(key_as));
     RLWE_ASSIGN_OR_RETURN(key.d_bs_, internal::PackKeys(key_bs));
+    RLWE_ASSIGN_OR_RETURN(key.ks_, internal::PrepareKey<ModularInt>(
+                                       *key.d_as_, *key.d_bs_, (int)key_as.size(),
+                                       ModuliSpan<ModularInt>(moduli.data(), moduli.size())));
     key.key_as_ = std::move(key_as);
     key.key_bs_ = std::move(key_bs);
     key.gadget_ = gadget;
@@ -637,9 +660,9 @@ class RnsGaloisKey {
     device::Buffer o0(words * count), o1(words * count);
     auto ids = DevicePrimes<ModularInt>(Moduli());
     std::vector<int> lbs(gadget_->LogBases().begin(), gadget_->LogBases().end());
-    RLWE_RETURN_IF_ERROR(device::FromCode(shg_galois_apply(
-        device::Context(), comps.data(), c1_is_ntt, d_as_->data(), d_bs_->data(), lbs.data(),
-        o0.data(), o1.data(), count, L, ids.data())));
+    RLWE_RETURN_IF_ERROR(device::FromCode(shg_galois_apply_key(
+        device::Context(), comps.data(), c1_is_ntt, ks_.get(), lbs.data(), o0.data(), o1.data(),
+        count, L, ids.data())));
     std::vector<RnsPolynomial<ModularInt>> out;
     for (int c = 0; c < count; ++c) {
       out.push_back(RnsPolynomial<ModularInt>::FromDevice(log_n, L, true, o0.data() + c * words));
@@ -650,6 +673,7 @@ class RnsGaloisKey {
 
   std::vector<RnsPolynomial<ModularInt>> key_as_, key_bs_;
   std::shared_ptr<device::Buffer> d_as_, d_bs_;
+  std::shared_ptr<shg_ks_key> ks_;  // over d_as_, d_bs_ (declared after them)
   const RnsGadget<ModularInt>* gadget_ = nullptr;
   int power_ = 1;
   std::vector<const PrimeModulus<ModularInt>*> moduli_;

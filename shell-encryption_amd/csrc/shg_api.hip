@@ -1034,7 +1034,9 @@ static hipError_t keyswitch_fused_one(shg_context* ctx, int v, uint16_t* planes,
                                       bool accumulate, const uint64_t* key_as,
                                       const uint64_t* key_bs, uint64_t* out_c0, uint64_t* out_c1,
                                       int count, int num_limbs, const LimbPrimes& lp, int log_n,
-                                      const shg::GadgetDesc& gd) {
+                                      const shg::GadgetDesc& gd,
+                                      const uint64_t* f64_as = nullptr,
+                                      const uint64_t* f64_bs = nullptr) {
   const size_t poly_words = (size_t)num_limbs << log_n;
   const size_t ct_words = (size_t)comps_per_ct * poly_words;
   hipStream_t s = ctx->stream;
@@ -1065,8 +1067,13 @@ static hipError_t keyswitch_fused_one(shg_context* ctx, int v, uint16_t* planes,
   ka.c0_stride = ct_words;
   ka.c1 = init_c1 ? comps + poly_words : nullptr;
   ka.c1_stride = ct_words;
-  ka.key_a = key_as;
-  ka.key_b = key_bs;
+  // FP64-form copies of the keys (shg_ks_key) serve the digit-plane kernel
+  // at 2^13 and 2^14; every other path reads the Montgomery words.
+  const bool f64_keys = f64_as && f64_bs && planes && (v & 3) == SHG_NTT_F64 && gd.small_digits &&
+                        log_n >= 13;
+  ka.key_a = f64_keys ? f64_as : key_as;
+  ka.key_b = f64_keys ? f64_bs : key_bs;
+  ka.key_f64 = f64_keys ? 1 : 0;
   ka.key_base = key_base;
   ka.out0 = out_c0;
   ka.out1 = out_c1;
@@ -1094,7 +1101,8 @@ static int keyswitch_locked(shg_context* ctx, const uint64_t* comps, int comps_p
                             int num_src, bool init_c1, const uint64_t* key_as,
                             const uint64_t* key_bs, const int* log_bs, uint64_t* out_c0,
                             uint64_t* out_c1, int count, int num_limbs, const int* primes,
-                            const LimbPrimes& lp, int log_n, const shg::GadgetDesc& gd) {
+                            const LimbPrimes& lp, int log_n, const shg::GadgetDesc& gd,
+                            const uint64_t* f64_as = nullptr, const uint64_t* f64_bs = nullptr) {
   const size_t n = (size_t)1 << log_n;
   const size_t poly_words = (size_t)num_limbs * n;
   int rc = SHG_OK;
@@ -1121,7 +1129,8 @@ static int keyswitch_locked(shg_context* ctx, const uint64_t* comps, int comps_p
     if (fused) {
       e = keyswitch_fused_one(ctx, v, use_planes ? planes : nullptr, src, comps, comps_per_ct,
                               src_comp[i], src_is_ntt[i] != 0, key_base[i], init_c1, i > 0,
-                              key_as, key_bs, out_c0, out_c1, count, num_limbs, lp, log_n, gd);
+                              key_as, key_bs, out_c0, out_c1, count, num_limbs, lp, log_n, gd,
+                              f64_as, f64_bs);
       continue;
     }
     // The unfused path decomposes Montgomery representations of the
@@ -1199,11 +1208,12 @@ static int keyswitch_common(shg_context* ctx, const uint64_t* comps, int comps_p
                             int num_src, bool init_c1, const uint64_t* key_as,
                             const uint64_t* key_bs, const int* log_bs, uint64_t* out_c0,
                             uint64_t* out_c1, int count, int num_limbs, const int* primes,
-                            const LimbPrimes& lp, int log_n, const shg::GadgetDesc& gd) {
+                            const LimbPrimes& lp, int log_n, const shg::GadgetDesc& gd,
+                            const uint64_t* f64_as = nullptr, const uint64_t* f64_bs = nullptr) {
   std::lock_guard<std::mutex> lock(ctx->mu);
   return keyswitch_locked(ctx, comps, comps_per_ct, src_comp, src_is_ntt, key_base, num_src,
                           init_c1, key_as, key_bs, log_bs, out_c0, out_c1, count, num_limbs, primes,
-                          lp, log_n, gd);
+                          lp, log_n, gd, f64_as, f64_bs);
 }
 
 }  // extern "C"
@@ -1250,10 +1260,25 @@ static int relin_prepare(shg_context* ctx, const uint64_t* comps, int degree, in
 
 extern "C" {
 
+static int relin_apply_impl(shg_context* ctx, const uint64_t* comps, int degree, int ntt_mask,
+                            const uint64_t* key_as, const uint64_t* key_bs, int num_key_polys,
+                            const int* host_log_bs, uint64_t* out_c0, uint64_t* out_c1,
+                            int count, int num_limbs, const int* primes, const uint64_t* f64_as,
+                            const uint64_t* f64_bs);
+
 int shg_relin_apply(shg_context* ctx, const uint64_t* comps, int degree, int ntt_mask,
                     const uint64_t* key_as, const uint64_t* key_bs, int num_key_polys,
                     const int* host_log_bs, uint64_t* out_c0, uint64_t* out_c1, int count,
                     int num_limbs, const int* primes) {
+  return relin_apply_impl(ctx, comps, degree, ntt_mask, key_as, key_bs, num_key_polys,
+                          host_log_bs, out_c0, out_c1, count, num_limbs, primes, nullptr, nullptr);
+}
+
+static int relin_apply_impl(shg_context* ctx, const uint64_t* comps, int degree, int ntt_mask,
+                            const uint64_t* key_as, const uint64_t* key_bs, int num_key_polys,
+                            const int* host_log_bs, uint64_t* out_c0, uint64_t* out_c1,
+                            int count, int num_limbs, const int* primes, const uint64_t* f64_as,
+                            const uint64_t* f64_bs) {
   LimbPrimes lp{};
   int log_n = 0;
   shg::GadgetDesc gd{};
@@ -1269,7 +1294,7 @@ int shg_relin_apply(shg_context* ctx, const uint64_t* comps, int degree, int ntt
   }
   return keyswitch_common(ctx, comps, degree + 1, src_comp, src_ntt, base, degree - 1, true,
                           key_as, key_bs, host_log_bs, out_c0, out_c1, count, num_limbs, primes,
-                          lp, log_n, gd);
+                          lp, log_n, gd, f64_as, f64_bs);
 }
 
 int shg_relin_apply_noalloc(shg_context* ctx, const uint64_t* comps, int degree, int ntt_mask,
@@ -1329,9 +1354,24 @@ int shg_relin_apply_noalloc(shg_context* ctx, const uint64_t* comps, int degree,
   return e == hipSuccess ? SHG_OK : hip_fail(e, "key switch launch");
 }
 
+static int galois_apply_impl(shg_context* ctx, const uint64_t* comps, int c1_is_ntt,
+                             const uint64_t* key_as, const uint64_t* key_bs,
+                             const int* host_log_bs, uint64_t* out_c0, uint64_t* out_c1,
+                             int count, int num_limbs, const int* primes,
+                             const uint64_t* f64_as, const uint64_t* f64_bs);
+
 int shg_galois_apply(shg_context* ctx, const uint64_t* comps, int c1_is_ntt, const uint64_t* key_as,
                      const uint64_t* key_bs, const int* host_log_bs, uint64_t* out_c0,
                      uint64_t* out_c1, int count, int num_limbs, const int* primes) {
+  return galois_apply_impl(ctx, comps, c1_is_ntt, key_as, key_bs, host_log_bs, out_c0, out_c1,
+                           count, num_limbs, primes, nullptr, nullptr);
+}
+
+static int galois_apply_impl(shg_context* ctx, const uint64_t* comps, int c1_is_ntt,
+                             const uint64_t* key_as, const uint64_t* key_bs,
+                             const int* host_log_bs, uint64_t* out_c0, uint64_t* out_c1,
+                             int count, int num_limbs, const int* primes,
+                             const uint64_t* f64_as, const uint64_t* f64_bs) {
   LimbPrimes lp{};
   int log_n = 0;
   int rc = check_primes(ctx, primes, num_limbs, &log_n, &lp);
@@ -1345,7 +1385,98 @@ int shg_galois_apply(shg_context* ctx, const uint64_t* comps, int c1_is_ntt, con
     return fail(SHG_E_INVALID_ARGUMENT, "data must be non-null 16-byte aligned pointers.");
   const int src_comp = 1, src_ntt = c1_is_ntt ? 1 : 0, base = 0;
   return keyswitch_common(ctx, comps, 2, &src_comp, &src_ntt, &base, 1, false, key_as, key_bs,
-                          host_log_bs, out_c0, out_c1, count, num_limbs, primes, lp, log_n, gd);
+                          host_log_bs, out_c0, out_c1, count, num_limbs, primes, lp, log_n, gd,
+                          f64_as, f64_bs);
+}
+
+// A key-switching key with FP64-form copies of its residues for the
+// digit-plane kernel; the Montgomery words stay the caller's.
+struct shg_ks_key {
+  const uint64_t* key_as = nullptr;
+  const uint64_t* key_bs = nullptr;
+  uint64_t* f64_as = nullptr;  // owned; null when a modulus is 2^50 or more
+  uint64_t* f64_bs = nullptr;
+  int num_key_polys = 0;
+  int num_limbs = 0;
+  int primes[SHG_MAX_LIMBS] = {};
+};
+
+int shg_ks_key_create(shg_context* ctx, const uint64_t* key_as, const uint64_t* key_bs,
+                      int num_key_polys, int num_limbs, const int* primes, shg_ks_key** out) {
+  if (!ctx || !out) return fail(SHG_E_INVALID_ARGUMENT, "null argument.");
+  LimbPrimes lp{};
+  int log_n = 0;
+  int rc = check_primes(ctx, primes, num_limbs, &log_n, &lp);
+  if (rc) return rc;
+  if (num_key_polys <= 0) return fail(SHG_E_INVALID_ARGUMENT, "num_key_polys must be positive.");
+  if (!aligned16(key_as) || !aligned16(key_bs))
+    return fail(SHG_E_INVALID_ARGUMENT, "keys must be non-null 16-byte aligned pointers.");
+  if (capturing(ctx)) return refuse_capture(ctx, "shg_ks_key_create");
+  auto key = std::make_unique<shg_ks_key>();
+  key->key_as = key_as;
+  key->key_bs = key_bs;
+  key->num_key_polys = num_key_polys;
+  key->num_limbs = num_limbs;
+  for (int i = 0; i < num_limbs; ++i) key->primes[i] = primes[i];
+  if (!width_class(ctx, primes, num_limbs)) {
+    const size_t words = (size_t)num_key_polys * num_limbs << log_n;
+    if (hipMalloc((void**)&key->f64_as, words * 8) != hipSuccess)
+      return fail(SHG_E_NO_MEMORY, "key copy of %zu bytes failed.", words * 8);
+    if (hipMalloc((void**)&key->f64_bs, words * 8) != hipSuccess) {
+      (void)hipFree(key->f64_as);  // already failing; report the allocation
+      return fail(SHG_E_NO_MEMORY, "key copy of %zu bytes failed.", words * 8);
+    }
+    hipError_t e = shg::launch_key_to_f64(key_as, key->f64_as, words, ctx->stream);
+    if (e == hipSuccess) e = shg::launch_key_to_f64(key_bs, key->f64_bs, words, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+      (void)hipFree(key->f64_as);  // already failing; report the launch
+      (void)hipFree(key->f64_bs);
+      return hip_fail(e, "key conversion");
+    }
+  }
+  *out = key.release();
+  return SHG_OK;
+}
+
+int shg_ks_key_destroy(shg_context* ctx, shg_ks_key* key) {
+  if (!ctx) return fail(SHG_E_INVALID_ARGUMENT, "null argument.");
+  if (!key) return SHG_OK;
+  if (capturing(ctx)) return refuse_capture(ctx, "shg_ks_key_destroy");
+  (void)hipStreamSynchronize(ctx->stream);  // launches that read the copies have finished
+  if (key->f64_as) (void)hipFree(key->f64_as);
+  if (key->f64_bs) (void)hipFree(key->f64_bs);
+  delete key;
+  return SHG_OK;
+}
+
+static int check_key(const shg_ks_key* key, int num_limbs, const int* primes) {
+  if (!key) return fail(SHG_E_INVALID_ARGUMENT, "`key` must not be null.");
+  if (!primes || key->num_limbs != num_limbs)
+    return fail(SHG_E_INVALID_ARGUMENT, "`key` was made for %d RNS moduli.", key->num_limbs);
+  for (int i = 0; i < num_limbs; ++i)
+    if (key->primes[i] != primes[i])
+      return fail(SHG_E_INVALID_ARGUMENT, "`key` was made for other RNS moduli.");
+  return SHG_OK;
+}
+
+int shg_relin_apply_key(shg_context* ctx, const uint64_t* comps, int degree, int ntt_mask,
+                        const shg_ks_key* key, const int* host_log_bs, uint64_t* out_c0,
+                        uint64_t* out_c1, int count, int num_limbs, const int* primes) {
+  int rc = check_key(key, num_limbs, primes);
+  if (rc) return rc;
+  return relin_apply_impl(ctx, comps, degree, ntt_mask, key->key_as, key->key_bs,
+                          key->num_key_polys, host_log_bs, out_c0, out_c1, count, num_limbs,
+                          primes, key->f64_as, key->f64_bs);
+}
+
+int shg_galois_apply_key(shg_context* ctx, const uint64_t* comps, int c1_is_ntt,
+                         const shg_ks_key* key, const int* host_log_bs, uint64_t* out_c0,
+                         uint64_t* out_c1, int count, int num_limbs, const int* primes) {
+  int rc = check_key(key, num_limbs, primes);
+  if (rc) return rc;
+  return galois_apply_impl(ctx, comps, c1_is_ntt, key->key_as, key->key_bs, host_log_bs, out_c0,
+                           out_c1, count, num_limbs, primes, key->f64_as, key->f64_bs);
 }
 
 // The NTT-form monomial -X^(n - 2^level) = X^(-2^level) mod each prime

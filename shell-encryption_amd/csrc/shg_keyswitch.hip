@@ -83,6 +83,16 @@ struct F64KsArith : F64Arith {
   __host__ __device__ static constexpr int ks_acc_every(int logn) { return ks_prep(logn) ? 8 : 3; }
 };
 
+// Keys held as FP64 values (shg_ks_key_create): the multiply-accumulate
+// takes each key word as it is, where F64KsArith converts it (an OR and a
+// subtraction per word).  Same products, same accumulated words.
+struct F64KsKeyArith : F64KsArith {
+  using F64KsArith::F64KsArith;
+  __device__ __forceinline__ void mac(Acc& a, Prep d, uint64_t key, const PrimeDev&) const {
+    a += f64_mulmod_h(d, __longlong_as_double((long long)key), q, qinv);
+  }
+};
+
 // LOGN = LOGM + HB.  With HB = 1 (n = 2^14) the limb is split between two
 // workgroups: each loads both halves of the digit, applies the first
 // Cooley-Tukey stage for its own half only, and runs the remaining stages on
@@ -603,6 +613,12 @@ static hipError_t dispatch_ks(int logn, const KeySwitchArgs& ka, const GadgetDes
   // at L = 4.  At 2^13 and 2^14 one 512-thread workgroup per CU is resident
   // either way and keyswitch_kernel measured 5% and 6% faster (2^14, L = 8:
   // 10.55 vs 11.2 ms per 512 ciphertexts).
+  if constexpr (std::is_same<A, F64KsKeyArith>::value) {
+    // FP64-form keys: the ring degrees whose default is keyswitch_kernel.
+    if (logn == 13) return SHG_KS(13, 4, 0);
+    if (logn == 14) return SHG_KS(13, SHG_KS_LOGE, 1);
+    return hipErrorInvalidValue;
+  }
   if constexpr (PL && std::is_same<A, F64KsArith>::value && SHG_KS_WAVE) {
     if ((SHG_KS_WAVE_LOGN >> logn) & 1) {
       if (logn == 12) return launch_ks_wave<2, 0>(ka, gd, primes, lp, count, num_limbs, s);
@@ -610,6 +626,7 @@ static hipError_t dispatch_ks(int logn, const KeySwitchArgs& ka, const GadgetDes
       if (logn == 14) return launch_ks_wave<3, 1>(ka, gd, primes, lp, count, num_limbs, s);
     }
   }
+  if constexpr (!std::is_same<A, F64KsKeyArith>::value) {
   switch (logn) {
     case 4: return SHG_KS(4, 4, 0);
     case 5: return SHG_KS(5, 4, 0);
@@ -624,14 +641,19 @@ static hipError_t dispatch_ks(int logn, const KeySwitchArgs& ka, const GadgetDes
     case 14: return SHG_KS(13, SHG_KS_LOGE, 1);
     default: return hipErrorInvalidValue;
   }
+  }
 #undef SHG_KS
 }
 
 hipError_t launch_keyswitch(const KeySwitchArgs& ka, const GadgetDesc& gd, int variant,
                             const PrimeDev* primes, const LimbPrimes& lp, int count, int num_limbs,
                             int logn, hipStream_t s) {
+  if (ka.key_f64 && !(variant == SHG_NTT_F64 && gd.small_digits && ka.planes && logn >= 13))
+    return hipErrorInvalidValue;  // FP64-form keys are read by that kernel only
   if (variant == SHG_NTT_F64) {
     if (gd.small_digits) {
+      if (ka.planes && ka.key_f64)
+        return dispatch_ks<F64KsKeyArith, true>(logn, ka, gd, primes, lp, count, num_limbs, s);
       if (ka.planes)
         return dispatch_ks<F64KsArith, true>(logn, ka, gd, primes, lp, count, num_limbs, s);
       return dispatch_ks<F64KsArith, false>(logn, ka, gd, primes, lp, count, num_limbs, s);

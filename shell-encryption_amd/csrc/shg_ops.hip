@@ -164,6 +164,19 @@ hipError_t launch_fused_mul_sum_add(bool subtract, uint64_t* x, const uint64_t* 
   return hipGetLastError();
 }
 
+__global__ void key_to_f64_kernel(const uint64_t* __restrict__ in, uint64_t* __restrict__ out,
+                                  size_t words) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < words) out[i] = (uint64_t)__double_as_longlong(u52_to_f64(in[i]));
+}
+
+hipError_t launch_key_to_f64(const uint64_t* in, uint64_t* out, size_t words, hipStream_t s) {
+  if (words == 0) return hipSuccess;
+  hipLaunchKernelGGL(key_to_f64_kernel, dim3(grid_for((long long)words, 256)), dim3(256), 0, s, in,
+                     out, words);
+  return hipGetLastError();
+}
+
 __global__ void mul_scalar_kernel(uint64_t* __restrict__ x, LimbPrimes slo, LimbPrimes shi,
                                   const PrimeDev* __restrict__ primes, LimbPrimes lp,
                                   int num_limbs, int logn, long long total) {

@@ -85,6 +85,10 @@ def lib():
         "shg_gadget_decompose": [VP, VP, INTP, VP, i, i, INTP],
         "shg_lazy_fma": [VP, VP, sz, VP, sz, i, VP, i, INTP],
         "shg_fused_mul_sum_add": [VP, i, VP, VP, VP, VP, i, i, INTP],
+        "shg_ks_key_create": [VP, VP, VP, i, i, INTP, ctypes.POINTER(VP)],
+        "shg_ks_key_destroy": [VP, VP],
+        "shg_relin_apply_key": [VP, VP, i, i, VP, INTP, VP, VP, i, i, INTP],
+        "shg_galois_apply_key": [VP, VP, i, VP, INTP, VP, VP, i, i, INTP],
         "shg_batch_limbs": [VP, ctypes.POINTER(LimbJob), i, i],
         "shg_batch_plan_create": [VP, ctypes.POINTER(LimbJob), i, i, ctypes.POINTER(VP)],
         "shg_batch_plan_destroy": [VP, VP],
@@ -343,6 +347,21 @@ class Context:
         _check(lib().shg_fused_mul_sum_add(self.h, int(bool(subtract)), x.p, a.p, b.p, c.p,
                                            int(count), len(primes), _ints(primes)))
 
+    def ks_key(self, key_as, key_bs, num_key_polys, primes):
+        """A prepared key-switching key (shg_ks_key_create); key_as/key_bs
+        must outlive it."""
+        return KsKey(self, key_as, key_bs, num_key_polys, primes)
+
+    def relin_apply_key(self, comps, degree, ntt_mask, key, log_bs, out0, out1, count, primes):
+        _check(lib().shg_relin_apply_key(self.h, comps.p, int(degree), int(ntt_mask), key.h,
+                                         _ints(log_bs), out0.p, out1.p, int(count), len(primes),
+                                         _ints(primes)))
+
+    def galois_apply_key(self, comps, c1_is_ntt, key, log_bs, out0, out1, count, primes):
+        _check(lib().shg_galois_apply_key(self.h, comps.p, int(bool(c1_is_ntt)), key.h,
+                                          _ints(log_bs), out0.p, out1.p, int(count), len(primes),
+                                          _ints(primes)))
+
     def batch_limbs(self, jobs, op):
         """jobs: (data_ptr, other_ptr or None, prime, log_n) tuples."""
         arr = (LimbJob * len(jobs))(*[LimbJob(d, o, p, ln) for (d, o, p, ln) in jobs])
@@ -439,6 +458,28 @@ class Context:
         c = ctypes.c_int()
         _check(lib().shg_autotune_ntt(self.h, int(log_n), int(limbs), ctypes.byref(c)))
         return c.value
+
+
+class KsKey:
+    """shg_ks_key: the caller's key arrays plus FP64-form copies."""
+
+    def __init__(self, ctx, key_as, key_bs, num_key_polys, primes):
+        self.ctx = ctx
+        self.keep = (key_as, key_bs)
+        self.h = VP()
+        _check(lib().shg_ks_key_create(ctx.h, key_as.p, key_bs.p, int(num_key_polys), len(primes),
+                                       _ints(primes), ctypes.byref(self.h)))
+
+    def close(self):
+        if self.h and self.ctx.h:
+            _check(lib().shg_ks_key_destroy(self.ctx.h, self.h))
+            self.h = VP()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # interpreter shutdown: the context may be gone
+            pass
 
 
 class BatchPlan:

@@ -549,3 +549,58 @@ def test_relin_apply_mixed_bases_n16384(ctx):
     for c in range(2):
         e0, e1 = rns.relin_apply(comps[c], kas, kbs, log_bs)
         assert np.array_equal(g0[c], e0) and np.array_equal(g1[c], e1), c
+
+
+@pytest.mark.parametrize("log_n,L,log_b,count", [(14, 8, 10, 3), (13, 3, 10, 2), (12, 2, 10, 4),
+                                                 (14, 2, 20, 1), (14, 2, 16, 2)])
+def test_prepared_key_relin_and_galois(ctx, log_n, L, log_b, count):
+    """Key switching with a prepared key (shg_ks_key: FP64-form copies read
+    by the digit-plane kernel at 2^13 and 2^14; the Montgomery words on every
+    other path) equals the oracle and the plain entry points word for word."""
+    qs, rns = _rns(log_n, L)
+    pids = ctx.primes(qs, log_n)
+    log_bs = [log_b] * L
+    k, _ = rns.gadget_dims(log_bs)
+    comps = np.stack([rns.uniform(800 + c, 3) for c in range(count)])
+    kas, kbs = rns.uniform(31, k), rns.uniform(32, k)
+    dc, dka, dkb = ctx.array(comps), ctx.array(kas), ctx.array(kbs)
+    key = ctx.ks_key(dka, dkb, k, pids)
+    o0, o1 = ctx.empty((count, L, rns.n)), ctx.empty((count, L, rns.n))
+    ctx.relin_apply_key(dc, 2, 0b111, key, log_bs, o0, o1, count, pids)
+    g0, g1 = o0.download(), o1.download()
+    for c in sorted({0, count - 1}):
+        e0, e1 = rns.relin_apply(comps[c], kas, kbs, log_bs)
+        assert np.array_equal(g0[c], e0) and np.array_equal(g1[c], e1), c
+    p0, p1 = ctx.empty((count, L, rns.n)), ctx.empty((count, L, rns.n))
+    ctx.relin_apply(dc, 2, 0b111, dka, dkb, k, log_bs, p0, p1, count, pids)
+    assert np.array_equal(p0.download(), g0) and np.array_equal(p1.download(), g1)
+    gal = np.ascontiguousarray(comps[:, :2])
+    dg = ctx.array(gal)
+    ctx.galois_apply_key(dg, True, key, log_bs, o0, o1, count, pids)
+    g0, g1 = o0.download(), o1.download()
+    e0, e1 = rns.galois_apply(gal[0, 0], gal[0, 1], kas, kbs, log_bs)
+    assert np.array_equal(g0[0], e0) and np.array_equal(g1[0], e1)
+    key.close()
+
+
+def test_prepared_key_wide_moduli_and_checks(ctx):
+    """Moduli of 2^50 and above keep no FP64 copy (the integer key switch
+    reads the Montgomery words); a key is tied to its moduli."""
+    log_n, L = 13, 2
+    qs = ntt_primes(log_n, L, 60)
+    rns = oracle.Rns(qs, log_n)
+    pids = ctx.primes(qs, log_n)
+    log_bs = [20] * L
+    k, _ = rns.gadget_dims(log_bs)
+    comps = rns.uniform(41, 3)[None]
+    kas, kbs = rns.uniform(42, k), rns.uniform(43, k)
+    dka, dkb = ctx.array(kas), ctx.array(kbs)
+    key = ctx.ks_key(dka, dkb, k, pids)
+    o0, o1 = ctx.empty((1, L, rns.n)), ctx.empty((1, L, rns.n))
+    ctx.relin_apply_key(ctx.array(comps), 2, 0b111, key, log_bs, o0, o1, 1, pids)
+    e0, e1 = rns.relin_apply(comps[0], kas, kbs, log_bs)
+    assert np.array_equal(o0.download()[0], e0) and np.array_equal(o1.download()[0], e1)
+    other = ctx.primes(ntt_primes(log_n, L), log_n)
+    with pytest.raises(sg.ShgError, match="other RNS moduli"):
+        ctx.relin_apply_key(ctx.array(comps), 2, 0b111, key, log_bs, o0, o1, 1, other)
+    key.close()
