@@ -539,8 +539,11 @@ int shg_prime_create(shg_context* ctx, uint64_t q, int log_n, int* prime) {
     fw[2 * i + 1] = std::get<1>(rec->ntt.psis_bitrev_constant[i]);
     iw[2 * i] = std::get<0>(rec->ntt.psis_inv_bitrev_constant[i]);
     iw[2 * i + 1] = std::get<1>(rec->ntt.psis_inv_bitrev_constant[i]);
-    fwd[i] = (double)fw[2 * i];
-    iwd[i] = (double)iw[2 * i];
+    // FP64 twiddles in balanced form (w - q for w > q/2): |w| <= q/2
+    // halves the quotient error of every FP64 product, which the reduction
+    // schedules rely on (shg_ntt_core.h, shg_keyswitch.hip).
+    fwd[i] = fw[2 * i] > q / 2 ? (double)fw[2 * i] - qd : (double)fw[2 * i];
+    iwd[i] = iw[2 * i] > q / 2 ? (double)iw[2 * i] - qd : (double)iw[2 * i];
   }
   const size_t bytes = 6 * n * 8;
   hipError_t e = hipMalloc(&rec->d_tables, bytes);
@@ -1426,8 +1429,11 @@ int shg_ks_key_create(shg_context* ctx, const uint64_t* key_as, const uint64_t* 
       (void)hipFree(key->f64_as);  // already failing; report the allocation
       return fail(SHG_E_NO_MEMORY, "key copy of %zu bytes failed.", words * 8);
     }
-    hipError_t e = shg::launch_key_to_f64(key_as, key->f64_as, words, ctx->stream);
-    if (e == hipSuccess) e = shg::launch_key_to_f64(key_bs, key->f64_bs, words, ctx->stream);
+    hipError_t e = shg::launch_key_to_f64(key_as, key->f64_as, ctx->d_primes, lp,
+                                          num_key_polys, num_limbs, log_n, ctx->stream);
+    if (e == hipSuccess)
+      e = shg::launch_key_to_f64(key_bs, key->f64_bs, ctx->d_primes, lp, num_key_polys,
+                                 num_limbs, log_n, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) {
       (void)hipFree(key->f64_as);  // already failing; report the launch

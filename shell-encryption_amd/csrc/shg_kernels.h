@@ -129,9 +129,12 @@ struct KeySwitchArgs {
   int accumulate;            // 1: out += ..., else out = init + ...
   int key_f64;               // 1: key words are FP64 values (launch_key_to_f64)
 };
-// out[i] = the bits of (double)in[i] for residues below 2^50 (exact): the
+// out = the residues of `count` polynomials (num_limbs limbs of 2^logn words)
+// as FP64 values in balanced form (w - q for w > q/2; exact below 2^50): the
 // key format the FP64 key switch reads without conversion.
-hipError_t launch_key_to_f64(const uint64_t* in, uint64_t* out, size_t words, hipStream_t s);
+hipError_t launch_key_to_f64(const uint64_t* in, uint64_t* out, const PrimeDev* primes,
+                             const LimbPrimes& lp, int count, int num_limbs, int logn,
+                             hipStream_t s);
 hipError_t launch_keyswitch(const KeySwitchArgs& ka, const GadgetDesc& gd, int variant,
                             const PrimeDev* primes, const LimbPrimes& lp, int count,
                             int num_limbs, int logn, hipStream_t s);

@@ -83,14 +83,27 @@ struct F64KsArith : F64Arith {
   __host__ __device__ static constexpr int ks_acc_every(int logn) { return ks_prep(logn) ? 8 : 3; }
 };
 
-// Keys held as FP64 values (shg_ks_key_create): the multiply-accumulate
-// takes each key word as it is, where F64KsArith converts it (an OR and a
-// subtraction per word).  Same products, same accumulated words.
+// Keys held as balanced FP64 values (shg_ks_key_create, |w| <= q/2): the
+// multiply-accumulate takes each key word as it is, where F64KsArith
+// converts it (an OR and a subtraction per word), and the tighter bounds
+// allow a lazier schedule.  With |w| <= q/2 (balanced twiddles and keys) an
+// FP64 product has |t| <= q/2 + 3 * 2^-53 |x| |w| <= q/2 + 0.1875 |x|, so a
+// Cooley-Tukey stage maps a bound B to 1.1875 B + q/2.  Stage 0 (digits
+// below 2^16 <= q/64) leaves B <= 0.532 q; stages 1-7 reach 7.99 q < 8 q,
+// one reduction follows stage 7 (B <= q/2), and stages 8-13 reach 6.22 q.
+// Those outputs enter the multiply-accumulate unreduced: each term is below
+// q/2 + 0.1875 * 6.22 q = 1.67 q, so four of them on a canonical or reduced
+// accumulator (<= q) stay below 7.7 q < 8 q < 2^53, and every sum and
+// product term is an exact integer.  Same residues, canonical outputs.
 struct F64KsKeyArith : F64KsArith {
   using F64KsArith::F64KsArith;
   __device__ __forceinline__ void mac(Acc& a, Prep d, uint64_t key, const PrimeDev&) const {
     a += f64_mulmod_h(d, __longlong_as_double((long long)key), q, qinv);
   }
+  __device__ static constexpr bool fwd_reduce_at(int s) { return s == 7; }
+  __host__ __device__ static constexpr int last_reduce(int logn) { return logn - 1 >= 7 ? 7 : -1; }
+  __host__ __device__ static constexpr bool ks_prep(int) { return false; }
+  __host__ __device__ static constexpr int ks_acc_every(int) { return 4; }
 };
 
 // LOGN = LOGM + HB.  With HB = 1 (n = 2^14) the limb is split between two

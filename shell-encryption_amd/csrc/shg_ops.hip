@@ -165,15 +165,23 @@ hipError_t launch_fused_mul_sum_add(bool subtract, uint64_t* x, const uint64_t* 
 }
 
 __global__ void key_to_f64_kernel(const uint64_t* __restrict__ in, uint64_t* __restrict__ out,
-                                  size_t words) {
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < words) out[i] = (uint64_t)__double_as_longlong(u52_to_f64(in[i]));
+                                  const PrimeDev* __restrict__ primes, LimbPrimes lp,
+                                  int num_limbs, int logn, long long total) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  const uint64_t q = primes[lp.id[(i >> logn) % num_limbs]].q;
+  const uint64_t w = in[i];
+  const double v = w > (q >> 1) ? u52_to_f64(w) - u52_to_f64(q) : u52_to_f64(w);
+  out[i] = (uint64_t)__double_as_longlong(v);
 }
 
-hipError_t launch_key_to_f64(const uint64_t* in, uint64_t* out, size_t words, hipStream_t s) {
-  if (words == 0) return hipSuccess;
-  hipLaunchKernelGGL(key_to_f64_kernel, dim3(grid_for((long long)words, 256)), dim3(256), 0, s, in,
-                     out, words);
+hipError_t launch_key_to_f64(const uint64_t* in, uint64_t* out, const PrimeDev* primes,
+                             const LimbPrimes& lp, int count, int num_limbs, int logn,
+                             hipStream_t s) {
+  const long long total = ((long long)count * num_limbs) << logn;
+  if (total == 0) return hipSuccess;
+  hipLaunchKernelGGL(key_to_f64_kernel, dim3(grid_for(total, 256)), dim3(256), 0, s, in, out,
+                     primes, lp, num_limbs, logn, total);
   return hipGetLastError();
 }
 

@@ -604,3 +604,34 @@ def test_prepared_key_wide_moduli_and_checks(ctx):
     with pytest.raises(sg.ShgError, match="other RNS moduli"):
         ctx.relin_apply_key(ctx.array(comps), 2, 0b111, key, log_bs, o0, o1, 1, other)
     key.close()
+
+
+@pytest.mark.parametrize("log_n,log_b", [(14, 16), (14, 10), (13, 16)])
+def test_prepared_key_extreme_inputs(ctx, log_n, log_b):
+    """The prepared-key schedule's bounds at their edge: c2 whose every
+    coefficient exports to q - 1 (for these moduli just below 2^50, digits
+    of all ones in nearly every position), c0 and c1 at q - 1, and key
+    residues at floor(q/2), floor(q/2) + 1, q - 1 and 0 (the largest
+    balanced magnitudes and both signs)."""
+    L = 3
+    qs, rns = _rns(log_n, L)
+    pids = ctx.primes(qs, log_n)
+    log_bs = [log_b] * L
+    k, _ = rns.gadget_dims(log_bs)
+    comps = np.empty((1, 3, L, rns.n), dtype=np.uint64)
+    kas = np.empty((k, L, rns.n), dtype=np.uint64)
+    kbs = np.empty_like(kas)
+    for i, q in enumerate(qs):
+        comps[0, :2, i, :] = q - 1
+        comps[0, 2, i, :] = rns.primes[i].mont.imp(q - 1)
+        pat = np.array([q // 2, q // 2 + 1, q - 1, 0], dtype=np.uint64)
+        kas[:, i, :] = np.resize(pat, rns.n)
+        kbs[:, i, :] = np.resize(pat[::-1], rns.n)
+    comps[0, 2] = rns.ntt_fwd(comps[0, 2])  # NTT form: the digit-plane path
+    dka, dkb = ctx.array(kas), ctx.array(kbs)
+    key = ctx.ks_key(dka, dkb, k, pids)
+    o0, o1 = ctx.empty((1, L, rns.n)), ctx.empty((1, L, rns.n))
+    ctx.relin_apply_key(ctx.array(comps), 2, 0b111, key, log_bs, o0, o1, 1, pids)
+    e0, e1 = rns.relin_apply(comps[0], kas, kbs, log_bs)
+    assert np.array_equal(o0.download()[0], e0) and np.array_equal(o1.download()[0], e1)
+    key.close()
