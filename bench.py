@@ -468,7 +468,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="skip the secondary measurements")
     ap.add_argument("--copy-probe", action="store_true",
@@ -580,7 +580,7 @@ def main():
     roofline = {"bound": "valu", "achieved": round(valu_tops, 2), "peak": FP64_OPS_PEAK_T,
                 "unit": "T FP64 ops/s", "frac": round(valu_tops / FP64_OPS_PEAK_T, 4),
                 "traffic": measured_traffic(),
-                "kernel": "keyswitch_kernel<13,4,1,F64KsArith,planes>",
+                "kernel": "keyswitch_kernel<13,4,1,F64KsKeyArith,planes>",
                 "avg_launch_ms": round(ks_avg_ms, 4),
                 "ops_per_launch": ops,
                 "op_costs": {"butterflies_per_key_switch": bfly_per_ks, "ops_per_butterfly": 8,
