@@ -41,3 +41,11 @@ def test_error_paths_without_device():
     assert L.shg_sync(None) == -1
     assert L.shg_ntt_fwd(None, None, 1, 1, None, 0) == -1
     assert b"context" in L.shg_last_error()
+    # Entry points added with the lazy absorbs, prepared keys and the autotuner.
+    assert L.shg_fused_mul_sum_add(None, 0, None, None, None, None, 1, 1, None) == -1
+    assert L.shg_autotune_ntt_moduli(None, None, 1, 1, None) == -1
+    assert L.shg_ks_key_create(None, None, None, 1, 1, None, None) == -1
+    assert L.shg_ks_key_destroy(None, None) == -1
+    assert L.shg_relin_apply_key(None, None, 2, 7, None, None, None, None, 1, 1, None) == -1
+    assert b"key" in L.shg_last_error()
+    assert L.shg_galois_apply_key(None, None, 1, None, None, None, None, 1, 1, None) == -1
