@@ -220,3 +220,50 @@ def test_batch_mod_reduce_uneven(ctx, is_ntt):
     with pytest.raises(sg.ShgError):
         plan.run(sg.BATCH_NTT_FWD)
     plan.close()
+
+
+def test_graph_prepared_key_relin_and_galois_n16384(ctx):
+    """Key switches with a prepared key (the FP64-key kernel at 2^14)
+    captured after one eager call at the same size and replayed three times,
+    with eager work and synchronisation between replays, against the eager
+    result and the oracle."""
+    log_n, L, cnt = 14, 3, 3
+    qs = ntt_primes(log_n, L)
+    rns = oracle.Rns(qs, log_n)
+    pids = ctx.primes(qs, log_n)
+    n = 1 << log_n
+    log_bs = [10] * L
+    k, _ = rns.gadget_dims(log_bs)
+    comps = np.stack([rns.uniform(900 + c, 3) for c in range(cnt)])
+    kas, kbs = rns.uniform(17, k), rns.uniform(18, k)
+    dc, dka, dkb = ctx.array(comps), ctx.array(kas), ctx.array(kbs)
+    key = ctx.ks_key(dka, dkb, k, pids)
+    o0, o1 = ctx.empty((cnt, L, n)), ctx.empty((cnt, L, n))
+    g0, g1 = ctx.empty((cnt, L, n)), ctx.empty((cnt, L, n))
+    gal = ctx.array(np.ascontiguousarray(comps[:, :2]))
+
+    def chain():
+        ctx.relin_apply_key(dc, 2, 0b111, key, log_bs, o0, o1, cnt, pids)
+        ctx.galois_apply_key(gal, True, key, log_bs, g0, g1, cnt, pids)
+
+    chain()
+    ctx.sync()
+    eager = [b.download() for b in (o0, o1, g0, g1)]
+    r0, r1 = rns.relin_apply(comps[0], kas, kbs, log_bs)
+    assert np.array_equal(eager[0][0], r0) and np.array_equal(eager[1][0], r1)
+    s0, s1 = rns.galois_apply(comps[cnt - 1, 0], comps[cnt - 1, 1], kas, kbs, log_bs)
+    assert np.array_equal(eager[2][cnt - 1], s0) and np.array_equal(eager[3][cnt - 1], s1)
+    ctx.graph_begin()
+    chain()
+    graph = ctx.graph_end()
+    zero = np.zeros((cnt, L, n), dtype=np.uint64)
+    for _ in range(3):
+        for b in (o0, o1, g0, g1):
+            b.upload(zero)
+        ctx.sync()
+        graph.launch()
+        ctx.sync()
+        for b, e in zip((o0, o1, g0, g1), eager):
+            assert np.array_equal(b.download(), e)
+    graph.close()
+    key.close()

@@ -635,3 +635,40 @@ def test_prepared_key_extreme_inputs(ctx, log_n, log_b):
     e0, e1 = rns.relin_apply(comps[0], kas, kbs, log_bs)
     assert np.array_equal(o0.download()[0], e0) and np.array_equal(o1.download()[0], e1)
     key.close()
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_relin_random_shapes(ctx, seed):
+    """Relinearization on randomly drawn shapes against the oracle: ring
+    degree 2^4..2^15, 1..8 moduli of 30..50 bits, a gadget base per limb
+    from 2^2 to 2^24 (digit planes, the stage-0 table and the unfused form
+    as the draw falls), degree 2 or 3, NTT or coefficient-form sources, and
+    the plain or prepared key."""
+    r = np.random.default_rng(1000 + seed)
+    log_n = int(r.integers(4, 16))
+    L = int(r.integers(1, 9 if log_n <= 14 else 3))
+    bound = int(r.integers(30, 51))
+    qs = ntt_primes(log_n, L, bound)
+    rns = oracle.Rns(qs, log_n)
+    pids = ctx.primes(qs, log_n)
+    log_bs = [int(b) for b in r.integers(2, 25, size=L)]
+    k, _ = rns.gadget_dims(log_bs)
+    degree = int(r.integers(2, 4))
+    mask = 0b11 | (int(r.integers(0, 1 << (degree - 1))) << 2)
+    count = int(r.integers(1, 4))
+    comps = np.stack([rns.uniform(2000 + 7 * seed + c, degree + 1) for c in range(count)])
+    nk = k * (degree - 1)
+    kas, kbs = rns.uniform(3000 + seed, nk), rns.uniform(4000 + seed, nk)
+    dc, dka, dkb = ctx.array(comps), ctx.array(kas), ctx.array(kbs)
+    o0, o1 = ctx.empty((count, L, rns.n)), ctx.empty((count, L, rns.n))
+    if seed % 2:
+        key = ctx.ks_key(dka, dkb, nk, pids)
+        ctx.relin_apply_key(dc, degree, mask, key, log_bs, o0, o1, count, pids)
+        key.close()
+    else:
+        ctx.relin_apply(dc, degree, mask, dka, dkb, nk, log_bs, o0, o1, count, pids)
+    g0, g1 = o0.download(), o1.download()
+    info = (log_n, L, bound, log_bs, degree, mask, count)
+    for c in range(count):
+        e0, e1 = rns.relin_apply(comps[c], kas, kbs, log_bs, ntt_mask=mask)
+        assert np.array_equal(g0[c], e0) and np.array_equal(g1[c], e1), (info, c)
