@@ -1,6 +1,6 @@
 """ctypes front-end of the CPU oracle (oracle/shell_oracle.c).
 
-TEST INFRASTRUCTURE ONLY.  The oracle is the checker for the HIP path: only
+TEST INFRASTRUCTURE ONLY.  The HIP path is tested against the oracle: only
 tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import this
 module.  The product library never routes through it.
 

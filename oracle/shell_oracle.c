@@ -1,7 +1,8 @@
 /*
  * shell_oracle.c -- CPU restatement of SHELL's RNS polynomial hot path.
  *
- * TEST INFRASTRUCTURE ONLY: the checker for the HIP path, never the product.
+ * TEST INFRASTRUCTURE ONLY: the HIP path is tested against it; it is never
+ * part of the product.
  * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load
  * it (through oracle/oracle.py).
  *

@@ -1,7 +1,7 @@
 /*
  * shell_oracle.h -- CPU restatement of SHELL's RNS polynomial hot path.
  *
- * TEST INFRASTRUCTURE ONLY.  This oracle is the checker for the HIP path:
+ * TEST INFRASTRUCTURE ONLY.  The HIP path is tested against this oracle:
  * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
  * load it.  The product library (libshellgpu.so) never links or calls it.
  *

@@ -8,7 +8,7 @@
 // second, fresh PRNG, and b = t*e + g_j*s' - a*s is formed with device
 // pointwise operations.  CreateFromKeyComponents builds either key from given
 // columns (the reference offers it for RnsGaloisKey; for RnsRelinKey it is
-// an addition of this build, for keys made elsewhere).  ApplyTo runs the
+// an addition to the reference's API, for keys made elsewhere).  ApplyTo runs the
 // fused key-switch kernel; ApplyToBatch takes many ciphertexts in one launch.
 // RnsGaloisKey::Compose is one batched Galois key switch of the other key's
 // substituted columns (rns/rns_galois_key.cc:349-406).

@@ -1,6 +1,6 @@
 // The serialized messages of the RNS path (rns/rns_serialization.proto:27-67)
 // and PrngType (serialization.proto:45-49), with the protobuf binary
-// encoding written out by hand (no protobuf runtime in this build):
+// encoding written out by hand (no protobuf runtime is linked):
 //   SerializedRnsPolynomial     1: int32 log_n   2: repeated bytes coeff_vectors
 //                               3: bool is_ntt
 //   SerializedRnsRlweCiphertext 1: repeated SerializedRnsPolynomial components
