@@ -335,6 +335,12 @@ int shg_sample_uniform_chacha(shg_context* ctx, const uint8_t* host_seed, size_t
 int shg_oblivious_expand_level(shg_context* ctx, uint64_t* const* host_cts, int count,
                                int level, const uint64_t* key_as, const uint64_t* key_bs,
                                const int* host_log_bs, int num_limbs, const int* primes);
+/* shg_oblivious_expand_level with the level's Galois key prepared
+ * (shg_ks_key_create); same results. */
+int shg_oblivious_expand_level_key(shg_context* ctx, uint64_t* const* host_cts,
+                                   int count, int level, const shg_ks_key* key,
+                                   const int* host_log_bs, int num_limbs,
+                                   const int* primes);
 /* All levels 0..levels-1; key_as / key_bs are host arrays of one device
  * key pointer per level. */
 int shg_oblivious_expand(shg_context* ctx, uint64_t* const* host_cts, int count, int levels,

@@ -1070,10 +1070,10 @@ static hipError_t keyswitch_fused_one(shg_context* ctx, int v, uint16_t* planes,
   ka.c0_stride = ct_words;
   ka.c1 = init_c1 ? comps + poly_words : nullptr;
   ka.c1_stride = ct_words;
-  // FP64-form copies of the keys (shg_ks_key) serve the digit-plane kernel
-  // at 2^13 and 2^14; every other path reads the Montgomery words.
+  // FP64-form copies of the keys (shg_ks_key) serve the digit-plane kernels
+  // at 2^12..2^14; every other path reads the Montgomery words.
   const bool f64_keys = f64_as && f64_bs && planes && (v & 3) == SHG_NTT_F64 && gd.small_digits &&
-                        log_n >= 13;
+                        log_n >= 12;
   ka.key_a = f64_keys ? f64_as : key_as;
   ka.key_b = f64_keys ? f64_bs : key_bs;
   ka.key_f64 = f64_keys ? 1 : 0;
@@ -1520,9 +1520,31 @@ static int expand_monomial(shg_context* ctx, const int* primes, int num_limbs, i
   return SHG_OK;
 }
 
+static int expand_level_impl(shg_context* ctx, uint64_t* const* host_cts, int count, int level,
+                             const uint64_t* key_as, const uint64_t* key_bs,
+                             const int* host_log_bs, int num_limbs, const int* primes,
+                             const uint64_t* f64_as, const uint64_t* f64_bs);
+
 int shg_oblivious_expand_level(shg_context* ctx, uint64_t* const* host_cts, int count, int level,
                                const uint64_t* key_as, const uint64_t* key_bs,
                                const int* host_log_bs, int num_limbs, const int* primes) {
+  return expand_level_impl(ctx, host_cts, count, level, key_as, key_bs, host_log_bs, num_limbs,
+                           primes, nullptr, nullptr);
+}
+
+int shg_oblivious_expand_level_key(shg_context* ctx, uint64_t* const* host_cts, int count,
+                                   int level, const shg_ks_key* key, const int* host_log_bs,
+                                   int num_limbs, const int* primes) {
+  int rc = check_key(key, num_limbs, primes);
+  if (rc) return rc;
+  return expand_level_impl(ctx, host_cts, count, level, key->key_as, key->key_bs, host_log_bs,
+                           num_limbs, primes, key->f64_as, key->f64_bs);
+}
+
+static int expand_level_impl(shg_context* ctx, uint64_t* const* host_cts, int count, int level,
+                             const uint64_t* key_as, const uint64_t* key_bs,
+                             const int* host_log_bs, int num_limbs, const int* primes,
+                             const uint64_t* f64_as, const uint64_t* f64_bs) {
   LimbPrimes lp{};
   int log_n = 0;
   int rc = check_primes(ctx, primes, num_limbs, &log_n, &lp);
@@ -1571,7 +1593,8 @@ int shg_oblivious_expand_level(shg_context* ctx, uint64_t* const* host_cts, int 
   }
   const int src_comp = 1, src_ntt = 1, base = 0;
   rc = keyswitch_locked(ctx, S, 2, &src_comp, &src_ntt, &base, 1, false, key_as, key_bs,
-                        host_log_bs, sub0, sub1, (int)cts, num_limbs, primes, lp, log_n, gd);
+                        host_log_bs, sub0, sub1, (int)cts, num_limbs, primes, lp, log_n, gd,
+                        f64_as, f64_bs);
   if (rc) return rc;
   hipError_t e = shg::launch_expand_update(d_cts, count, m, sub0, sub1, mono, ctx->d_primes, lp,
                                            log_n, num_limbs, ctx->stream);

@@ -378,11 +378,10 @@ __global__ void __launch_bounds__((1 << LOGM) >> LOGE)
 // thread (wave w, lane l) holds, for gg = 0..3, the four consecutive words
 // tp4 + 256 gg + 0..3 of the slice.  Workgroup order and remap as in
 // keyswitch_kernel.
-template <int LW, int HB>
+template <int LW, int HB, class A>
 __global__ void __launch_bounds__(64 << LW)
     keyswitch_wave_kernel(KeySwitchArgs ka, GadgetDesc gd, const PrimeDev* __restrict__ primes,
                           LimbPrimes lp, int count, int num_limbs) {
-  using A = F64KsArith;
   using V = double;
   using Lay = WaveLayout<LW>;
   constexpr int LOGM = Lay::LOGM;
@@ -587,13 +586,13 @@ __global__ void __launch_bounds__(64 << LW)
   }
 }
 
-template <int LW, int HB>
+template <int LW, int HB, class A = F64KsArith>
 static hipError_t launch_ks_wave(const KeySwitchArgs& ka, const GadgetDesc& gd,
                                  const PrimeDev* primes, const LimbPrimes& lp, int count,
                                  int num_limbs, hipStream_t s) {
   constexpr int T = 64 << LW;
   const size_t shmem = (size_t)lds_words<10 + LW>() * 8;
-  auto k = keyswitch_wave_kernel<LW, HB>;
+  auto k = keyswitch_wave_kernel<LW, HB, A>;
   if (shmem > 65536) set_max_dynamic_shmem((const void*)k, shmem);
   hipLaunchKernelGGL(k, dim3((unsigned)(count * num_limbs) << HB), dim3(T), shmem, s, ka, gd,
                      primes, lp, count, num_limbs);
@@ -627,7 +626,8 @@ static hipError_t dispatch_ks(int logn, const KeySwitchArgs& ka, const GadgetDes
   // either way and keyswitch_kernel measured 5% and 6% faster (2^14, L = 8:
   // 10.55 vs 11.2 ms per 512 ciphertexts).
   if constexpr (std::is_same<A, F64KsKeyArith>::value) {
-    // FP64-form keys: the ring degrees whose default is keyswitch_kernel.
+    // FP64-form keys at 2^12 (the wave-exchange kernel), 2^13 and 2^14.
+    if (logn == 12) return launch_ks_wave<2, 0, A>(ka, gd, primes, lp, count, num_limbs, s);
     if (logn == 13) return SHG_KS(13, 4, 0);
     if (logn == 14) return SHG_KS(13, SHG_KS_LOGE, 1);
     return hipErrorInvalidValue;
@@ -661,7 +661,7 @@ static hipError_t dispatch_ks(int logn, const KeySwitchArgs& ka, const GadgetDes
 hipError_t launch_keyswitch(const KeySwitchArgs& ka, const GadgetDesc& gd, int variant,
                             const PrimeDev* primes, const LimbPrimes& lp, int count, int num_limbs,
                             int logn, hipStream_t s) {
-  if (ka.key_f64 && !(variant == SHG_NTT_F64 && gd.small_digits && ka.planes && logn >= 13))
+  if (ka.key_f64 && !(variant == SHG_NTT_F64 && gd.small_digits && ka.planes && logn >= 12))
     return hipErrorInvalidValue;  // FP64-form keys are read by that kernel only
   if (variant == SHG_NTT_F64) {
     if (gd.small_digits) {

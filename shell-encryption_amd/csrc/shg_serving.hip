@@ -68,6 +68,7 @@ struct shg_pir_server {
   std::vector<int> primes;
   std::vector<int> log_bs;
   std::vector<const uint64_t*> key_as, key_bs;
+  std::vector<shg_ks_key*> keys;  // the level keys, prepared at creation
   int levels = 0;
   int log_n = 0;
   size_t ct_words = 0;  // 2 * L * n
@@ -118,8 +119,8 @@ int shg_pir_server::step(std::vector<Slot*>& active) {
     for (Slot* sl : active)
       if (sl->stage == j) ptrs.push_back(sl->expanded);
     if (ptrs.empty()) continue;
-    int rc = shg_oblivious_expand_level(ctx, ptrs.data(), (int)ptrs.size(), j, key_as[j],
-                                        key_bs[j], log_bs.data(), L, primes.data());
+    int rc = shg_oblivious_expand_level_key(ctx, ptrs.data(), (int)ptrs.size(), j, keys[j],
+                                            log_bs.data(), L, primes.data());
     if (rc) return rc;
     ++stage_runs;
     batched_items += (long long)ptrs.size();
@@ -250,6 +251,25 @@ int shg_pir_server_create(shg_context* ctx, const int* primes, int num_limbs,
   s->log_bs.assign(host_log_bs, host_log_bs + num_limbs);
   s->key_as.assign(key_as, key_as + cfg->levels);
   s->key_bs.assign(key_bs, key_bs + cfg->levels);
+  // Each level's key is prepared once (FP64-form copies for the digit-plane
+  // key switch); its dimension is the gadget's, sum_i ceil(log q_i / log b_i).
+  int k = 0;
+  for (int i = 0; i < num_limbs; ++i) {
+    uint64_t params[10];
+    int rc = shg_prime_info(ctx, primes[i], params, nullptr);
+    if (rc) return rc;
+    if (host_log_bs[i] <= 0) return set_error(SHG_E_INVALID_ARGUMENT, "gadget base log must be positive.");
+    k += (int)((params[1] + host_log_bs[i] - 1) / host_log_bs[i]);
+  }
+  for (int j = 0; j < cfg->levels; ++j) {
+    shg_ks_key* key = nullptr;
+    int rc = shg_ks_key_create(ctx, key_as[j], key_bs[j], k, num_limbs, primes, &key);
+    if (rc) {
+      for (shg_ks_key* made : s->keys) (void)shg_ks_key_destroy(ctx, made);
+      return rc;
+    }
+    s->keys.push_back(key);
+  }
   s->levels = cfg->levels;
   s->log_n = cfg->log_n;
   s->cfg = *cfg;
@@ -396,6 +416,7 @@ int shg_pir_server_destroy(shg_pir_server* s) {
     if (sl.expanded) shg_free(s->ctx, sl.expanded);
     if (sl.answer) shg_free(s->ctx, sl.answer);
   }
+  for (shg_ks_key* key : s->keys) (void)shg_ks_key_destroy(s->ctx, key);
   delete s;
   return SHG_OK;
 }

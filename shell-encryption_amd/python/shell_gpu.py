@@ -89,6 +89,7 @@ def lib():
         "shg_ks_key_destroy": [VP, VP],
         "shg_relin_apply_key": [VP, VP, i, i, VP, INTP, VP, VP, i, i, INTP],
         "shg_galois_apply_key": [VP, VP, i, VP, INTP, VP, VP, i, i, INTP],
+        "shg_oblivious_expand_level_key": [VP, ctypes.POINTER(VP), i, i, VP, INTP, i, INTP],
         "shg_batch_limbs": [VP, ctypes.POINTER(LimbJob), i, i],
         "shg_batch_plan_create": [VP, ctypes.POINTER(LimbJob), i, i, ctypes.POINTER(VP)],
         "shg_batch_plan_destroy": [VP, VP],
@@ -384,6 +385,12 @@ class Context:
         _check(lib().shg_oblivious_expand_level(self.h, self._ptrs(bufs), len(bufs), int(level),
                                                 key_as.p, key_bs.p, _ints(log_bs), len(primes),
                                                 _ints(primes)))
+
+    def expand_level_key(self, bufs, level, key, log_bs, primes):
+        """expand_level with a prepared Galois key (KsKey)."""
+        _check(lib().shg_oblivious_expand_level_key(self.h, self._ptrs(bufs), len(bufs),
+                                                    int(level), key.h, _ints(log_bs),
+                                                    len(primes), _ints(primes)))
 
     def expand(self, bufs, levels, keys, log_bs, primes):
         """keys: one (key_as, key_bs) DeviceArray pair per level."""

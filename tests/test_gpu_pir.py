@@ -84,3 +84,30 @@ def test_expansion_selects_a_row_and_decrypts(ctx):
         ans = out.download()
         for c in range(C):
             assert np.array_equal(bgv.decrypt(ans[c]), rows[sel * C + c] % t), (sel, c)
+
+
+@pytest.mark.parametrize("log_n,levels", [(12, 5), (13, 3), (10, 3)])
+def test_expand_with_prepared_keys(ctx, log_n, levels):
+    """Expansion levels with prepared Galois keys (the FP64-key kernels at
+    2^12 and 2^13, the Montgomery words below) equal the oracle's
+    composition word for word."""
+    L = 2
+    qs = ntt_primes(log_n, L)
+    rns = oracle.Rns(qs, log_n)
+    pids = ctx.primes(qs, log_n)
+    log_bs = [10] * L
+    k, _ = rns.gadget_dims(log_bs)
+    keys = [(rns.uniform(1700 + j, k), rns.uniform(1800 + j, k)) for j in range(levels)]
+    dkeys = [(ctx.array(a), ctx.array(b)) for a, b in keys]
+    pkeys = [ctx.ks_key(a, b, k, pids) for a, b in dkeys]
+    K = 1 << levels
+    queries = [rns.uniform(1900 + g, 2) for g in range(2)]
+    bufs = [ctx.array(np.concatenate([q[None], np.zeros((K - 1, 2, L, rns.n), dtype=np.uint64)]))
+            for q in queries]
+    for j in range(levels):
+        ctx.expand_level_key(bufs, j, pkeys[j], log_bs, pids)
+    ctx.sync()
+    for q, b in zip(queries, bufs):
+        assert np.array_equal(b.download(), expand_oracle(rns, q, levels, keys, log_bs))
+    for key in pkeys:
+        key.close()
