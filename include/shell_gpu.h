@@ -79,14 +79,20 @@ typedef enum shg_ntt_variant {
 /* FP64 forward and inverse transforms of 2^12..2^14-coefficient limbs with
  * a single workgroup-wide exchange per limb; the other exchanges stay inside
  * each wave, so waves run independently between limbs.  Without the other
- * flags the FP64 forward transform at 2^14 and the inverse at 2^13 and 2^14
- * already are these. */
+ * flags the FP64 inverse transform at 2^13 already is this one. */
 #define SHG_NTT_WAVE_EXCHANGE 32
 /* Stage split: 2^13..2^15-coefficient limbs as one radix-2 stage streamed
  * over HBM (first for the forward transform, last for the inverse) plus
  * 2^(log n - 1)-word slices, two per limb: twice the workgroups for batches
  * too small to fill the chip, at the cost of one more pass over the data. */
 #define SHG_NTT_STAGE_SPLIT 64
+/* FP64 forward and inverse transforms of 2^14-coefficient limbs with 32
+ * words per thread and each LDS exchange in two halves, so that a
+ * workgroup (one limb) needs 76 KiB of LDS and two of them share a CU: one
+ * computes while the other loads or stores.  Without the other flags the
+ * FP64 transforms at 2^14 already are these; other shapes and the integer
+ * variant ignore the flag. */
+#define SHG_NTT_TWO_PER_CU 128
 
 /* ---- context, memory, streams ------------------------------------------ */
 int shg_create(int device, shg_context** out);

@@ -666,7 +666,7 @@ static int ntt_common(shg_context* ctx, bool inverse, uint64_t* polys, int count
   if (count == 0) return SHG_OK;
   if (!aligned16(polys))
     return fail(SHG_E_INVALID_ARGUMENT, "polynomial data must be a non-null 16-byte aligned pointer.");
-  if (variant < 0 || (variant & 3) == 3 || (variant & ~127) != 0)
+  if (variant < 0 || (variant & 3) == 3 || (variant & ~255) != 0)
     return fail(SHG_E_INVALID_ARGUMENT, "unknown transform variant %d.", variant);
   const long long rows = (long long)count * num_limbs;
   if ((rows << (log_n > shg::kMaxLogM ? log_n - shg::kMaxLogM : 0)) > 0x7fffffffLL)
@@ -2149,7 +2149,7 @@ std::vector<int> ntt_candidates(int log_n, int wide) {
       out.push_back(b | shg::kNttCfgPrefetch);
       if (log_n >= 12) out.push_back(b | shg::kNttCfgAlt | shg::kNttCfgPrefetch);
     }
-    if (f64 && (log_n == 12 || log_n == 13)) out.push_back(b | shg::kNttCfgWave);
+    if (f64 && log_n >= 12 && log_n <= 14) out.push_back(b | shg::kNttCfgWave);
     if (log_n == 14) out.push_back(b | shg::kNttCfgSplit);
     if (log_n >= 13 && log_n <= 15) out.push_back(b | shg::kNttCfgHalf);
   }

@@ -22,13 +22,15 @@ constexpr int kMaxLogM = 14;
 // workgroups in kNttCfgSplit, FP64 forward transforms of 2^12..2^14-word
 // limbs with one workgroup-wide exchange (the rest inside each wave) in
 // kNttCfgWave, one streamed radix-2 stage plus half-limb slices for
-// 2^13..2^15-word limbs in kNttCfgHalf.  A bare variant is a valid
-// configuration.
+// 2^13..2^15-word limbs in kNttCfgHalf, FP64 transforms of 2^14-word limbs
+// with 32 words per lane and two workgroups per CU in kNttCfgPair.  A bare
+// variant is a valid configuration.
 constexpr int kNttCfgAlt = 4;
 constexpr int kNttCfgPrefetch = 8;
 constexpr int kNttCfgSplit = 16;
 constexpr int kNttCfgWave = 32;
 constexpr int kNttCfgHalf = 64;
+constexpr int kNttCfgPair = 128;
 hipError_t launch_ntt(bool inverse, int cfg, uint64_t* data, const PrimeDev* primes,
                       const LimbPrimes& lp, int rows, int num_limbs, int logn, int max_logm,
                       hipStream_t s, int inv_mode = 1);

@@ -201,8 +201,8 @@ __global__ void __launch_bounds__((1 << LOGM) >> LOGE)
       for (int e = 0; e < (1 << R); e += 2) {
 #if SHG_KS_KEY_BUF
         const int off = (int)((((uint32_t)(gg * T + t) << R) + e) * 8u);
-        const u64x2 kbv = __builtin_bit_cast(u64x2, __builtin_amdgcn_raw_buffer_load_b128(rb, off, 0, 0));
-        const u64x2 kav = __builtin_bit_cast(u64x2, __builtin_amdgcn_raw_buffer_load_b128(ra, off, 0, 0));
+        const U64Pair kbv = buffer_load_u64x2(rb, off, 0);
+        const U64Pair kav = buffer_load_u64x2(ra, off, 0);
 #else
         const u64x2 kbv = *reinterpret_cast<const u64x2*>(kb + base + e);
         const u64x2 kav = *reinterpret_cast<const u64x2*>(kap + base + e);
@@ -536,8 +536,8 @@ __global__ void __launch_bounds__(64 << LW)
       for (int e = 0; e < 4; e += 2) {
 #if SHG_KS_KEY_BUF
         const int bo = (int)((uint32_t)(tp4 + off + e) * 8u);
-        const u64x2 kbv = __builtin_bit_cast(u64x2, __builtin_amdgcn_raw_buffer_load_b128(rb, bo, 0, 0));
-        const u64x2 kav = __builtin_bit_cast(u64x2, __builtin_amdgcn_raw_buffer_load_b128(ra, bo, 0, 0));
+        const U64Pair kbv = buffer_load_u64x2(rb, bo, 0);
+        const U64Pair kav = buffer_load_u64x2(ra, bo, 0);
 #else
         const u64x2 kbv = *reinterpret_cast<const u64x2*>(kb + off + e);
         const u64x2 kav = *reinterpret_cast<const u64x2*>(kap + off + e);

@@ -492,6 +492,23 @@ struct FwdLast<LOGM, LOGE, LOGM> {
 
 typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
 
+// Two 64-bit words from one 16-byte buffer load, assembled from the load's
+// four dwords into plain scalars.  Passing them through a two-lane 64-bit
+// vector and converting a lane to double has let the optimizer (hipcc of
+// ROCm 7.2) shrink the load to its first 8 bytes, leaving the second word
+// unread; scalars keep the 16-byte load and cost no instruction.
+struct U64Pair {
+  uint64_t x, y;
+};
+__device__ __forceinline__ U64Pair buffer_load_u64x2(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+  const u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0);
+  U64Pair out;
+  out.x = (uint64_t)v.x | ((uint64_t)v.y << 32);
+  out.y = (uint64_t)v.z | ((uint64_t)v.w << 32);
+  return out;
+}
+
 // Inverse pass: stages I0..I0+R-1 (stage i toggles local bit i).  Local
 // stage i of sub-block h is global stage i with block index carrying h.
 // Stage j of the pass needs 2^(R-1-j) twiddles per group, stored at

@@ -21,6 +21,7 @@ BATCH_NTT_FWD, BATCH_NTT_INV, BATCH_MUL, BATCH_MOD_REDUCE = 0, 1, 2, 3
 NTT_AUTO, NTT_INT, NTT_F64 = 0, 1, 2
 NTT_TILING_ALT, NTT_PREFETCH, NTT_SPLIT_LIMB, NTT_WAVE_EXCHANGE = 4, 8, 16, 32
 NTT_STAGE_SPLIT = 64
+NTT_TWO_PER_CU = 128
 BASIS_EXACT, BASIS_SCALE, BASIS_APPROX = 0, 1, 2
 
 _lib = None

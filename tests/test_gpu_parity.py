@@ -454,16 +454,19 @@ CONFIGS = [sg.NTT_F64 | sg.NTT_TILING_ALT, sg.NTT_F64 | sg.NTT_PREFETCH,
            sg.NTT_F64 | sg.NTT_SPLIT_LIMB, sg.NTT_INT | sg.NTT_SPLIT_LIMB,
            sg.NTT_F64 | sg.NTT_WAVE_EXCHANGE, sg.NTT_INT | sg.NTT_WAVE_EXCHANGE,
            sg.NTT_F64 | sg.NTT_STAGE_SPLIT, sg.NTT_INT | sg.NTT_STAGE_SPLIT,
-           sg.NTT_F64 | sg.NTT_STAGE_SPLIT | sg.NTT_TILING_ALT]
+           sg.NTT_F64 | sg.NTT_STAGE_SPLIT | sg.NTT_TILING_ALT, sg.NTT_F64 | sg.NTT_TWO_PER_CU,
+           sg.NTT_INT | sg.NTT_TWO_PER_CU]
 
 
 @pytest.mark.parametrize("cfg", CONFIGS)
 @pytest.mark.parametrize("log_n,L,count", [(12, 2, 3), (13, 3, 2), (14, 4, 3), (15, 2, 2)])
 def test_ntt_launch_configs(ctx, cfg, log_n, L, count):
     """Every register tiling, the prefetching loop, the split forward
-    transform (2^14 only) and the wave-exchange forward and inverse
-    transforms (FP64, 2^12..2^14; other shapes ignore the flags), with more
-    limbs than resident workgroups at 2^14 so the persistent loops iterate.
+    transform (2^14 only), the wave-exchange forward and inverse
+    transforms (FP64, 2^12..2^14; other shapes ignore the flags) and the
+    two-workgroups-per-CU transforms (FP64, 2^14; also the default there),
+    with more limbs than resident workgroups at 2^14 so the persistent loops
+    iterate.
     The inverse is checked against the oracle on its own input as well as by
     the round trip."""
     if log_n == 14:
@@ -495,7 +498,7 @@ def test_autotune_then_auto(ctx):
     qs, rns = _rns(13, 4)
     pids = ctx.primes(qs, 13)
     chosen = ctx.autotune_ntt(13, 256)
-    assert chosen & 3 in (sg.NTT_INT, sg.NTT_F64) and chosen & ~127 == 0
+    assert chosen & 3 in (sg.NTT_INT, sg.NTT_F64) and chosen & ~255 == 0
     x = rns.uniform(77, 64)  # 256 limbs: the tuned shape
     d = ctx.array(x)
     ctx.ntt_fwd(d, 64, pids)
@@ -518,7 +521,7 @@ def test_autotune_moduli_then_auto(ctx, log_n, L, count, log_bound):
     rns = oracle.Rns(qs, log_n)
     pids = ctx.primes(qs, log_n)
     chosen = ctx.autotune_ntt_moduli(pids, count)
-    assert chosen & ~127 == 0
+    assert chosen & ~255 == 0
     if log_bound > 50:
         assert chosen & 3 == sg.NTT_INT
     x = rns.uniform(31 + log_n, count)
