@@ -35,7 +35,7 @@ LOG_B = 10
 BATCH = 512
 NTT_BATCH = 1024
 # Untimed forward + inverse pairs before each timed NTT measurement.
-NTT_WARMUP = 15
+NTT_WARMUP = 50
 
 
 def primes_for(log_n, count, log_bound=50):
@@ -616,14 +616,14 @@ def main():
         res = {}
         for name, v in (("f64", sg.NTT_F64), ("int", sg.NTT_INT)):
             # Warm-up: the GPU idled while the input was generated, and its
-            # clock takes some 20-30 launches of this size to ramp back up.
+            # clock takes tens of milliseconds of load to ramp back up.
             for _ in range(NTT_WARMUP):
                 ctx.ntt_fwd(d_x, NTT_BATCH, pids, v)
                 ctx.ntt_inv(d_x, NTT_BATCH, pids, v)
             ctx.sync()
             ctx.profile(True)
             ctx.profile_reset()
-            reps = 10
+            reps = 20
             t0 = time.perf_counter()
             for _ in range(reps):
                 ctx.ntt_fwd(d_x, NTT_BATCH, pids, v)

@@ -508,14 +508,11 @@ __global__ void __launch_bounds__(w32::kThreads, 4)
     for (int e = 0; e < 32; ++e) y[e] = lds[sl.x1_l2 + 17 * e];
     pin_words(y);
   }
-  // Exchange 2 writes into every wave's region.
-  __syncthreads();
   // The last pass's twiddle runs, wanted after the middle pass: stages 10
-  // and 11 whole, the first twiddle of stages 12 and 13.
+  // and 11 whole, the first twiddle of stage 13.
   double tw10[2], tw11[4];
   tw_run<2>(twr, 1024 + (hi3 << 1), tw10);
   tw_run<4>(twr, 2048 + (hi3 << 2), tw11);
-  const double b12 = __builtin_bit_cast(double, ld64(twr, (4096 + (hi3 << 3)) * 8, 0));
   const double b13 = __builtin_bit_cast(double, ld64(twr, (8192 + (hi3 << 4)) * 8, 0));
   __builtin_amdgcn_sched_barrier(0);
   // Stages 5-9 on register bits 4..0 of layout 2: twiddle psi_bitrev[2^s +
@@ -538,6 +535,10 @@ __global__ void __launch_bounds__(w32::kThreads, 4)
     stage(std::integral_constant<int, 8>{});
     stage(std::integral_constant<int, 9>{});
   }
+  // Exchange 2 writes into every wave's region, which waves 4-7 read in
+  // round 1 of exchange 1 (the middle pass runs before this barrier, so
+  // waves 0-3 do not wait for those reads).
+  __syncthreads();
   // Exchange 2 (inside the wave), round r: the words with index bit 4 = r
   // (layout 2: even or odd registers; layout 3: registers 16 r .. 16 r + 15).
   double z[32];
@@ -552,7 +553,10 @@ __global__ void __launch_bounds__(w32::kThreads, 4)
     wave_sync_lds();
   }
   // Stages 10-13 on register bits 3..0 of layout 3: twiddle psi_bitrev[2^s +
-  // (hi3 << (4 - kb)) + (e >> (kb + 1))].
+  // (hi3 << (4 - kb)) + (e >> (kb + 1))].  Stage 12's run is loaded here,
+  // two stages ahead of its use (earlier it would not fit in the VGPRs).
+  double tw12[8];
+  tw_run<8>(twr, 4096 + (hi3 << 3), tw12);
   ct_stage<3>(z, tw10, ar);
   w32_fwd_reduce<A, 10>(z, ar);
   ct_stage<2>(z, tw11, ar);
@@ -562,8 +566,7 @@ __global__ void __launch_bounds__(w32::kThreads, 4)
 #pragma unroll
     for (int c = 1; c < 16; ++c) v[c] = ct[c];
     double tw[16];
-    tw_from_base<8>(b12, v, tw, ar);
-    ct_stage<1>(z, tw, ar);
+    ct_stage<1>(z, tw12, ar);
     w32_fwd_reduce<A, 12>(z, ar);
     tw_from_base<16>(b13, v, tw, ar);
     ct_stage<0>(z, tw, ar);
