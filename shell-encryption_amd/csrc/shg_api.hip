@@ -820,6 +820,14 @@ static int mod_reduce_impl(shg_context* ctx, uint64_t* polys, uint64_t k,
       uni.ql_inv[i] = args.ql_inv[i];
     }
     if (v == SHG_NTT_F64) fill_modreduce_f64(ctx, uni, k);
+    if (v == SHG_NTT_F64 && log_n == 14 && is_ntt && count * (long long)num_limbs <= 0x7fffffffLL) {
+      // Two launches on the two-workgroups-per-CU transforms.
+      rc = ensure_scratch(ctx, ((size_t)count << 14) * 8);
+      if (rc) return rc;
+      hipError_t e = shg::launch_modreduce_w32(uni, (size_t)num_limbs * n, count, ctx->d_primes,
+                                               (uint64_t*)ctx->scratch, ctx->stream);
+      return e == hipSuccess ? SHG_OK : hip_fail(e, "mod_reduce launch");
+    }
     const int grid = shg::modreduce_uniform_grid(v, log_n, count);
     rc = ensure_scratch(ctx, ((size_t)grid << 16) * 8);
     if (rc) return rc;

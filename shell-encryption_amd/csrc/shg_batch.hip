@@ -20,6 +20,7 @@
 #include "shg_kernels.h"
 #include "shg_ntt_core.h"
 #include "shg_ntt_wave.h"
+#include "shg_ntt_w32.h"
 
 // 2^14-word slices of FP64 jobs on the wave-exchange layout (shg_ntt_wave.h).
 #ifndef SHG_BATCH_WAVE
@@ -469,6 +470,101 @@ hipError_t launch_modreduce_uniform(const PolyJobDev& uni, size_t stride, int co
   else
     hipLaunchKernelGGL((batch_modreduce_kernel<IntArith, true>), dim3(grid), dim3(kT), shmem, s,
                        nullptr, count, uni, stride, primes, scratch);
+  return hipGetLastError();
+}
+
+// ModReduceLsb of `count` equal polynomials in NTT form at n = 2^14 with
+// FP64 arithmetic, on the two-workgroups-per-CU transforms (shg_ntt_w32.h)
+// in two launches; the same operations as batch_modreduce_kernel's FP64
+// path, so the same words.
+// Launch 1, one workgroup per polynomial: delta = the inverse transform of
+// a_l scaled by k n^-1 2^-64 (the exported k a_l, canonical), lifted to the
+// balanced representative of ConvertModulusBalanced (v > q_l / 2 -> v - q_l)
+// and stored in scratch as a double.
+struct W32ModDelta {
+  __amdgpu_buffer_rsrc_t a, s;
+  const F64Arith* ar;
+  uint64_t half;
+  __device__ __forceinline__ void load(int vs, int k, double& lo, double& hi) const {
+    const U64Pair v = w32::ld128(a, vs, k * 1024);
+    lo = ar->load(v.x);
+    hi = ar->load(v.y);
+  }
+  __device__ __forceinline__ void store(int v1, int e, double x) const {
+    const uint64_t v = ar->fwd_out(x);
+    const double d = u52_to_f64(v) - (v > half ? ar->q : 0.0);
+    w32::st64(s, (uint64_t)__double_as_longlong(d), v1, e * 4096);
+  }
+};
+// Launch 2, one workgroup per (polynomial, limb i < L - 1): the forward
+// transform mod q_i of delta mod q_i, then a_i <- a_i c1[i] + NTT c2[i]
+// (exact products; c1 = r q_l^-1 2^-128, c2 = t q_l^-1 2^-64).
+struct W32ModLimb {
+  __amdgpu_buffer_rsrc_t s, a;
+  const F64Arith* ar;
+  double c1, c1q, c2, c2q;
+  __device__ __forceinline__ double load(int v1, int e) const {
+    const double d = __longlong_as_double((long long)w32::ld64(s, v1, e * 4096));
+    return f64_reduce(d, ar->q, ar->qinv);
+  }
+  __device__ __forceinline__ uint64_t combine(uint64_t aw, double z) const {
+    const double sum = f64_mulmod(u52_to_f64(aw), c1, c1q, ar->q) + f64_mulmod(z, c2, c2q, ar->q);
+    return f64_to_u52(f64_canon(f64_reduce(sum, ar->q, ar->qinv), ar->q));
+  }
+  __device__ __forceinline__ void store(int vs, int k, double lo, double hi) const {
+    const U64Pair ai = w32::ld128(a, vs, k * 1024);
+    U64Pair out;
+    out.x = combine(ai.x, lo);
+    out.y = combine(ai.y, hi);
+    w32::st128(a, out, vs, k * 1024);
+  }
+};
+
+__global__ void __launch_bounds__(w32::kThreads, 4)
+    modreduce_w32_delta_kernel(PolyJobDev uni, size_t stride, const PrimeDev* __restrict__ primes,
+                               uint64_t* __restrict__ scratch) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int p = blockIdx.x;
+  const int L = uni.num_limbs;
+  const PrimeDev& Pl = primes[uni.prime[L - 1]];
+  const F64Arith ar(Pl);
+  const W32ModDelta io{w32::rsrc(uni.data + (size_t)p * stride + ((size_t)(L - 1) << w32::kLogN)),
+                       w32::rsrc(scratch + ((size_t)p << w32::kLogN)), &ar, Pl.q >> 1};
+  w32::inv_limb(ar, Pl, reinterpret_cast<double*>(smem), uni.f_kl, io);
+}
+
+// Workgroups are remapped so that each XCD runs one contiguous range of
+// (polynomial, limb) items: the L - 1 limbs of a polynomial then read its
+// delta through one L2.
+__global__ void __launch_bounds__(w32::kThreads, 4)
+    modreduce_w32_limb_kernel(PolyJobDev uni, size_t stride, const PrimeDev* __restrict__ primes,
+                              const uint64_t* __restrict__ scratch) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  unsigned b = blockIdx.x;
+  if ((gridDim.x & 7) == 0) b = (b & 7) * (gridDim.x >> 3) + (b >> 3);
+  const int nr = uni.num_limbs - 1;
+  const int p = (int)b / nr;
+  const int i = (int)b - p * nr;
+  const PrimeDev& Pi = primes[uni.prime[i]];
+  const F64Arith ar(Pi);
+  const double c1 = uni.f_c1[i], c2 = uni.f_c2[i];
+  const W32ModLimb io{w32::rsrc(scratch + ((size_t)p << w32::kLogN)),
+                      w32::rsrc(uni.data + (size_t)p * stride + ((size_t)i << w32::kLogN)), &ar,
+                      c1, c1 * ar.qinv, c2, c2 * ar.qinv};
+  w32::fwd_limb(ar, Pi, reinterpret_cast<double*>(smem), io);
+}
+
+hipError_t launch_modreduce_w32(const PolyJobDev& uni, size_t stride, int count,
+                                const PrimeDev* primes, uint64_t* scratch, hipStream_t s) {
+  if (count <= 0) return hipSuccess;
+  const size_t shmem = (size_t)w32::kLdsWords * 8;
+  set_max_dynamic_shmem((const void*)modreduce_w32_delta_kernel, shmem);
+  set_max_dynamic_shmem((const void*)modreduce_w32_limb_kernel, shmem);
+  hipLaunchKernelGGL(modreduce_w32_delta_kernel, dim3(count), dim3(w32::kThreads), shmem, s, uni,
+                     stride, primes, scratch);
+  hipLaunchKernelGGL(modreduce_w32_limb_kernel, dim3(count * (uni.num_limbs - 1)),
+                     dim3(w32::kThreads), shmem, s, uni, stride, primes,
+                     (const uint64_t*)scratch);
   return hipGetLastError();
 }
 

@@ -221,6 +221,10 @@ int modreduce_uniform_grid(int variant, int log_n, int count);
 hipError_t launch_modreduce_uniform(const PolyJobDev& uni, size_t stride, int count, int variant,
                                     const PrimeDev* primes, uint64_t* scratch, int grid,
                                     hipStream_t s);
+// The same for FP64 at n = 2^14 in NTT form on the two-workgroups-per-CU
+// transforms: two launches, scratch of count * 2^14 words.
+hipError_t launch_modreduce_w32(const PolyJobDev& uni, size_t stride, int count,
+                                const PrimeDev* primes, uint64_t* scratch, hipStream_t s);
 hipError_t launch_batch_limbs(const LimbJobDev* jobs, int num_jobs, int op, int variant,
                               const PrimeDev* primes, int grid, size_t shmem, hipStream_t s);
 // FP64 forward / inverse (x n^-1) transforms of `rows` limbs of 2^15 words,
