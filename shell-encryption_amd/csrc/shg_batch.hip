@@ -503,6 +503,9 @@ struct W32ModLimb {
   __amdgpu_buffer_rsrc_t s, a;
   const F64Arith* ar;
   double c1, c1q, c2, c2q;
+  static constexpr bool kAux = true;  // a_i is read before the store
+  using Aux = U64Pair;
+  __device__ __forceinline__ Aux aux(int vs, int k) const { return w32::ld128(a, vs, k * 1024); }
   __device__ __forceinline__ double load(int v1, int e) const {
     const double d = __longlong_as_double((long long)w32::ld64(s, v1, e * 4096));
     return f64_reduce(d, ar->q, ar->qinv);
@@ -511,8 +514,7 @@ struct W32ModLimb {
     const double sum = f64_mulmod(u52_to_f64(aw), c1, c1q, ar->q) + f64_mulmod(z, c2, c2q, ar->q);
     return f64_to_u52(f64_canon(f64_reduce(sum, ar->q, ar->qinv), ar->q));
   }
-  __device__ __forceinline__ void store(int vs, int k, double lo, double hi) const {
-    const U64Pair ai = w32::ld128(a, vs, k * 1024);
+  __device__ __forceinline__ void store(int vs, int k, double lo, double hi, Aux ai) const {
     U64Pair out;
     out.x = combine(ai.x, lo);
     out.y = combine(ai.y, hi);

@@ -422,10 +422,13 @@ __global__ void __launch_bounds__(64 << LW)
 struct W32InPlaceFwd {
   __amdgpu_buffer_rsrc_t g;
   const F64Arith* ar;
+  static constexpr bool kAux = false;
+  struct Aux {};
+  __device__ __forceinline__ Aux aux(int, int) const { return {}; }
   __device__ __forceinline__ double load(int v1, int e) const {
     return ar->load(w32::ld64(g, v1, e * 4096));
   }
-  __device__ __forceinline__ void store(int vs, int k, double lo, double hi) const {
+  __device__ __forceinline__ void store(int vs, int k, double lo, double hi, Aux) const {
     U64Pair v;
     v.x = ar->fwd_out(lo);
     v.y = ar->fwd_out(hi);

@@ -171,9 +171,10 @@ __device__ __forceinline__ void inv_reduce_after(double (&x)[32], const A& ar) {
 // caller provides the LDS (kLdsWords words) and the input/output policy:
 // io.load(v1, e) returns word 512 e + 64 wave + lane (v1 = that word's
 // byte offset for e = 0) as a double of magnitude at most q, and
-// io.store(vs, k, lo, hi) receives the unreduced outputs (|x| < 8q) of
+// io.store(vs, k, lo, hi, aux) receives the unreduced outputs (|x| < 8q) of
 // index 2 lane + 128 k + 2048 wave and the one after it (vs = the byte
-// offset of the first for k = 0).
+// offset of the first for k = 0), and with IO::kAux the value io.aux(vs, k)
+// returned, loaded ahead of the store (IO::Aux is its type).
 template <class A, class IO>
 __device__ __forceinline__ void fwd_limb(const A& ar, const PrimeDev& P, double* lds, const IO& io) {
   using cptr = const double __attribute__((address_space(4)))*;
@@ -300,6 +301,14 @@ __device__ __forceinline__ void fwd_limb(const A& ar, const PrimeDev& P, double*
     ct_stage<0>(z, tw, ar);
     fwd_reduce_after<A, 13>(z, ar);
   }
+  // A store policy that reads the output positions first (IO::kAux) gets
+  // its words for k < 8 loaded now and those for k + 8 while k is stored.
+  const int vs = ((lane << 1) | (wave << 11)) * 8;
+  typename IO::Aux aux[16];
+  if constexpr (IO::kAux) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) aux[k] = io.aux(vs, k);
+  }
   // Exchange 3 (inside the wave), round r: the words with index bit 0 = r,
   // to layout S: lane l, register k holds index r + 2 l + 128 k + 2048 wave.
   double so[2][16];
@@ -315,9 +324,13 @@ __device__ __forceinline__ void fwd_limb(const A& ar, const PrimeDev& P, double*
       wave_sync_lds();
     }
   }
-  const int vs = ((lane << 1) | (wave << 11)) * 8;
 #pragma unroll
-  for (int k = 0; k < 16; ++k) io.store(vs, k, so[0][k], so[1][k]);
+  for (int k = 0; k < 16; ++k) {
+    if constexpr (IO::kAux) {
+      if (k < 8) aux[k + 8] = io.aux(vs, k + 8);
+    }
+    io.store(vs, k, so[0][k], so[1][k], aux[k]);
+  }
 }
 
 // Inverse transform of one limb by one workgroup, the output times sc
