@@ -485,6 +485,8 @@ struct W32ModDelta {
   __amdgpu_buffer_rsrc_t a, s;
   const F64Arith* ar;
   uint64_t half;
+  static constexpr bool kAll = false;
+  __device__ __forceinline__ void store_all(int, const double (&)[32]) const {}
   __device__ __forceinline__ void load(int vs, int k, double& lo, double& hi) const {
     const U64Pair v = w32::ld128(a, vs, k * 1024);
     lo = ar->load(v.x);

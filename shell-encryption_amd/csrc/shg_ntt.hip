@@ -47,6 +47,11 @@
 #define SHG_NTT15_HELD 1
 #endif
 // The key switch's inverse + digit planes at 2^14 (FP64) on the wave layout.
+// The key switch's inverse + digit planes at 2^14 (FP64) on the
+// two-workgroups-per-CU inverse.
+#ifndef SHG_DIG_W32
+#define SHG_DIG_W32 1
+#endif
 #ifndef SHG_DIG_WAVE
 #define SHG_DIG_WAVE 1
 #endif
@@ -438,6 +443,8 @@ struct W32InPlaceFwd {
 struct W32InPlaceInv {
   __amdgpu_buffer_rsrc_t g;
   const F64Arith* ar;
+  static constexpr bool kAll = false;
+  __device__ __forceinline__ void store_all(int, const double (&)[32]) const {}
   __device__ __forceinline__ void load(int vs, int k, double& lo, double& hi) const {
     const U64Pair v = w32::ld128(g, vs, k * 1024);
     lo = ar->load(v.x);
@@ -628,11 +635,72 @@ __global__ void __launch_bounds__((1 << LOGM) >> LOGE)
   }
 }
 
+// The same at n = 2^14 with FP64 arithmetic on the two-workgroups-per-CU
+// inverse: its output layout gives thread t the positions t + 512 e
+// (e = 0..31), which are exactly the 32 consecutive plane entries
+// (p % 512) * 32 + p / 512 that key-switch thread t reads, so each plane
+// takes four 16-byte stores per thread and no further exchange.
+struct W32Digits {
+  __amdgpu_buffer_rsrc_t g;
+  const F64Arith* ar;
+  uint16_t* base;  // digit 0's plane at this workgroup's first entry
+  int nd, lb;
+  static constexpr bool kAll = true;
+  __device__ __forceinline__ void load(int vs, int k, double& lo, double& hi) const {
+    const U64Pair v = w32::ld128(g, vs, k * 1024);
+    lo = ar->load(v.x);
+    hi = ar->load(v.y);
+  }
+  __device__ __forceinline__ void store(int, int, double) const {}
+  __device__ __forceinline__ void store_all(int v1, const double (&x)[32]) const {
+    uint64_t v[32];
+#pragma unroll
+    for (int e = 0; e < 32; ++e) v[e] = ar->fwd_out(x[e]);
+    const uint32_t mask = (1u << lb) - 1;
+    uint16_t* dst0 = base + (size_t)(v1 / 8) * 32;
+    for (int dd = 0; dd < nd; ++dd) {
+      const int sh = dd * lb;
+      uint32_t w[16];
+#pragma unroll
+      for (int i = 0; i < 32; i += 2)
+        w[i / 2] = ((uint32_t)(v[i] >> sh) & mask) | (((uint32_t)(v[i + 1] >> sh) & mask) << 16);
+      uint4* dst = reinterpret_cast<uint4*>(dst0 + ((size_t)dd << w32::kLogN));
+#pragma unroll
+      for (int j = 0; j < 4; ++j) dst[j] = make_uint4(w[4 * j], w[4 * j + 1], w[4 * j + 2], w[4 * j + 3]);
+    }
+  }
+};
+
+__global__ void __launch_bounds__(w32::kThreads, 4)
+    ntt_inv_digits_w32_kernel(const uint64_t* __restrict__ in, size_t in_stride,
+                              uint16_t* __restrict__ planes, GadgetDesc gd,
+                              const PrimeDev* __restrict__ primes, LimbPrimes lp, int num_limbs) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int row = blockIdx.x;
+  const int ct = row / num_limbs;
+  const int l = row - ct * num_limbs;
+  const PrimeDev& P = primes[lp.id[l]];
+  const F64Arith ar(P);
+  const W32Digits io{w32::rsrc(in + (size_t)ct * in_stride + ((size_t)l << w32::kLogN)), &ar,
+                     planes + (((size_t)ct * gd.k + gd.limb_first[l]) << w32::kLogN),
+                     gd.limb_digits[l], gd.limb_log_b[l]};
+  w32::inv_limb(ar, P, reinterpret_cast<double*>(smem), ar.ninvx, io);
+}
+
 template <int LOGM, class A>
 static hipError_t launch_inv_digits(const uint64_t* in, size_t in_stride, uint16_t* planes,
                                     const GadgetDesc& gd, const PrimeDev* primes,
                                     const LimbPrimes& lp, int count, int num_limbs,
                                     hipStream_t s) {
+  if constexpr (LOGM == 14 && std::is_same<A, F64Arith>::value && SHG_DIG_W32) {
+    static_assert(ks_planes_vc(14) == 32, "the plane layout gives a thread 32 entries");
+    const size_t shmem = (size_t)w32::kLdsWords * 8;
+    set_max_dynamic_shmem((const void*)ntt_inv_digits_w32_kernel, shmem);
+    hipLaunchKernelGGL(ntt_inv_digits_w32_kernel, dim3((unsigned)(count * num_limbs)),
+                       dim3(w32::kThreads), shmem, s, in, in_stride, planes, gd, primes, lp,
+                       num_limbs);
+    return hipGetLastError();
+  }
   constexpr int LOGE = 4;
   constexpr int T = (1 << LOGM) >> LOGE;
   const size_t shmem = (size_t)lds_words<LOGM>() * 8;

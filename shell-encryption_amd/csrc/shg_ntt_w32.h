@@ -337,7 +337,8 @@ __device__ __forceinline__ void fwd_limb(const A& ar, const PrimeDev& P, double*
 // (folded into the last stage; sc = 1 reduces exactly).  io.load(vs, k,
 // lo, hi) gives the input words of index 2 lane + 128 k + 2048 wave and the
 // one after it as doubles of magnitude at most q; io.store(v1, e, x)
-// receives the output of index 512 e + 64 wave + lane (|x| < q).
+// receives the output of index 512 e + 64 wave + lane (|x| < 2q), or with
+// IO::kAll io.store_all(v1, x) receives all 32 at once.
 template <class A, class IO>
 __device__ __forceinline__ void inv_limb(const A& ar, const PrimeDev& P, double* lds, double sc,
                                          const IO& io) {
@@ -486,8 +487,12 @@ __device__ __forceinline__ void inv_limb(const A& ar, const PrimeDev& P, double*
     }
   }
   const int v1 = ((wave << 6) | lane) * 8;
+  if constexpr (IO::kAll) {
+    io.store_all(v1, x);
+  } else {
 #pragma unroll
-  for (int e = 0; e < 32; ++e) io.store(v1, e, x[e]);
+    for (int e = 0; e < 32; ++e) io.store(v1, e, x[e]);
+  }
 }
 
 }  // namespace w32
