@@ -182,21 +182,31 @@ def stream_copy_gbs():
 
 def ntt_probe():
     """Only the FP64 transforms of the NTT leg's shape (1024 RNS polynomials
-    at n=2^14, L=8), four forward then four inverse launches: the workload of
-    the NTT counter runs."""
+    at n=2^14, L=8): NTT_WARMUP forward + inverse pairs, then four forward
+    and four inverse launches (the workload of the NTT profiles; their last
+    four dispatches of each kind are the measured ones), then the modulus
+    switch of the same polynomials, warmed up the same way."""
     import shell_gpu as sg
     qs = primes_for(LOG_N, L)
     rng = np.random.default_rng(20261203)
     ctx = sg.Context(0)
     pids = ctx.primes(qs, LOG_N)
     d_x = ctx.array(uniform(rng, qs, (NTT_BATCH,), 1 << LOG_N))
+    for _ in range(NTT_WARMUP):
+        ctx.ntt_fwd(d_x, NTT_BATCH, pids, sg.NTT_F64)
+        ctx.ntt_inv(d_x, NTT_BATCH, pids, sg.NTT_F64)
     for _ in range(4):
         ctx.ntt_fwd(d_x, NTT_BATCH, pids, sg.NTT_F64)
     for _ in range(4):
         ctx.ntt_inv(d_x, NTT_BATCH, pids, sg.NTT_F64)
+    ql_inv = ctx.ql_inv(pids)
+    for _ in range(NTT_WARMUP // 2 + 4):
+        ctx.mod_reduce_lsb(d_x, 65537, ql_inv, True, NTT_BATCH, pids)
     ctx.sync()
-    print(json.dumps({"ntt_probe": "forward and inverse FP64", "polys": NTT_BATCH,
-                      "n": 1 << LOG_N, "L": L, "launches": 8}))
+    print(json.dumps({"ntt_probe": "forward and inverse FP64, then ModReduceLsb",
+                      "polys": NTT_BATCH, "n": 1 << LOG_N, "L": L,
+                      "launches": {"ntt_fwd": NTT_WARMUP + 4, "ntt_inv": NTT_WARMUP + 4,
+                                   "mod_reduce_lsb": NTT_WARMUP // 2 + 4}}))
     return 0
 
 
