@@ -267,3 +267,44 @@ def test_graph_prepared_key_relin_and_galois_n16384(ctx):
             assert np.array_equal(b.download(), e)
     graph.close()
     key.close()
+
+
+@pytest.mark.gpu
+def test_graph_two_per_cu_ntt_and_modswitch_n16384(ctx):
+    """FP64 transforms at 2^14 (the two-workgroups-per-CU kernels) and the
+    two-launch modulus switch (inverse of the dropped limbs into scratch,
+    then one workgroup per remaining limb) captured after one eager call at
+    the same size and replayed three times, with eager uploads and
+    synchronisation between replays, against the eager result and the
+    oracle."""
+    log_n, L, cnt = 14, 4, 5
+    qs = ntt_primes(log_n, L)
+    rns = oracle.Rns(qs, log_n)
+    pids = ctx.primes(qs, log_n)
+    x = rns.uniform(4242, cnt)
+    ql_inv = ctx.ql_inv(pids)
+    d = ctx.array(x)
+
+    def chain():
+        ctx.ntt_fwd(d, cnt, pids)
+        ctx.ntt_inv(d, cnt, pids)
+        ctx.ntt_fwd(d, cnt, pids)
+        ctx.mod_reduce_lsb(d, 65537, ql_inv, True, cnt, pids)
+
+    chain()
+    ctx.sync()
+    eager = d.download()
+    # The first L - 1 limbs hold the result; compare them with the oracle.
+    for c in (0, cnt - 1):
+        exp = rns.mod_reduce_lsb(rns.ntt_fwd(x[c]), 65537, ql_inv, True)
+        assert np.array_equal(eager[c, :L - 1], exp), c
+    ctx.graph_begin()
+    chain()
+    graph = ctx.graph_end()
+    for _ in range(3):
+        d.upload(x)
+        ctx.sync()
+        graph.launch()
+        ctx.sync()
+        assert np.array_equal(d.download(), eager)
+    graph.close()
