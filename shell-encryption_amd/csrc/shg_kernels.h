@@ -163,9 +163,13 @@ hipError_t launch_ntt_inv_digits(int variant, const uint64_t* in, size_t in_stri
                                  uint16_t* planes, const GadgetDesc& gd, const PrimeDev* primes,
                                  const LimbPrimes& lp, int count, int num_limbs, int logn,
                                  hipStream_t s);
-// Words per thread of the key switch at n = 2^14 (two 2^13-word slices).
+// Words per thread (log2) of the key switch at n = 2^14 (two 2^13-word
+// slices): 8 words with 1024 threads (four waves per SIMD, four exchanges
+// per digit) measured 8.73 against 8.82 ms per 512 ciphertexts at L = 8
+// for 16 words with 512 threads (two waves per SIMD, three exchanges), and
+// its digit planes (16 entries per thread) store faster: 0.27 vs 0.32 ms.
 #ifndef SHG_KS_LOGE
-#define SHG_KS_LOGE 4
+#define SHG_KS_LOGE 3
 #endif
 constexpr int ks_planes_vc(int logn) { return logn >= 14 ? (2 << SHG_KS_LOGE) : 16; }
 

@@ -636,7 +636,9 @@ static hipError_t dispatch_ks(int logn, const KeySwitchArgs& ka, const GadgetDes
     if ((SHG_KS_WAVE_LOGN >> logn) & 1) {
       if (logn == 12) return launch_ks_wave<2, 0>(ka, gd, primes, lp, count, num_limbs, s);
       if (logn == 13) return launch_ks_wave<3, 0>(ka, gd, primes, lp, count, num_limbs, s);
-      if (logn == 14) return launch_ks_wave<3, 1>(ka, gd, primes, lp, count, num_limbs, s);
+      if constexpr (ks_planes_vc(14) == 32) {  // its plane layout at 2^14
+        if (logn == 14) return launch_ks_wave<3, 1>(ka, gd, primes, lp, count, num_limbs, s);
+      }
     }
   }
   if constexpr (!std::is_same<A, F64KsKeyArith>::value) {

@@ -637,9 +637,11 @@ __global__ void __launch_bounds__((1 << LOGM) >> LOGE)
 
 // The same at n = 2^14 with FP64 arithmetic on the two-workgroups-per-CU
 // inverse: its output layout gives thread t the positions t + 512 e
-// (e = 0..31), which are exactly the 32 consecutive plane entries
-// (p % 512) * 32 + p / 512 that key-switch thread t reads, so each plane
-// takes four 16-byte stores per thread and no further exchange.
+// (e = 0..31).  With 32 entries per key-switch thread (VC = 32) those are
+// exactly the plane entries 32 t .. 32 t + 31 ((p % 512) * 32 + p / 512);
+// with 16 (VC = 16) the even e give entries 16 t + e / 2 and the odd e
+// entries 16 (t + 512) + e / 2.  Either way each plane takes four 16-byte
+// stores per thread and no further exchange.
 struct W32Digits {
   __amdgpu_buffer_rsrc_t g;
   const F64Arith* ar;
@@ -656,17 +658,36 @@ struct W32Digits {
     uint64_t v[32];
 #pragma unroll
     for (int e = 0; e < 32; ++e) v[e] = ar->fwd_out(x[e]);
+    constexpr int VC = ks_planes_vc(w32::kLogN);
+    static_assert(VC == 32 || VC == 16, "plane layouts of 32 or 16 entries per thread");
     const uint32_t mask = (1u << lb) - 1;
-    uint16_t* dst0 = base + (size_t)(v1 / 8) * 32;
+    const int t = v1 / 8;
     for (int dd = 0; dd < nd; ++dd) {
       const int sh = dd * lb;
+      uint16_t* plane = base + ((size_t)dd << w32::kLogN);
+      auto digit = [&](int e) { return (uint32_t)(v[e] >> sh) & mask; };
       uint32_t w[16];
+      if constexpr (VC == 32) {
+        // w[i] holds entries 2i, 2i + 1 of the thread's run: e = 2i, 2i + 1.
 #pragma unroll
-      for (int i = 0; i < 32; i += 2)
-        w[i / 2] = ((uint32_t)(v[i] >> sh) & mask) | (((uint32_t)(v[i + 1] >> sh) & mask) << 16);
-      uint4* dst = reinterpret_cast<uint4*>(dst0 + ((size_t)dd << w32::kLogN));
+        for (int i = 0; i < 16; ++i) w[i] = digit(2 * i) | (digit(2 * i + 1) << 16);
+        uint4* dst = reinterpret_cast<uint4*>(plane + (size_t)t * 32);
 #pragma unroll
-      for (int j = 0; j < 4; ++j) dst[j] = make_uint4(w[4 * j], w[4 * j + 1], w[4 * j + 2], w[4 * j + 3]);
+        for (int j = 0; j < 4; ++j) dst[j] = make_uint4(w[4 * j], w[4 * j + 1], w[4 * j + 2], w[4 * j + 3]);
+      } else {
+        // Even e: entries 16 t + e / 2; odd e: 16 (t + 512) + e / 2.
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+          w[i] = digit(4 * i) | (digit(4 * i + 2) << 16);
+          w[8 + i] = digit(4 * i + 1) | (digit(4 * i + 3) << 16);
+        }
+        uint4* d0 = reinterpret_cast<uint4*>(plane + (size_t)t * 16);
+        uint4* d1 = reinterpret_cast<uint4*>(plane + (size_t)(t + 512) * 16);
+        d0[0] = make_uint4(w[0], w[1], w[2], w[3]);
+        d0[1] = make_uint4(w[4], w[5], w[6], w[7]);
+        d1[0] = make_uint4(w[8], w[9], w[10], w[11]);
+        d1[1] = make_uint4(w[12], w[13], w[14], w[15]);
+      }
     }
   }
 };
@@ -693,7 +714,6 @@ static hipError_t launch_inv_digits(const uint64_t* in, size_t in_stride, uint16
                                     const LimbPrimes& lp, int count, int num_limbs,
                                     hipStream_t s) {
   if constexpr (LOGM == 14 && std::is_same<A, F64Arith>::value && SHG_DIG_W32) {
-    static_assert(ks_planes_vc(14) == 32, "the plane layout gives a thread 32 entries");
     const size_t shmem = (size_t)w32::kLdsWords * 8;
     set_max_dynamic_shmem((const void*)ntt_inv_digits_w32_kernel, shmem);
     hipLaunchKernelGGL(ntt_inv_digits_w32_kernel, dim3((unsigned)(count * num_limbs)),
