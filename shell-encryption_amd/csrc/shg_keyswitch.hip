@@ -354,9 +354,14 @@ __global__ void __launch_bounds__((1 << LOGM) >> LOGE)
       }
     }
   }
+  // The thread's output offsets are formed again here from an opaque copy
+  // of its index: kept from the initial loads across the digit loop they
+  // would be spilled (at 8 words per thread the loop needs all 128 VGPRs).
+  int t_out = threadIdx.x;
+  asm volatile("" : "+v"(t_out));
 #pragma unroll
   for (int gg = 0; gg < G; ++gg) {
-    const size_t base = (size_t)(gg * T + t) << R;
+    const size_t base = (size_t)(gg * T + t_out) << R;
 #pragma unroll
     for (int e = 0; e < (1 << R); e += 2) {
       u64x2 a, b2;
