@@ -592,7 +592,7 @@ def main():
     roofline = {"bound": "valu", "achieved": round(valu_tops, 2), "peak": FP64_OPS_PEAK_T,
                 "unit": "T FP64 ops/s", "frac": round(valu_tops / FP64_OPS_PEAK_T, 4),
                 "traffic": measured_traffic(),
-                "kernel": "keyswitch_kernel<13,4,1,F64KsKeyArith,planes>",
+                "kernel": "keyswitch_kernel<13,3,1,F64KsKeyArith,planes>",
                 "avg_launch_ms": round(ks_avg_ms, 4),
                 "ops_per_launch": ops,
                 "op_costs": {"butterflies_per_key_switch": bfly_per_ks, "ops_per_butterfly": 8,
