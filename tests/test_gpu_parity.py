@@ -675,3 +675,30 @@ def test_relin_random_shapes(ctx, seed):
     for c in range(count):
         e0, e1 = rns.relin_apply(comps[c], kas, kbs, log_bs, ntt_mask=mask)
         assert np.array_equal(g0[c], e0) and np.array_equal(g1[c], e1), (info, c)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("L,count,other_bits,last_bits",
+                         [(2, 1, 50, 50), (12, 3, 50, 50), (5, 2, 30, 50), (5, 2, 50, 30)])
+def test_mod_reduce_lsb_n16384_two_launch_shapes(ctx, L, count, other_bits, last_bits):
+    """ModReduceLsb at n = 2^14 in NTT form with FP64 arithmetic runs as two
+    launches (the dropped limbs' inverse into scratch, then one workgroup per
+    remaining limb).  Its edges: two limbs and a single polynomial (one
+    workgroup in each launch), twelve limbs, and moduli of different widths
+    either way round, so that the balanced lift from q_l to q_i crosses
+    sizes (rns_polynomial.cc:262-300, modulus_conversion.h:30-83)."""
+    log_n = 14
+    if other_bits == last_bits:
+        qs = ntt_primes(log_n, L, other_bits)
+    else:
+        qs = ntt_primes(log_n, L - 1, other_bits) + ntt_primes(log_n, 1, last_bits)
+    rns = oracle.Rns(qs, log_n)
+    pids = ctx.primes(qs, log_n)
+    x = rns.uniform(7100 + L + other_bits, count)
+    ql_inv = ctx.ql_inv(pids)
+    d = ctx.array(x)
+    ctx.mod_reduce_lsb(d, 65537, ql_inv, True, count, pids)
+    got = d.download()
+    for c in range(count):
+        assert np.array_equal(got[c, :L - 1], rns.mod_reduce_lsb(x[c], 65537, ql_inv, True)), c
+        assert np.array_equal(got[c, L - 1], x[c, L - 1])
