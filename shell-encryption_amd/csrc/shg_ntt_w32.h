@@ -187,16 +187,22 @@ __device__ __forceinline__ void fwd_limb(const A& ar, const PrimeDev& P, double*
   const Slots sl(wave, lane);
   const int hi3 = lane | (wave << 6);  // index bits 5..13 of layout 3
 
-  // Layout 1: word e of the thread is index 512 e + 64 wave + lane.
+  // The middle pass's twiddle block, loaded first (loads complete in
+  // order) and copied into LDS below.
+  const double t2a = __builtin_bit_cast(double, ld64(twr, (32 + t) * 8, 0));
+  const double t2b = t < kTwWords - 512 ? __builtin_bit_cast(double, ld64(twr, (544 + t) * 8, 0)) : 0.0;
+  // Layout 1: word e of the thread is index 512 e + 64 wave + lane.  The
+  // words are loaded in stage-0 pairs (e, e + 16), so that the first
+  // butterflies wait for the first loads only.
   double x[32];
   {
     const int v1 = ((wave << 6) | lane) * 8;
 #pragma unroll
-    for (int e = 0; e < 32; ++e) x[e] = io.load(v1, e);
+    for (int e = 0; e < 16; ++e) {
+      x[e] = io.load(v1, e);
+      x[e + 16] = io.load(v1, e + 16);
+    }
   }
-  // The middle pass's twiddle block, loaded now and copied into LDS below.
-  const double t2a = __builtin_bit_cast(double, ld64(twr, (32 + t) * 8, 0));
-  const double t2b = t < kTwWords - 512 ? __builtin_bit_cast(double, ld64(twr, (544 + t) * 8, 0)) : 0.0;
   t2[t] = t2a;
   if (t < kTwWords - 512) t2[512 + t] = t2b;
   // Stages 0-4 on register bits 4..0: twiddle psi_bitrev[2^s + (e >> (kb + 1))],
