@@ -37,6 +37,13 @@
 // Key rows read through buffer loads (descriptor per digit in SGPRs, one
 // 32-bit lane offset shared by every digit): 10.27-10.29 vs 10.44-10.49 ms
 // per 512 ciphertexts at n = 2^14, L = 8, together with the stage-0 FMA.
+// Key words requested per batch ahead of their products (16-byte pairs of
+// each key polynomial): at 2^14 with 8 words per thread, 2 pairs measured
+// 8.63-8.64 ms per 512 ciphertexts, 1 pair 8.80, one load per product 8.65;
+// 4 pairs spill.
+#ifndef SHG_KS_KEY_BATCH
+#define SHG_KS_KEY_BATCH 2
+#endif
 #ifndef SHG_KS_KEY_BUF
 #define SHG_KS_KEY_BUF 1
 #endif
@@ -179,7 +186,11 @@ __global__ void __launch_bounds__((1 << LOGM) >> LOGE)
 
   // Forward transform of digit j's x (stage 0 already applied for HB = 1)
   // and its multiply-accumulate with key row j.
-  auto transform_and_mac = [&](int j, V (&x)[E]) {
+  // `issue` runs after the first key words are requested: the transform's
+  // twiddle loads have been waited for by then, and memory loads complete
+  // in order, so loads issued there (the next digit's plane words) hold up
+  // neither the transform nor the first products.
+  auto transform_and_mac = [&](int j, V (&x)[E], auto&& issue) {
     Passes::run(x, lds0, lds1, t, h, ar);
     if constexpr (SHG_KS_DB && (Passes::kExchanges & 1)) {
       V* tmp = lds0;
@@ -194,19 +205,52 @@ __global__ void __launch_bounds__((1 << LOGM) >> LOGE)
     const __amdgpu_buffer_rsrc_t ra =
         __builtin_amdgcn_make_buffer_rsrc(const_cast<uint64_t*>(kap), (short)0, 0x7fffffff, 0x00020000);
 #endif
+#if SHG_KS_KEY_BUF
+    // The key words are requested SHG_KS_KEY_BATCH 16-byte pairs (of each
+    // key polynomial) at a time ahead of their products, so the load
+    // latency is paid once per batch rather than once per pair.
+    constexpr int kPairs = E / 2;
+    constexpr int kBatch = SHG_KS_KEY_BATCH < kPairs ? SHG_KS_KEY_BATCH : kPairs;
+    static_assert(kPairs % kBatch == 0, "key batches must tile the pairs");
+#pragma unroll
+    for (int p0 = 0; p0 < kPairs; p0 += kBatch) {
+      U64Pair kbr[kBatch], kar[kBatch];
+#pragma unroll
+      for (int i = 0; i < kBatch; ++i) {
+        const int r = 2 * (p0 + i);
+        const int gg = r >> R, e = r & ((1 << R) - 1);
+        const int off = (int)((((uint32_t)(gg * T + t) << R) + e) * 8u);
+        kbr[i] = buffer_load_u64x2(rb, off, 0);
+        kar[i] = buffer_load_u64x2(ra, off, 0);
+      }
+      if (p0 == 0) issue();
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int i = 0; i < kBatch; ++i) {
+        const int r = 2 * (p0 + i);
+        typename A::Prep d0, d1;
+        if constexpr (kPrep) {
+          d0 = ar.prep(x[r]);
+          d1 = ar.prep(x[r + 1]);
+        } else {
+          d0 = x[r];
+          d1 = x[r + 1];
+        }
+        ar.mac(acc0[r], d0, kbr[i].x, P);
+        ar.mac(acc0[r + 1], d1, kbr[i].y, P);
+        ar.mac(acc1[r], d0, kar[i].x, P);
+        ar.mac(acc1[r + 1], d1, kar[i].y, P);
+      }
+    }
+#else
+    issue();
 #pragma unroll
     for (int gg = 0; gg < G; ++gg) {
       const size_t base = (size_t)(gg * T + t) << R;
 #pragma unroll
       for (int e = 0; e < (1 << R); e += 2) {
-#if SHG_KS_KEY_BUF
-        const int off = (int)((((uint32_t)(gg * T + t) << R) + e) * 8u);
-        const U64Pair kbv = buffer_load_u64x2(rb, off, 0);
-        const U64Pair kav = buffer_load_u64x2(ra, off, 0);
-#else
         const u64x2 kbv = *reinterpret_cast<const u64x2*>(kb + base + e);
         const u64x2 kav = *reinterpret_cast<const u64x2*>(kap + base + e);
-#endif
         const int r = gg * (1 << R) + e;
         typename A::Prep d0, d1;
         if constexpr (kPrep) {
@@ -222,6 +266,7 @@ __global__ void __launch_bounds__((1 << LOGM) >> LOGE)
         ar.mac(acc1[r + 1], d1, kav.y, P);
       }
     }
+#endif
     if (((j + 1) % kAccEvery) == 0) {
 #pragma unroll
       for (int e = 0; e < E; ++e) {
@@ -235,8 +280,8 @@ __global__ void __launch_bounds__((1 << LOGM) >> LOGE)
 
   if constexpr (PL) {
     // Digits from 16-bit planes: thread t's VC digits of a plane are
-    // contiguous (VC / 8 16-byte loads); the next digit's are loaded while
-    // this one is transformed.
+    // contiguous (VC / 8 16-byte loads); the next digit's are requested
+    // after this one's transform, behind its first key words.
     constexpr int VC = ks_planes_vc(LOGN);
     // HB = 1 with digits of at most kKsTabLogB bits: the stage-0 products
     // w * v for every digit value v (the very values ct_side forms, so the
@@ -262,10 +307,6 @@ __global__ void __launch_bounds__((1 << LOGM) >> LOGE)
     for (int w = 0; w < NW; ++w) cur[w] = pl[w];
     for (int j = 0; j < gd.k; ++j) {
       uint4 nxt[NW];
-      if (j + 1 < gd.k) {
-#pragma unroll
-        for (int w = 0; w < NW; ++w) nxt[w] = pl[(size_t)(j + 1) * kPlane + w];
-      }
       uint32_t wd[VC / 2];
 #pragma unroll
       for (int w = 0; w < NW; ++w) {
@@ -304,11 +345,15 @@ __global__ void __launch_bounds__((1 << LOGM) >> LOGE)
           for (int e = 0; e < E; ++e) x[e] = ar.ct_side(digit(e), digit(E + e), w, sgn);
         }
       }
-      transform_and_mac(j, x);
-      if (j + 1 < gd.k) {
+      // The last digit reloads its own plane (unconditional loads keep the
+      // compiler's count of outstanding loads exact at every wait).
+      const size_t jn = j + 1 < gd.k ? (size_t)(j + 1) : (size_t)j;
+      transform_and_mac(j, x, [&] {
 #pragma unroll
-        for (int w = 0; w < NW; ++w) cur[w] = nxt[w];
-      }
+        for (int w = 0; w < NW; ++w) nxt[w] = pl[jn * kPlane + w];
+      });
+#pragma unroll
+      for (int w = 0; w < NW; ++w) cur[w] = nxt[w];
     }
   } else {
     const uint64_t* src_ct = ka.src + ct * ka.src_stride;
@@ -350,7 +395,7 @@ __global__ void __launch_bounds__((1 << LOGM) >> LOGE)
         } else {
           load_digits([&](uint64_t s64) { return ar.digit_in(s64 & mask, P); });
         }
-        transform_and_mac(j, x);
+        transform_and_mac(j, x, [] {});
       }
     }
   }
