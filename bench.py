@@ -571,7 +571,7 @@ def main():
         e0, e1 = rns.relin_apply(comps[c], kas, kbs, log_bs)
         bit_exact &= bool(np.array_equal(o0[c], e0) and np.array_equal(o1[c], e1))
 
-    # Dominant kernel: keyswitch_kernel, bound by FP64 vector issue.  Work
+    # Dominant kernel: keyswitch_n14_kernel, bound by FP64 vector issue.  Work
     # per key switch (relinearization of one degree-2 ciphertext at n=2^14,
     # L=8, k=40): 320 forward limb transforms of 114,688 butterflies, each
     # butterfly one exact FMA-based modular product (6 FP64 operations) and
@@ -592,7 +592,7 @@ def main():
     roofline = {"bound": "valu", "achieved": round(valu_tops, 2), "peak": FP64_OPS_PEAK_T,
                 "unit": "T FP64 ops/s", "frac": round(valu_tops / FP64_OPS_PEAK_T, 4),
                 "traffic": measured_traffic(),
-                "kernel": "keyswitch_kernel<13,3,1,F64KsKeyArith,planes>",
+                "kernel": "keyswitch_n14_kernel<F64KsKeyArith>",
                 "avg_launch_ms": round(ks_avg_ms, 4),
                 "ops_per_launch": ops,
                 "op_costs": {"butterflies_per_key_switch": bfly_per_ks, "ops_per_butterfly": 8,

@@ -172,6 +172,16 @@ hipError_t launch_ntt_inv_digits(int variant, const uint64_t* in, size_t in_stri
 #define SHG_KS_LOGE 3
 #endif
 constexpr int ks_planes_vc(int logn) { return logn >= 14 ? (2 << SHG_KS_LOGE) : 16; }
+// FP64 digit planes at n = 2^14 read by keyswitch_n14_kernel: row r (16
+// entries, one key-switch thread's digits) holds limb positions
+// (r >> 6) + 16 (r & 63) + 1024 k, k = 0..15; the row of position p is
+// plane_row14(p & 1023).  Without it row r holds r + 1024 k.
+#ifndef SHG_KS_N14
+#define SHG_KS_N14 1
+#endif
+__host__ __device__ constexpr int plane_row14(int p) {
+  return SHG_KS_N14 ? 64 * (p & 15) + ((p >> 4) & 63) : p;
+}
 
 // Sets hipFuncAttributeMaxDynamicSharedMemorySize once per kernel and device.
 void set_max_dynamic_shmem(const void* kernel, size_t bytes);

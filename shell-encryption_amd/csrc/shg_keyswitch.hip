@@ -24,6 +24,7 @@
 #include "shg_kernels.h"
 #include "shg_ntt_core.h"
 #include "shg_ntt_wave.h"
+#include "shg_ntt_w32.h"
 
 #ifndef SHG_KS_XCD_REMAP
 #define SHG_KS_XCD_REMAP 1
@@ -55,6 +56,7 @@
 #define SHG_KS_STAGE0_TABLE 1
 #endif
 constexpr int kKsTabLogB = 11;
+
 // Ring degrees (bit log n) that take the wave-exchange kernel.
 #ifndef SHG_KS_WAVE_LOGN
 #define SHG_KS_WAVE_LOGN (1 << 12)
@@ -420,6 +422,309 @@ __global__ void __launch_bounds__((1 << LOGM) >> LOGE)
   }
 }
 
+// The FP64 digit-plane key switch at n = 2^14 with one workgroup-wide
+// exchange per digit.  A workgroup of 16 waves x 64 lanes x 8 words takes
+// half h of output limb l of one ciphertext (positions P = 2^13 h + p of the
+// limb); stage 0 combines both halves of the digit, stages 1-13 run on the
+// slice p (13 bits) through five layouts of word e of thread (wave w, lane l):
+//   A: p = w + 16 l + 2^10 e                      stages 1-3 (bits 12..10)
+//   B: p = w + 16 (l & 7) + 2^7 e + 2^10 (l >> 3) stages 4-6 (bits 9..7)
+//   C: p = w + 16 e + 2^7 (l & 7) + 2^10 (l >> 3) stages 7-9 (bits 6..4)
+//   D: p = b0 + 2 e + 16 (l & 15) + 2^8 b8 + 2^9 w, b0 = l bit 4, b8 = l bit 5
+//                                                 stages 10-12 (bits 3..1)
+//   E: p = (e & 1) + 2 l + 2^7 (e >> 1) + 2^9 w   stage 13 (bit 0), products
+// A -> B, B -> C and D -> E stay inside each wave (private LDS regions of 575
+// words; no barrier), C -> D crosses waves (one barrier).  Stage s of the
+// limb transform takes psi_bitrev[2^s + (P >> (14 - s))]: per pass of three
+// stages a run of 1, 2 and 4 twiddles from a base B0 (h in pass 1, through
+// the scalar cache; 8 h + (l >> 3), 64 h + l and 2^9 h + 2^5 w + 16 b8 +
+// (l & 15) after), and psi_bitrev[2^13 + 2^12 h + 2^8 w + 64 k + l] for the
+// pair k of stage 13.  Exchange C -> D has a single buffer and no second
+// barrier: its slot of p is 546 r + 34 a + m + (m >> 4) with m = p bits
+// 8..4 and (r, a) = (p >> 9, p & 15) for even digits, (p & 15, p >> 9) for
+// odd ones, so each wave writes a digit's words exactly where it read the
+// previous digit's (r names the reading wave in even digits and the writing
+// wave in odd ones) and no wave overwrites words another has yet to read.
+// Every slot map keeps the register index out of the lane part (immediate
+// LDS offsets) and no two lanes of a 16-lane group share a bank pair.  The
+// digit planes hold, in row t = 64 w + l, the 16 digits of limb positions
+// w + 16 l + 2^10 k (k < 8: half 0), so thread t reads row t.
+namespace ks14 {
+constexpr int kRegion = 575;                  // in-wave exchange words per wave
+constexpr int kX3Words = 8733;                // workgroup exchange words
+constexpr int kLdsWords = kX3Words + 16 * kRegion;  // then the stage-0 table
+}  // namespace ks14
+
+template <class A, int S0>
+__device__ __forceinline__ void ks14_pass(double (&x)[8], const double (&tw)[7], const A& ar) {
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    const int kb = 2 - j;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      if (e & (1 << kb)) continue;
+      ar.ct(x[e], x[e | (1 << kb)], ar.tw(tw[(1 << j) - 1 + (e >> (kb + 1))]));
+    }
+    if (A::fwd_reduce_at(S0 + j) && !(A::kSkipLastReduce && S0 + j == 13)) {
+#pragma unroll
+      for (int e = 0; e < 8; ++e) ar.fwd_reduce(x[e]);
+    }
+  }
+}
+
+// Twiddles of the pass starting at stage S0 from base b0: psi_bitrev[2^S0 +
+// b0], [2^(S0+1) + 2 b0 + 0..1], [2^(S0+2) + 4 b0 + 0..3].
+template <int S0>
+__device__ __forceinline__ void ks14_tw(__amdgpu_buffer_rsrc_t r, int b0, double (&tw)[7]) {
+  w32::tw_run<1>(r, (1 << S0) + b0, tw);
+  w32::tw_run<2>(r, (2 << S0) + 2 * b0, tw + 1);
+  w32::tw_run<4>(r, (4 << S0) + 4 * b0, tw + 3);
+}
+
+template <class A>
+__global__ void __launch_bounds__(1024)
+    keyswitch_n14_kernel(KeySwitchArgs ka, GadgetDesc gd, const PrimeDev* __restrict__ primes,
+                         LimbPrimes lp, int count, int num_limbs) {
+  using cptr = const double __attribute__((address_space(4)))*;
+  constexpr int E = 8;
+  constexpr int LOGN = 14;
+  constexpr bool kPrep = A::ks_prep(LOGN);
+  constexpr int kAccEvery = A::ks_acc_every(LOGN);
+  extern __shared__ __align__(16) unsigned char smem[];
+  double* x3 = reinterpret_cast<double*>(smem);
+  unsigned b = blockIdx.x;
+#if SHG_KS_XCD_REMAP
+  if ((gridDim.x & 7) == 0) b = (b & 7) * (gridDim.x >> 3) + (b >> 3);
+#endif
+  const int h = (int)(b & 1);
+  const int lc = (int)(b >> 1);  // l * count + ct: limb-major order
+  const int l = lc / count;
+  const int ct = lc - l * count;
+  const int t = threadIdx.x;
+  const int lane = t & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
+  const PrimeDev& P = primes[lp.id[l]];
+  const A ar(P);
+  const __amdgpu_buffer_rsrc_t twr = w32::rsrc(P.fwd);
+  const cptr ctw = (cptr)(size_t)(const void*)P.fwd;
+  double* rg = x3 + ks14::kX3Words + wave * ks14::kRegion;  // this wave's region
+  const size_t limb_off = ((size_t)l << LOGN) + ((size_t)h << (LOGN - 1));
+  const size_t poly_words = (size_t)num_limbs << LOGN;
+
+  // Layout E word offsets: pair k = e >> 1 at 2 l + 128 k + 512 w.
+  const int pe = 2 * lane + 512 * wave;
+  double acc0[E], acc1[E];
+  {
+    const uint64_t* i0 = ka.accumulate ? ka.out0 + ct * ka.out_stride
+                                       : (ka.c0 ? ka.c0 + ct * ka.c0_stride : nullptr);
+    const uint64_t* i1 = ka.accumulate ? ka.out1 + ct * ka.out_stride
+                                       : (ka.c1 ? ka.c1 + ct * ka.c1_stride : nullptr);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const size_t o = limb_off + pe + 128 * k;
+      u64x2 a = i0 ? *reinterpret_cast<const u64x2*>(i0 + o) : u64x2{0, 0};
+      u64x2 c = i1 ? *reinterpret_cast<const u64x2*>(i1 + o) : u64x2{0, 0};
+      acc0[2 * k] = ar.acc_in(a.x);
+      acc0[2 * k + 1] = ar.acc_in(a.y);
+      acc1[2 * k] = ar.acc_in(c.x);
+      acc1[2 * k + 1] = ar.acc_in(c.y);
+    }
+  }
+
+  // Stage-0 products w v for every digit value v (digits of at most
+  // kKsTabLogB bits), as in keyswitch_kernel.
+  const bool use_tab = gd.max_log_b <= kKsTabLogB;
+  double* tab = x3 + ks14::kLdsWords;
+  if (use_tab) {
+    const auto w1 = ar.fwd_tw(1);
+    for (int v = t; v < (1 << gd.max_log_b); v += 1024) tab[v] = ar.ct_product((double)v, w1);
+    __syncthreads();
+  }
+  const auto w1 = ar.fwd_tw(1);
+  const double sgn = h ? -1.0 : 1.0;
+
+  // Lane parts of the exchange slots (the register part is an immediate).
+  const int b0 = (lane >> 4) & 1, b8 = (lane >> 5) & 1;
+  const int x1w = lane;                                   // + 72 e
+  const int x1r = (lane & 7) + 72 * (lane >> 3);          // + 8 e
+  const int x2w = x1r;                                    // + 9 e
+  const int x2r = 9 * (lane & 7) + 72 * (lane >> 3);      // + e
+  const int x3w0 = (lane >> 2) * 546 + 34 * wave + 8 * (lane & 3) + ((lane & 3) >> 1);  // + e
+  const int x3w1 = 546 * wave + 34 * (lane >> 2) + 8 * (lane & 3) + ((lane & 3) >> 1);  // + e
+  const int x3r0 = 546 * wave + 34 * b0 + (lane & 15) + 17 * b8;                         // + 68 e
+  const int x3r1 = 546 * b0 + 34 * wave + (lane & 15) + 17 * b8;                         // + 1092 e
+  const int x4w = b0 + 17 * (lane & 15) + 272 * b8;       // + 2 e
+  const int x4r = 2 * lane + (lane >> 3);                 // + (e & 1) + 136 (e >> 1)
+  // Twiddle bases of passes 2-5.
+  const int tb2 = 8 * h + (lane >> 3);
+  const int tb3 = 64 * h + lane;
+  const int tb4 = 512 * h + 32 * wave + 16 * b8 + (lane & 15);
+  const int tb5 = 8192 + 4096 * h + 256 * wave + lane;
+
+  // Digit planes: row t holds this thread's 16 digits of the plane.
+  const uint4* pl = reinterpret_cast<const uint4*>(ka.planes + ((size_t)ct * gd.k << LOGN)) +
+                    (size_t)t * 2;
+  constexpr size_t kPlane = ((size_t)1 << LOGN) / 8;  // uint4 per plane
+  uint4 cur[2];
+  cur[0] = pl[0];
+  cur[1] = pl[1];
+  for (int j = 0; j < gd.k; ++j) {
+    double x[E];
+    {
+      uint32_t wd[8] = {cur[0].x, cur[0].y, cur[0].z, cur[0].w, cur[1].x, cur[1].y, cur[1].z, cur[1].w};
+      auto dig = [&](int i) { return (i & 1) ? (wd[i / 2] >> 16) : (wd[i / 2] & 0xffffu); };
+      if (use_tab) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) x[e] = fma(sgn, tab[dig(E + e)], ar.digit_in32(dig(e), P));
+      } else {
+#pragma unroll
+        for (int e = 0; e < E; ++e)
+          x[e] = ar.ct_side(ar.digit_in32(dig(e), P), ar.digit_in32(dig(E + e), P), w1, sgn);
+      }
+    }
+    double tw[7];
+    // Pass 1 (layout A), uniform twiddles.
+    tw[0] = ctw[2 + h];
+    tw[1] = ctw[4 + 2 * h];
+    tw[2] = ctw[5 + 2 * h];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) tw[3 + c] = ctw[8 + 4 * h + c];
+    ks14_pass<A, 1>(x, tw, ar);
+    // A -> B inside the wave.
+    ks14_tw<4>(twr, tb2, tw);
+#pragma unroll
+    for (int e = 0; e < E; ++e) rg[x1w + 72 * e] = x[e];
+    wave_sync_lds();
+#pragma unroll
+    for (int e = 0; e < E; ++e) x[e] = rg[x1r + 8 * e];
+    pin_words(x);
+    wave_sync_lds();
+    ks14_pass<A, 4>(x, tw, ar);
+    // B -> C inside the wave.
+    ks14_tw<7>(twr, tb3, tw);
+#pragma unroll
+    for (int e = 0; e < E; ++e) rg[x2w + 9 * e] = x[e];
+    wave_sync_lds();
+#pragma unroll
+    for (int e = 0; e < E; ++e) x[e] = rg[x2r + e];
+    pin_words(x);
+    wave_sync_lds();
+    ks14_pass<A, 7>(x, tw, ar);
+    // C -> D across the workgroup.
+    ks14_tw<10>(twr, tb4, tw);
+    if (j & 1) {
+#pragma unroll
+      for (int e = 0; e < E; ++e) x3[x3w1 + e] = x[e];
+      __syncthreads();
+#pragma unroll
+      for (int e = 0; e < E; ++e) x[e] = x3[x3r1 + 1092 * e];
+    } else {
+#pragma unroll
+      for (int e = 0; e < E; ++e) x3[x3w0 + e] = x[e];
+      __syncthreads();
+#pragma unroll
+      for (int e = 0; e < E; ++e) x[e] = x3[x3r0 + 68 * e];
+    }
+    pin_words(x);
+    ks14_pass<A, 10>(x, tw, ar);
+    // D -> E inside the wave, then stage 13 on register bit 0.
+    double t5[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) t5[k] = __builtin_bit_cast(double, w32::ld64(twr, (tb5 + 64 * k) * 8, 0));
+#pragma unroll
+    for (int e = 0; e < E; ++e) rg[x4w + 2 * e] = x[e];
+    wave_sync_lds();
+#pragma unroll
+    for (int e = 0; e < E; ++e) x[e] = rg[x4r + (e & 1) + 136 * (e >> 1)];
+    pin_words(x);
+    wave_sync_lds();
+#pragma unroll
+    for (int k = 0; k < 4; ++k) ar.ct(x[2 * k], x[2 * k + 1], ar.tw(t5[k]));
+    if (A::fwd_reduce_at(13) && !A::kSkipLastReduce) {
+#pragma unroll
+      for (int e = 0; e < E; ++e) ar.fwd_reduce(x[e]);
+    }
+    // Products with key row j: both key polynomials' pairs requested at
+    // once, then the next digit's plane words behind them.
+    const uint64_t* kbp = ka.key_b + (size_t)(ka.key_base + j) * poly_words + limb_off;
+    const uint64_t* kap = ka.key_a + (size_t)(ka.key_base + j) * poly_words + limb_off;
+    const __amdgpu_buffer_rsrc_t rb =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<uint64_t*>(kbp), (short)0, 0x7fffffff, 0x00020000);
+    const __amdgpu_buffer_rsrc_t ra =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<uint64_t*>(kap), (short)0, 0x7fffffff, 0x00020000);
+    const size_t jn = j + 1 < gd.k ? (size_t)(j + 1) : (size_t)j;
+    uint4 nxt[2];
+#pragma unroll
+    for (int k0 = 0; k0 < 4; k0 += 2) {
+      U64Pair kbr[2], kar[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        kbr[i] = buffer_load_u64x2(rb, (pe + 128 * (k0 + i)) * 8, 0);
+        kar[i] = buffer_load_u64x2(ra, (pe + 128 * (k0 + i)) * 8, 0);
+      }
+      if (k0 == 0) {
+        nxt[0] = pl[jn * kPlane];
+        nxt[1] = pl[jn * kPlane + 1];
+      }
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        const int r = 2 * (k0 + i);
+        typename A::Prep d0, d1;
+        if constexpr (kPrep) {
+          d0 = ar.prep(x[r]);
+          d1 = ar.prep(x[r + 1]);
+        } else {
+          d0 = x[r];
+          d1 = x[r + 1];
+        }
+        ar.mac(acc0[r], d0, kbr[i].x, P);
+        ar.mac(acc0[r + 1], d1, kbr[i].y, P);
+        ar.mac(acc1[r], d0, kar[i].x, P);
+        ar.mac(acc1[r + 1], d1, kar[i].y, P);
+      }
+    }
+    if (((j + 1) % kAccEvery) == 0) {
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        ar.acc_reduce(acc0[e]);
+        ar.acc_reduce(acc1[e]);
+      }
+    }
+    cur[0] = nxt[0];
+    cur[1] = nxt[1];
+  }
+  // Output offsets formed again from an opaque copy of the thread index.
+  int t_out = threadIdx.x;
+  asm volatile("" : "+v"(t_out));
+  const int po = 2 * (t_out & 63) + 512 * (t_out >> 6);
+  uint64_t* o0 = ka.out0 + ct * ka.out_stride + limb_off;
+  uint64_t* o1 = ka.out1 + ct * ka.out_stride + limb_off;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    u64x2 a, c;
+    a.x = ar.acc_out(acc0[2 * k]);
+    a.y = ar.acc_out(acc0[2 * k + 1]);
+    c.x = ar.acc_out(acc1[2 * k]);
+    c.y = ar.acc_out(acc1[2 * k + 1]);
+    *reinterpret_cast<u64x2*>(o0 + po + 128 * k) = a;
+    *reinterpret_cast<u64x2*>(o1 + po + 128 * k) = c;
+  }
+}
+
+template <class A>
+static hipError_t launch_ks_n14(const KeySwitchArgs& ka, const GadgetDesc& gd,
+                                const PrimeDev* primes, const LimbPrimes& lp, int count,
+                                int num_limbs, hipStream_t s) {
+  const bool tab = gd.max_log_b <= kKsTabLogB;
+  const size_t shmem = (size_t)ks14::kLdsWords * 8 + (tab ? ((size_t)8 << gd.max_log_b) : 0);
+  auto k = keyswitch_n14_kernel<A>;
+  set_max_dynamic_shmem((const void*)k, shmem);
+  hipLaunchKernelGGL(k, dim3((unsigned)(count * num_limbs) << 1), dim3(1024), shmem, s, ka, gd,
+                     primes, lp, count, num_limbs);
+  return hipGetLastError();
+}
+
 // The digit-plane key switch on the wave-exchange layout (shg_ntt_wave.h):
 // slices of 2^(10+LW) words (n = 2^12: LW = 2; 2^13: LW = 3; 2^14: LW = 3
 // with HB = 1), FP64.  Per digit: one workgroup-wide LDS exchange between
@@ -675,6 +980,9 @@ static hipError_t dispatch_ks(int logn, const KeySwitchArgs& ka, const GadgetDes
   // at L = 4.  At 2^13 and 2^14 one 512-thread workgroup per CU is resident
   // either way and keyswitch_kernel measured 5% and 6% faster (2^14, L = 8:
   // 10.55 vs 11.2 ms per 512 ciphertexts).
+  if constexpr (PL && SHG_KS_N14 && std::is_same<typename A::V, double>::value) {
+    if (logn == 14) return launch_ks_n14<A>(ka, gd, primes, lp, count, num_limbs, s);
+  }
   if constexpr (std::is_same<A, F64KsKeyArith>::value) {
     // FP64-form keys at 2^12 (the wave-exchange kernel), 2^13 and 2^14.
     if (logn == 12) return launch_ks_wave<2, 0, A>(ka, gd, primes, lp, count, num_limbs, s);

@@ -617,7 +617,11 @@ __global__ void __launch_bounds__((1 << LOGM) >> LOGE)
 #pragma unroll
   for (int i = 0; i < E; ++i) {
     const int idx = t * E + i;
-    v[i] = lds_u[lpad((idx % VC) * TC + idx / VC)];
+    // Row t of 16 entries: positions (t >> 6) + 16 (t & 63) + 1024 i at
+    // n = 2^14 (plane_row14), t + TC i otherwise.
+    const int pos = LOGM == 14 && SHG_KS_N14 ? (t >> 6) + 16 * (t & 63) + 1024 * i
+                                              : (idx % VC) * TC + idx / VC;
+    v[i] = lds_u[lpad(pos)];
   }
   const int lb = gd.limb_log_b[l];
   const int nd = gd.limb_digits[l];
@@ -639,9 +643,10 @@ __global__ void __launch_bounds__((1 << LOGM) >> LOGE)
 // inverse: its output layout gives thread t the positions t + 512 e
 // (e = 0..31).  With 32 entries per key-switch thread (VC = 32) those are
 // exactly the plane entries 32 t .. 32 t + 31 ((p % 512) * 32 + p / 512);
-// with 16 (VC = 16) the even e give entries 16 t + e / 2 and the odd e
-// entries 16 (t + 512) + e / 2.  Either way each plane takes four 16-byte
-// stores per thread and no further exchange.
+// with 16 (VC = 16), row r of 16 entries holds positions (r >> 6) +
+// 16 (r & 63) + 1024 k (plane_row14), so the even e give row
+// 64 (t & 15) + (t >> 4) and the odd e the row 32 above it.  Either way each
+// plane takes four 16-byte stores per thread and no further exchange.
 struct W32Digits {
   __amdgpu_buffer_rsrc_t g;
   const F64Arith* ar;
@@ -675,14 +680,15 @@ struct W32Digits {
 #pragma unroll
         for (int j = 0; j < 4; ++j) dst[j] = make_uint4(w[4 * j], w[4 * j + 1], w[4 * j + 2], w[4 * j + 3]);
       } else {
-        // Even e: entries 16 t + e / 2; odd e: 16 (t + 512) + e / 2.
+        // Even e: row plane_row14(t), odd e: row plane_row14(t + 512), entry
+        // e / 2 of each.
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
           w[i] = digit(4 * i) | (digit(4 * i + 2) << 16);
           w[8 + i] = digit(4 * i + 1) | (digit(4 * i + 3) << 16);
         }
-        uint4* d0 = reinterpret_cast<uint4*>(plane + (size_t)t * 16);
-        uint4* d1 = reinterpret_cast<uint4*>(plane + (size_t)(t + 512) * 16);
+        uint4* d0 = reinterpret_cast<uint4*>(plane + (size_t)plane_row14(t) * 16);
+        uint4* d1 = reinterpret_cast<uint4*>(plane + (size_t)plane_row14(t + 512) * 16);
         d0[0] = make_uint4(w[0], w[1], w[2], w[3]);
         d0[1] = make_uint4(w[4], w[5], w[6], w[7]);
         d1[0] = make_uint4(w[8], w[9], w[10], w[11]);
