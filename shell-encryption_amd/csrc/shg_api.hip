@@ -391,6 +391,9 @@ int build_gadget(shg_context* ctx, const int* primes, int num_limbs, const int* 
   }
   gd->k = k;
   gd->max_log_b = max_b;
+  bool uniform = num_limbs > 0;
+  for (int i = 1; i < num_limbs; ++i) uniform &= log_bs[i] == log_bs[0];
+  gd->plane_bits = uniform && (max_b == 8 || max_b == 10 || max_b == 12) ? max_b : 0;
   gd->small_digits = max_b + 6 < 64 && (1ull << (max_b + 6)) <= min_q;
   return SHG_OK;
 }

@@ -106,6 +106,9 @@ struct GadgetDesc {
   // d > q_i / 2 stands for d - q_i.
   unsigned char limb_balance[SHG_MAX_LIMBS];
   int any_balance;
+  // Digit bits of packed planes at n = 2^14 (plane_pack_bits): the common
+  // base log when every limb has the same one and it is 8, 10 or 12; else 0.
+  int plane_bits;
 };
 hipError_t launch_gadget_decompose(const uint64_t* a, uint64_t* digits, const GadgetDesc& gd,
                                    const PrimeDev* primes, const LimbPrimes& lp, int count,
@@ -181,6 +184,16 @@ constexpr int ks_planes_vc(int logn) { return logn >= 14 ? (2 << SHG_KS_LOGE) : 
 #endif
 __host__ __device__ constexpr int plane_row14(int p) {
   return SHG_KS_N14 ? 64 * (p & 15) + ((p >> 4) & 63) : p;
+}
+// Packed planes at n = 2^14: with every digit of b = gd.plane_bits bits a
+// row's 16 digits take W = ceil(16 b / 32) dwords (digit i at bits
+// [b i, b i + b)), and dword k of row r sits at dword k * 1024 + r of the
+// plane's slot (the slot keeps its 2^14 16-bit entries; the rest is unused).
+#ifndef SHG_KS_PACK
+#define SHG_KS_PACK 1
+#endif
+__host__ __device__ constexpr int plane_pack_bits(int plane_bits) {
+  return SHG_KS_PACK && SHG_KS_N14 ? plane_bits : 0;
 }
 
 // Sets hipFuncAttributeMaxDynamicSharedMemorySize once per kernel and device.

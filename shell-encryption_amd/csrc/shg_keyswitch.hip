@@ -448,7 +448,8 @@ __global__ void __launch_bounds__((1 << LOGM) >> LOGE)
 // Every slot map keeps the register index out of the lane part (immediate
 // LDS offsets) and no two lanes of a 16-lane group share a bank pair.  The
 // digit planes hold, in row t = 64 w + l, the 16 digits of limb positions
-// w + 16 l + 2^10 k (k < 8: half 0), so thread t reads row t.
+// w + 16 l + 2^10 k (k < 8: half 0), so thread t reads row t (16-bit entries,
+// or packed at the common base log: plane_pack_bits).
 namespace ks14 {
 constexpr int kRegion = 575;                  // in-wave exchange words per wave
 constexpr int kX3Words = 8733;                // workgroup exchange words
@@ -481,7 +482,7 @@ __device__ __forceinline__ void ks14_tw(__amdgpu_buffer_rsrc_t r, int b0, double
   w32::tw_run<4>(r, (4 << S0) + 4 * b0, tw + 3);
 }
 
-template <class A>
+template <class A, int PB>
 __global__ void __launch_bounds__(1024)
     keyswitch_n14_kernel(KeySwitchArgs ka, GadgetDesc gd, const PrimeDev* __restrict__ primes,
                          LimbPrimes lp, int count, int num_limbs) {
@@ -561,18 +562,36 @@ __global__ void __launch_bounds__(1024)
   const int tb4 = 512 * h + 32 * wave + 16 * b8 + (lane & 15);
   const int tb5 = 8192 + 4096 * h + 256 * wave + lane;
 
-  // Digit planes: row t holds this thread's 16 digits of the plane.
-  const uint4* pl = reinterpret_cast<const uint4*>(ka.planes + ((size_t)ct * gd.k << LOGN)) +
-                    (size_t)t * 2;
-  constexpr size_t kPlane = ((size_t)1 << LOGN) / 8;  // uint4 per plane
-  uint4 cur[2];
-  cur[0] = pl[0];
-  cur[1] = pl[1];
+  // Digit planes: row t holds this thread's 16 digits of the plane, as 16-bit
+  // entries (PB = 0: two 16-byte loads) or packed at PB bits (PW dwords, the
+  // k-th at dword k * 1024 + t of the plane).
+  constexpr int PW = PB ? (16 * PB + 31) / 32 : 8;
+  const uint32_t* pl = reinterpret_cast<const uint32_t*>(ka.planes + ((size_t)ct * gd.k << LOGN));
+  constexpr size_t kPlane = ((size_t)1 << LOGN) / 2;  // dwords per plane
+  auto load_row = [&](size_t j, uint32_t (&w)[8]) {
+    if constexpr (PB == 0) {
+      const uint4* r = reinterpret_cast<const uint4*>(pl + j * kPlane) + (size_t)t * 2;
+      const uint4 a = r[0], c = r[1];
+      w[0] = a.x; w[1] = a.y; w[2] = a.z; w[3] = a.w;
+      w[4] = c.x; w[5] = c.y; w[6] = c.z; w[7] = c.w;
+    } else {
+#pragma unroll
+      for (int q = 0; q < 8; ++q) w[q] = q < PW ? pl[j * kPlane + q * 1024 + t] : 0u;
+    }
+  };
+  uint32_t cur[8];
+  load_row(0, cur);
   for (int j = 0; j < gd.k; ++j) {
     double x[E];
     {
-      uint32_t wd[8] = {cur[0].x, cur[0].y, cur[0].z, cur[0].w, cur[1].x, cur[1].y, cur[1].z, cur[1].w};
-      auto dig = [&](int i) { return (i & 1) ? (wd[i / 2] >> 16) : (wd[i / 2] & 0xffffu); };
+      uint32_t dv[16];
+      if constexpr (PB == 0) {
+#pragma unroll
+        for (int i = 0; i < 16; ++i) dv[i] = (i & 1) ? (cur[i / 2] >> 16) : (cur[i / 2] & 0xffffu);
+      } else {
+        unpack_row16<PB>(cur, dv);
+      }
+      auto dig = [&](int i) { return dv[i]; };
       if (use_tab) {
 #pragma unroll
         for (int e = 0; e < E; ++e) x[e] = fma(sgn, tab[dig(E + e)], ar.digit_in32(dig(e), P));
@@ -653,7 +672,7 @@ __global__ void __launch_bounds__(1024)
     const __amdgpu_buffer_rsrc_t ra =
         __builtin_amdgcn_make_buffer_rsrc(const_cast<uint64_t*>(kap), (short)0, 0x7fffffff, 0x00020000);
     const size_t jn = j + 1 < gd.k ? (size_t)(j + 1) : (size_t)j;
-    uint4 nxt[2];
+    uint32_t nxt[8];
 #pragma unroll
     for (int k0 = 0; k0 < 4; k0 += 2) {
       U64Pair kbr[2], kar[2];
@@ -662,10 +681,7 @@ __global__ void __launch_bounds__(1024)
         kbr[i] = buffer_load_u64x2(rb, (pe + 128 * (k0 + i)) * 8, 0);
         kar[i] = buffer_load_u64x2(ra, (pe + 128 * (k0 + i)) * 8, 0);
       }
-      if (k0 == 0) {
-        nxt[0] = pl[jn * kPlane];
-        nxt[1] = pl[jn * kPlane + 1];
-      }
+      if (k0 == 0) load_row(jn, nxt);
       __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
       for (int i = 0; i < 2; ++i) {
@@ -691,8 +707,8 @@ __global__ void __launch_bounds__(1024)
         ar.acc_reduce(acc1[e]);
       }
     }
-    cur[0] = nxt[0];
-    cur[1] = nxt[1];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) cur[q] = nxt[q];
   }
   // Output offsets formed again from an opaque copy of the thread index.
   int t_out = threadIdx.x;
@@ -718,7 +734,10 @@ static hipError_t launch_ks_n14(const KeySwitchArgs& ka, const GadgetDesc& gd,
                                 int num_limbs, hipStream_t s) {
   const bool tab = gd.max_log_b <= kKsTabLogB;
   const size_t shmem = (size_t)ks14::kLdsWords * 8 + (tab ? ((size_t)8 << gd.max_log_b) : 0);
-  auto k = keyswitch_n14_kernel<A>;
+  const int pb = plane_pack_bits(gd.plane_bits);
+  auto k = pb == 8 ? keyswitch_n14_kernel<A, 8>
+           : pb == 10 ? keyswitch_n14_kernel<A, 10>
+           : pb == 12 ? keyswitch_n14_kernel<A, 12> : keyswitch_n14_kernel<A, 0>;
   set_max_dynamic_shmem((const void*)k, shmem);
   hipLaunchKernelGGL(k, dim3((unsigned)(count * num_limbs) << 1), dim3(1024), shmem, s, ka, gd,
                      primes, lp, count, num_limbs);

@@ -627,8 +627,27 @@ __global__ void __launch_bounds__((1 << LOGM) >> LOGE)
   const int nd = gd.limb_digits[l];
   const uint32_t mask = (1u << lb) - 1;
   uint16_t* base = planes + ((size_t)ct * gd.k + gd.limb_first[l]) * n + (size_t)t * E;
+  const int pb = LOGM == 14 ? plane_pack_bits(gd.plane_bits) : 0;
   for (int dd = 0; dd < nd; ++dd) {
     const int sh = dd * lb;
+    if (pb) {
+      // Packed rows (plane_pack_bits): this thread's row t.
+      uint32_t dv[16], w[8];
+#pragma unroll
+      for (int i = 0; i < 16; ++i) dv[i] = (uint32_t)(v[i] >> sh) & mask;
+      const int wn = pb == 8 ? 4 : (pb == 10 ? 5 : 6);
+      if (pb == 8)
+        pack_row16<8>(dv, w);
+      else if (pb == 10)
+        pack_row16<10>(dv, w);
+      else
+        pack_row16<12>(dv, w);
+      uint32_t* p32 = reinterpret_cast<uint32_t*>(base - (size_t)t * E + (size_t)dd * n);
+#pragma unroll
+      for (int k = 0; k < 6; ++k)
+        if (k < wn) p32[k * 1024 + t] = w[k];
+      continue;
+    }
     uint32_t w[E / 2];
 #pragma unroll
     for (int i = 0; i < E; i += 2)
@@ -651,7 +670,8 @@ struct W32Digits {
   __amdgpu_buffer_rsrc_t g;
   const F64Arith* ar;
   uint16_t* base;  // digit 0's plane at this workgroup's first entry
-  int nd, lb;
+  int nd, lb, plane_bits;
+  double* lds;     // the transform's LDS, free once it has run
   static constexpr bool kAll = true;
   __device__ __forceinline__ void load(int vs, int k, double& lo, double& hi) const {
     const U64Pair v = w32::ld128(g, vs, k * 1024);
@@ -659,6 +679,51 @@ struct W32Digits {
     hi = ar->load(v.y);
   }
   __device__ __forceinline__ void store(int, int, double) const {}
+  // Packed planes (plane_pack_bits, B bits per digit): each thread packs its
+  // rows plane_row14(t) (even e) and plane_row14(t + 512) (odd e) into W
+  // dwords; batches of planes go through LDS (dword k of row r at
+  // (plane, k) * 1088 + r + 4 (r >> 6): the rows a wave writes fall on
+  // distinct banks) and leave as 16-byte stores of consecutive dwords, since
+  // rows 64 apart sit in one wave.
+  template <int B>
+  __device__ __forceinline__ void store_packed(const uint64_t (&v)[32], int t, uint32_t mask) const {
+    constexpr int W = (16 * B + 31) / 32;
+    constexpr int kPer = 1088 * W;                   // LDS dwords per plane
+    constexpr int kBatch = (2 * w32::kLdsWords) / kPer;  // planes per batch
+    uint32_t* l32 = reinterpret_cast<uint32_t*>(lds);
+    const int r0 = plane_row14(t), r1 = plane_row14(t + 512);
+    const int s0 = r0 + 4 * (r0 >> 6), s1 = r1 + 4 * (r1 >> 6);
+    for (int d0 = 0; d0 < nd; d0 += kBatch) {
+      const int nb = nd - d0 < kBatch ? nd - d0 : kBatch;
+      __syncthreads();  // the previous batch's (or the transform's) LDS reads are done
+      for (int i = 0; i < nb; ++i) {
+        const int sh = (d0 + i) * lb;
+        uint32_t dv0[16], dv1[16], p0[8], p1[8];
+#pragma unroll
+        for (int c = 0; c < 16; ++c) {
+          dv0[c] = (uint32_t)(v[2 * c] >> sh) & mask;
+          dv1[c] = (uint32_t)(v[2 * c + 1] >> sh) & mask;
+        }
+        pack_row16<B>(dv0, p0);
+        pack_row16<B>(dv1, p1);
+#pragma unroll
+        for (int k = 0; k < W; ++k) {
+          l32[(i * W + k) * 1088 + s0] = p0[k];
+          l32[(i * W + k) * 1088 + s1] = p1[k];
+        }
+      }
+      __syncthreads();
+      // Chunk c (4 dwords) of the batch: plane d0 + c / (256 W), dword
+      // 4 (c % (256 W)) of that plane, i.e. sub-array k, rows 4 c' .. 4 c' + 3.
+      for (int c = t; c < nb * W * 256; c += 512) {
+        const int i = c / (256 * W), cw = c - i * 256 * W;
+        const int k = cw >> 8, r = (cw & 255) * 4;
+        const uint4 val = *reinterpret_cast<const uint4*>(l32 + (i * W + k) * 1088 + r + 4 * (r >> 6));
+        uint32_t* dst = reinterpret_cast<uint32_t*>(base + ((size_t)(d0 + i) << w32::kLogN));
+        *reinterpret_cast<uint4*>(dst + k * 1024 + r) = val;
+      }
+    }
+  }
   __device__ __forceinline__ void store_all(int v1, const double (&x)[32]) const {
     uint64_t v[32];
 #pragma unroll
@@ -667,6 +732,14 @@ struct W32Digits {
     static_assert(VC == 32 || VC == 16, "plane layouts of 32 or 16 entries per thread");
     const uint32_t mask = (1u << lb) - 1;
     const int t = v1 / 8;
+    if constexpr (VC == 16) {
+      switch (plane_pack_bits(plane_bits)) {
+        case 8: return store_packed<8>(v, t, mask);
+        case 10: return store_packed<10>(v, t, mask);
+        case 12: return store_packed<12>(v, t, mask);
+        default: break;
+      }
+    }
     for (int dd = 0; dd < nd; ++dd) {
       const int sh = dd * lb;
       uint16_t* plane = base + ((size_t)dd << w32::kLogN);
@@ -710,7 +783,8 @@ __global__ void __launch_bounds__(w32::kThreads, 4)
   const F64Arith ar(P);
   const W32Digits io{w32::rsrc(in + (size_t)ct * in_stride + ((size_t)l << w32::kLogN)), &ar,
                      planes + (((size_t)ct * gd.k + gd.limb_first[l]) << w32::kLogN),
-                     gd.limb_digits[l], gd.limb_log_b[l]};
+                     gd.limb_digits[l], gd.limb_log_b[l], gd.plane_bits,
+                     reinterpret_cast<double*>(smem)};
   w32::inv_limb(ar, P, reinterpret_cast<double*>(smem), ar.ninvx, io);
 }
 

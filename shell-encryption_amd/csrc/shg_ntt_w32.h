@@ -502,6 +502,33 @@ __device__ __forceinline__ void inv_limb(const A& ar, const PrimeDev& P, double*
 }
 
 }  // namespace w32
+
+// A packed plane row: 16 digits of B bits in W = ceil(16 B / 32) dwords,
+// digit i at bits [B i, B i + B) (plane_pack_bits).
+template <int B>
+__device__ __forceinline__ void pack_row16(const uint32_t (&dv)[16], uint32_t (&w)[8]) {
+#pragma unroll
+  for (int k = 0; k < 8; ++k) w[k] = 0;
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    const int o = B * i;
+    w[o >> 5] |= dv[i] << (o & 31);
+    if ((o & 31) + B > 32) w[(o >> 5) + 1] |= dv[i] >> (32 - (o & 31));
+  }
+}
+template <int B>
+__device__ __forceinline__ void unpack_row16(const uint32_t (&w)[8], uint32_t (&dv)[16]) {
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    const int o = B * i;
+    uint32_t v;
+    if ((o & 31) + B > 32)
+      v = __builtin_amdgcn_alignbit(w[(o >> 5) + 1], w[o >> 5], o & 31);
+    else
+      v = w[o >> 5] >> (o & 31);
+    dv[i] = v & ((1u << B) - 1);
+  }
+}
 }  // namespace shg
 
 #endif  // SHG_NTT_W32_H_

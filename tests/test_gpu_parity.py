@@ -306,11 +306,13 @@ def test_gadget_decompose(ctx):
 # Bases above 2^16 take the path without digit planes: an inverse transform
 # with the ExportInt scaling fused (n^-1 2^-64), then the key switch.
 # At 2^14 (split limbs) bases up to 2^11 take stage 0 from a table of
-# products (7, 10, 11 bits), 12 and 16 bits the products themselves.
+# products (7, 8, 10, 11 bits), 12 and 16 bits the products themselves; a
+# common base of 8, 10 or 12 bits packs the digit planes at that width.
 @pytest.mark.parametrize("log_n,L,log_b,count", [(6, 2, 10, 3), (12, 3, 10, 2), (12, 2, 10, 64),
                                                  (13, 4, 10, 2), (14, 8, 10, 2), (15, 2, 25, 1),
                                                  (13, 2, 20, 2), (14, 3, 20, 1), (14, 3, 7, 2),
-                                                 (14, 2, 11, 2), (14, 2, 12, 1), (14, 2, 16, 1)])
+                                                 (14, 2, 11, 2), (14, 2, 12, 1), (14, 2, 16, 1),
+                                                 (14, 3, 8, 2)])
 def test_relin_apply(ctx, log_n, L, log_b, count):
     qs, rns = _rns(log_n, L)
     pids = ctx.primes(qs, log_n)
@@ -555,7 +557,8 @@ def test_relin_apply_mixed_bases_n16384(ctx):
 
 
 @pytest.mark.parametrize("log_n,L,log_b,count", [(14, 8, 10, 3), (13, 3, 10, 2), (12, 2, 10, 4),
-                                                 (14, 2, 20, 1), (14, 2, 16, 2)])
+                                                 (14, 2, 20, 1), (14, 2, 16, 2), (14, 2, 8, 1),
+                                                 (14, 3, 12, 2)])
 def test_prepared_key_relin_and_galois(ctx, log_n, L, log_b, count):
     """Key switching with a prepared key (shg_ks_key: FP64-form copies read
     by the digit-plane kernel at 2^13 and 2^14; the Montgomery words on every
