@@ -57,6 +57,11 @@
 #endif
 constexpr int kKsTabLogB = 11;
 
+// keyswitch_n14_kernel: its first in-wave exchange through cross-lane
+// operations (v_permlane32_swap, v_permlane16_swap, DPP) instead of LDS.
+#ifndef SHG_KS14_X1_LANES
+#define SHG_KS14_X1_LANES 0
+#endif
 // Ring degrees (bit log n) that take the wave-exchange kernel.
 #ifndef SHG_KS_WAVE_LOGN
 #define SHG_KS_WAVE_LOGN (1 << 12)
@@ -456,6 +461,58 @@ constexpr int kX3Words = 8733;                // workgroup exchange words
 constexpr int kLdsWords = kX3Words + 16 * kRegion;  // then the stage-0 table
 }  // namespace ks14
 
+// Layout A -> B without LDS: B swaps register bit k with lane bit 3 + k
+// (k = 0, 1, 2).  Each is a transposition of a register pair (x[e], x[e |
+// 2^k], bit k of e clear) with a lane bit m: lanes with bit m clear take
+// the partner lane's x[e] into x[e | 2^k], lanes with it set the partner's
+// x[e | 2^k] into x[e].  Lane bit 5: v_permlane32_swap (lanes 32-63 of the
+// first operand with lanes 0-31 of the second); bit 4: v_permlane16_swap
+// (odd rows of 16 with even rows); bit 3: DPP row_ror:8 (lane l ^ 8 of the
+// row) written to banks 2-3 or 0-1 of each row.
+__device__ __forceinline__ void ks14_lane_transpose(double (&x)[8]) {
+  auto split = [](double v, uint32_t& lo, uint32_t& hi) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    lo = (uint32_t)u;
+    hi = (uint32_t)(u >> 32);
+  };
+  auto join = [](uint32_t lo, uint32_t hi) {
+    return __builtin_bit_cast(double, (uint64_t)lo | ((uint64_t)hi << 32));
+  };
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {  // register bit 2 <-> lane bit 5
+    uint32_t al, ah, bl, bh;
+    split(x[e], al, ah);
+    split(x[e | 4], bl, bh);
+    const auto lo = __builtin_amdgcn_permlane32_swap(al, bl, false, false);
+    const auto hi = __builtin_amdgcn_permlane32_swap(ah, bh, false, false);
+    x[e] = join(lo[0], hi[0]);
+    x[e | 4] = join(lo[1], hi[1]);
+  }
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {  // register bit 1 <-> lane bit 4
+    if (e & 2) continue;
+    uint32_t al, ah, bl, bh;
+    split(x[e], al, ah);
+    split(x[e | 2], bl, bh);
+    const auto lo = __builtin_amdgcn_permlane16_swap(al, bl, false, false);
+    const auto hi = __builtin_amdgcn_permlane16_swap(ah, bh, false, false);
+    x[e] = join(lo[0], hi[0]);
+    x[e | 2] = join(lo[1], hi[1]);
+  }
+#pragma unroll
+  for (int e = 0; e < 8; e += 2) {  // register bit 0 <-> lane bit 3
+    uint32_t al, ah, bl, bh;
+    split(x[e], al, ah);
+    split(x[e | 1], bl, bh);
+    const uint32_t nal = __builtin_amdgcn_update_dpp(al, bl, 0x128, 0xf, 0xc, false);
+    const uint32_t nah = __builtin_amdgcn_update_dpp(ah, bh, 0x128, 0xf, 0xc, false);
+    const uint32_t nbl = __builtin_amdgcn_update_dpp(bl, al, 0x128, 0xf, 0x3, false);
+    const uint32_t nbh = __builtin_amdgcn_update_dpp(bh, ah, 0x128, 0xf, 0x3, false);
+    x[e] = join(nal, nah);
+    x[e | 1] = join(nbl, nbh);
+  }
+}
+
 template <class A, int S0>
 __device__ __forceinline__ void ks14_pass(double (&x)[8], const double (&tw)[7], const A& ar) {
 #pragma unroll
@@ -611,13 +668,17 @@ __global__ void __launch_bounds__(1024)
     ks14_pass<A, 1>(x, tw, ar);
     // A -> B inside the wave.
     ks14_tw<4>(twr, tb2, tw);
+    if constexpr (SHG_KS14_X1_LANES) {
+      ks14_lane_transpose(x);
+    } else {
 #pragma unroll
-    for (int e = 0; e < E; ++e) rg[x1w + 72 * e] = x[e];
-    wave_sync_lds();
+      for (int e = 0; e < E; ++e) rg[x1w + 72 * e] = x[e];
+      wave_sync_lds();
 #pragma unroll
-    for (int e = 0; e < E; ++e) x[e] = rg[x1r + 8 * e];
-    pin_words(x);
-    wave_sync_lds();
+      for (int e = 0; e < E; ++e) x[e] = rg[x1r + 8 * e];
+      pin_words(x);
+      wave_sync_lds();
+    }
     ks14_pass<A, 4>(x, tw, ar);
     // B -> C inside the wave.
     ks14_tw<7>(twr, tb3, tw);
