@@ -698,14 +698,9 @@ struct W32Digits {
       __syncthreads();  // the previous batch's (or the transform's) LDS reads are done
       for (int i = 0; i < nb; ++i) {
         const int sh = (d0 + i) * lb;
-        uint32_t dv0[16], dv1[16], p0[8], p1[8];
-#pragma unroll
-        for (int c = 0; c < 16; ++c) {
-          dv0[c] = (uint32_t)(v[2 * c] >> sh) & mask;
-          dv1[c] = (uint32_t)(v[2 * c + 1] >> sh) & mask;
-        }
-        pack_row16<B>(dv0, p0);
-        pack_row16<B>(dv1, p1);
+        uint32_t p0[8], p1[8];
+        pack_row16_fn<B>([&](int c) { return (uint32_t)(v[2 * c] >> sh) & mask; }, p0);
+        pack_row16_fn<B>([&](int c) { return (uint32_t)(v[2 * c + 1] >> sh) & mask; }, p1);
 #pragma unroll
         for (int k = 0; k < W; ++k) {
           l32[(i * W + k) * 1088 + s0] = p0[k];

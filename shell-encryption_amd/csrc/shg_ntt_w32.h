@@ -505,16 +505,21 @@ __device__ __forceinline__ void inv_limb(const A& ar, const PrimeDev& P, double*
 
 // A packed plane row: 16 digits of B bits in W = ceil(16 B / 32) dwords,
 // digit i at bits [B i, B i + B) (plane_pack_bits).
-template <int B>
-__device__ __forceinline__ void pack_row16(const uint32_t (&dv)[16], uint32_t (&w)[8]) {
+template <int B, class F>
+__device__ __forceinline__ void pack_row16_fn(const F& digit, uint32_t (&w)[8]) {
 #pragma unroll
   for (int k = 0; k < 8; ++k) w[k] = 0;
 #pragma unroll
   for (int i = 0; i < 16; ++i) {
     const int o = B * i;
-    w[o >> 5] |= dv[i] << (o & 31);
-    if ((o & 31) + B > 32) w[(o >> 5) + 1] |= dv[i] >> (32 - (o & 31));
+    const uint32_t d = digit(i);
+    w[o >> 5] |= d << (o & 31);
+    if ((o & 31) + B > 32) w[(o >> 5) + 1] |= d >> (32 - (o & 31));
   }
+}
+template <int B>
+__device__ __forceinline__ void pack_row16(const uint32_t (&dv)[16], uint32_t (&w)[8]) {
+  pack_row16_fn<B>([&](int i) { return dv[i]; }, w);
 }
 template <int B>
 __device__ __forceinline__ void unpack_row16(const uint32_t (&w)[8], uint32_t (&dv)[16]) {

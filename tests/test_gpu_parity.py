@@ -345,6 +345,26 @@ def test_relin_degree3_and_coeff_form_component(ctx):
     assert np.array_equal(o0.download()[0], e0) and np.array_equal(o1.download()[0], e1)
 
 
+@pytest.mark.parametrize("log_b", [10, 16])
+def test_relin_degree3_n16384_accumulates(ctx, log_b):
+    """Degree 3 at 2^14: the second source component's key switch adds onto
+    the first one's output (digit planes packed at 10 bits, or 16-bit rows),
+    for two ciphertexts."""
+    qs, rns = _rns(14, 2)
+    pids = ctx.primes(qs, 14)
+    log_bs = [log_b, log_b]
+    k, _ = rns.gadget_dims(log_bs)
+    comps = np.stack([rns.uniform(40 + c, 4) for c in range(2)])
+    kas, kbs = rns.uniform(41, 2 * k), rns.uniform(42, 2 * k)
+    dc, dka, dkb = ctx.array(comps), ctx.array(kas), ctx.array(kbs)
+    o0, o1 = ctx.empty((2, 2, rns.n)), ctx.empty((2, 2, rns.n))
+    ctx.relin_apply(dc, 3, 0b1111, dka, dkb, 2 * k, log_bs, o0, o1, 2, pids)
+    g0, g1 = o0.download(), o1.download()
+    for c in range(2):
+        e0, e1 = rns.relin_apply(comps[c], kas, kbs, log_bs)
+        assert np.array_equal(g0[c], e0) and np.array_equal(g1[c], e1), c
+
+
 @pytest.mark.parametrize("log_n,L,count", [(7, 2, 2), (12, 2, 16), (14, 8, 2)])
 def test_galois_apply(ctx, log_n, L, count):
     qs, rns = _rns(log_n, L)
