@@ -592,7 +592,7 @@ def main():
     roofline = {"bound": "valu", "achieved": round(valu_tops, 2), "peak": FP64_OPS_PEAK_T,
                 "unit": "T FP64 ops/s", "frac": round(valu_tops / FP64_OPS_PEAK_T, 4),
                 "traffic": measured_traffic(),
-                "kernel": "keyswitch_n14_kernel<F64KsKeyArith>",
+                "kernel": "keyswitch_n14_kernel<F64KsKeyArith, 10>",
                 "avg_launch_ms": round(ks_avg_ms, 4),
                 "ops_per_launch": ops,
                 "op_costs": {"butterflies_per_key_switch": bfly_per_ks, "ops_per_butterfly": 8,
