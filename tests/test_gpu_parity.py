@@ -480,6 +480,21 @@ CONFIGS = [sg.NTT_F64 | sg.NTT_TILING_ALT, sg.NTT_F64 | sg.NTT_PREFETCH,
            sg.NTT_INT | sg.NTT_TWO_PER_CU]
 
 
+@pytest.mark.parametrize("cfg", [sg.NTT_F64, sg.NTT_F64 | sg.NTT_TWO_PER_CU])
+def test_ntt_fwd_n16384_every_limb(ctx, cfg):
+    """Every one of 640 limbs of a forward transform at 2^14 (more limbs than
+    resident workgroups) equals the oracle's, not only a sample."""
+    log_n, L, count = 14, 4, 160
+    qs, rns = _rns(log_n, L)
+    pids = ctx.primes(qs, log_n)
+    x = rns.uniform(777, count)
+    d = ctx.array(x)
+    ctx.ntt_fwd(d, count, pids, cfg)
+    got = d.download()
+    bad = [c for c in range(count) if not np.array_equal(got[c], rns.ntt_fwd(x[c]))]
+    assert not bad, bad
+
+
 @pytest.mark.parametrize("cfg", CONFIGS)
 @pytest.mark.parametrize("log_n,L,count", [(12, 2, 3), (13, 3, 2), (14, 4, 3), (15, 2, 2)])
 def test_ntt_launch_configs(ctx, cfg, log_n, L, count):
