@@ -330,7 +330,7 @@ __global__ void __launch_bounds__(kT)
                  [&](int p, uint64_t v) {
                    const double sum = f64_mulmod(u52_to_f64(ai[p]), c1, c1q, ari.q) +
                                       f64_mulmod(u52_to_f64(v), c2, c2q, ari.q);
-                   ai[p] = f64_to_u52(f64_canon(f64_reduce(sum, ari.q, ari.qinv), ari.q));
+                   ai[p] = f64_canon_u52(f64_reduce(sum, ari.q, ari.qinv), ari.q);
                  });
         }
         // delta (s0) is rewritten by the next job.
@@ -514,7 +514,7 @@ struct W32ModLimb {
   }
   __device__ __forceinline__ uint64_t combine(uint64_t aw, double z) const {
     const double sum = f64_mulmod(u52_to_f64(aw), c1, c1q, ar->q) + f64_mulmod(z, c2, c2q, ar->q);
-    return f64_to_u52(f64_canon(f64_reduce(sum, ar->q, ar->qinv), ar->q));
+    return f64_canon_u52(f64_reduce(sum, ar->q, ar->qinv), ar->q);
   }
   __device__ __forceinline__ void store(int vs, int k, double lo, double hi, Aux ai) const {
     U64Pair out;

@@ -205,6 +205,15 @@ __device__ __forceinline__ double f64_reduce(double x, double q, double qinv) {
 // Balanced (|x| < q) -> canonical [0, q).
 __device__ __forceinline__ double f64_canon(double x, double q) { return x < 0.0 ? x + q : x; }
 
+// f64_to_u52(f64_canon(x, q)) for integral x in (-q, q) with one addition:
+// the mantissa of 2^52 + x, or of 2^52 + q + x for x < 0 (q + 2^52 is
+// exact for q < 2^52 and loop-invariant).
+__device__ __forceinline__ uint64_t f64_canon_u52(double x, double q) {
+  const double two52 = 4503599627370496.0;
+  const double y = x + (x < 0.0 ? q + two52 : two52);
+  return (uint64_t)__double_as_longlong(y) & 0x000FFFFFFFFFFFFFull;
+}
+
 }  // namespace shg
 
 #endif  // SHG_DEVICE_H_

@@ -237,12 +237,12 @@ struct F64Arith {
   __device__ __forceinline__ void fwd_reduce(V& x) const { x = f64_reduce(x, q, qinv); }
   __device__ __forceinline__ void inv_reduce(V& x) const { x = f64_reduce(x, q, qinv); }
   __device__ __forceinline__ uint64_t fwd_out(V x) const {
-    return f64_to_u52(f64_canon(f64_reduce(x, q, qinv), q));
+    return f64_canon_u52(f64_reduce(x, q, qinv), q);
   }
   __device__ __forceinline__ uint64_t inv_out(V x, int mode) const {
     if (mode == 0) return fwd_out(x);
     const double y = mode == 1 ? f64_mulmod(x, ninv, ninvq, q) : f64_mulmod(x, ninvx, ninvxq, q);
-    return f64_to_u52(f64_canon(f64_reduce(y, q, qinv), q));
+    return f64_canon_u52(f64_reduce(y, q, qinv), q);
   }
 
   // Key-switch accumulation.  The digit value d (not its Montgomery form)
@@ -277,7 +277,7 @@ struct F64Arith {
   }
   __device__ __forceinline__ void acc_reduce(Acc& a) const { a = f64_reduce(a, q, qinv); }
   __device__ __forceinline__ uint64_t acc_out(Acc a) const {
-    return f64_to_u52(f64_canon(f64_reduce(a, q, qinv), q));
+    return f64_canon_u52(f64_reduce(a, q, qinv), q);
   }
   // Reduction schedule (s, i: 0-based global stage after which to reduce).
   // For q < 2^50, f64_mulmod(x, w) with w < q returns |t| <= q/2 +

@@ -149,8 +149,8 @@ __global__ void pir_partial_single_kernel(const uint64_t* __restrict__ ex, int K
   for (int cc = 0; cc < kCB; ++cc) {
     if (c0 + cc >= C) continue;
     const long long base = ((long long)(c0 + cc) * KS + ks) * 2 * poly + off;
-    part[base] = f64_to_u52(f64_canon(f64_reduce(a0[cc], q, qinv), q));
-    part[base + poly] = f64_to_u52(f64_canon(f64_reduce(a1[cc], q, qinv), q));
+    part[base] = f64_canon_u52(f64_reduce(a0[cc], q, qinv), q);
+    part[base + poly] = f64_canon_u52(f64_reduce(a1[cc], q, qinv), q);
   }
 }
 
@@ -244,8 +244,8 @@ __global__ void __launch_bounds__(kTile)
     for (int cc = 0; cc < kCB; ++cc) {
       if (c0 + cc >= C) continue;
       const long long base = (((long long)(g0 + jq) * C + c0 + cc) * KS + ks) * 2 * poly + off;
-      part[base] = f64_to_u52(f64_canon(f64_reduce(a0[jq][cc], q, qinv), q));
-      part[base + poly] = f64_to_u52(f64_canon(f64_reduce(a1[jq][cc], q, qinv), q));
+      part[base] = f64_canon_u52(f64_reduce(a0[jq][cc], q, qinv), q);
+      part[base + poly] = f64_canon_u52(f64_reduce(a1[jq][cc], q, qinv), q);
     }
   }
 }
