@@ -55,6 +55,13 @@ def test_keyswitch_512_ciphertexts_n16384_L8(ctx):
     for c in (0, B // 2 + 3, B - 1):
         e0, e1 = rns.relin_apply(comps[c], kas, kbs, log_bs)
         assert np.array_equal(g0[c], e0) and np.array_equal(g1[c], e1), ("relin", c)
+    # The prepared key (FP64-form words, the bench's timed kernel) gives the
+    # same words for every one of the 512 ciphertexts.
+    key = ctx.ks_key(dka, dkb, k, pids)
+    p0, p1 = ctx.empty((B, L, n)), ctx.empty((B, L, n))
+    ctx.relin_apply_key(dc, 2, 0b111, key, log_bs, p0, p1, B, pids)
+    assert np.array_equal(p0.download(), g0) and np.array_equal(p1.download(), g1)
+    del p0, p1
     # Galois x -> x^5 of (c0, c1): substitution then the key switch.
     gal = ctx.array(np.ascontiguousarray(comps[:, :2]))
     sub = ctx.empty((B, 2, L, n))
